@@ -1,0 +1,889 @@
+// abi.cpp -- host side of the sgfim C ABI (include/sgfim.h).
+//
+// A context owns one HIP stream on one device and every device buffer of one
+// loaded transaction set (or one set of windows). The search is driven in rounds:
+// each round is one replay of a captured hipGraph (or the same kernels launched
+// directly), followed by one small device-to-host copy of the control block and,
+// when MFIs were confirmed, of the new MFI ring entries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sgfim.h"
+#include "sgfim_impl.h"
+
+#define SGFIM_VERSION "sgfim 0.1 (gfx950)"
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct sgfim_result {
+    std::vector<int32_t> items;
+    std::vector<int64_t> off{0};
+    std::vector<int64_t> sup;
+    std::vector<int32_t> round;
+    std::vector<double> time_s;
+    std::vector<int32_t> seg;
+};
+
+struct SearchState {
+    bool active = false;
+    bool done = false;
+    int status = SGFIM_OK;
+    sgfim_mine_args args{};
+    int order_mode = 0;
+    bool use_graph = true;
+    SgSearchDev D{};
+    std::vector<DevBuf> bufs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t* h_ctl = nullptr;        // pinned copy of the control block
+    int64_t ring_read = 0;
+    int64_t rounds = 0;
+    int64_t pair_counts = 0;
+    std::chrono::steady_clock::time_point t0;
+    sgfim_result pending;            // MFIs confirmed but not yet taken
+    sgfim_round_stats stats{};
+    bool cancel = false;
+    // device copies used by the rounds
+    int32_t* d_roots = nullptr;
+    int32_t n_roots = 0;
+};
+
+struct sgfim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t mem_cap = 0;
+    int64_t mem_used = 0;
+    sgfim_opts opts{};
+    // loaded data
+    bool loaded = false;
+    int64_t n_txn = 0;
+    int32_t d = 0, dpad = 0;
+    int64_t W = 0, Wp = 0;
+    std::vector<int32_t> ids;           // dense -> original id
+    std::vector<int64_t> h_sup;         // item supports
+    uint64_t* d_rows = nullptr;
+    int32_t* d_sup = nullptr;
+    int32_t* d_pair = nullptr;
+    bool pair_valid = false;
+    std::vector<int32_t> h_pair;        // host copy of the pair matrix (d x d), filled lazily
+    uint64_t* d_ok = nullptr;
+    SgSegment* d_seg = nullptr;
+    int64_t* d_seg_item_base = nullptr;
+    std::vector<DevBuf> data_bufs;
+    // proposal network
+    bool scorer_set = false;
+    SgScorer scorer{};
+    std::vector<DevBuf> scorer_bufs;
+    // trace
+    bool trace = false;
+    int64_t trace_cap = 0;
+    // search
+    SearchState* S = nullptr;
+    sgfim_round_stats last{};
+};
+
+namespace {
+
+int fail(sgfim_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), SGFIM_E_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                       \
+    } while (0)
+
+int dev_alloc(sgfim_ctx* c, std::vector<DevBuf>& owner, void** p, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (c->mem_cap > 0 && c->mem_used + (int64_t)bytes > c->mem_cap)
+        return fail(c, SGFIM_E_NOMEM, "device memory cap of %lld bytes exceeded (need %zu more, %lld in use)",
+                    (long long)c->mem_cap, bytes, (long long)c->mem_used);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, SGFIM_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    owner.push_back(DevBuf{*p, bytes});
+    c->mem_used += (int64_t)bytes;
+    return SGFIM_OK;
+}
+
+template <typename T>
+int dev_alloc_t(sgfim_ctx* c, std::vector<DevBuf>& owner, T** p, size_t count)
+{
+    void* q = nullptr;
+    int rc = dev_alloc(c, owner, &q, count * sizeof(T));
+    *p = (T*)q;
+    return rc;
+}
+
+void dev_free_all(sgfim_ctx* c, std::vector<DevBuf>& owner)
+{
+    for (auto& b : owner) {
+        if (b.p) (void)hipFree(b.p);
+        c->mem_used -= (int64_t)b.bytes;
+    }
+    owner.clear();
+}
+
+void search_free(sgfim_ctx* c)
+{
+    if (!c->S) return;
+    SearchState* S = c->S;
+    if (S->exec) (void)hipGraphExecDestroy(S->exec);
+    if (S->graph) (void)hipGraphDestroy(S->graph);
+    if (S->h_ctl) (void)hipHostFree(S->h_ctl);
+    dev_free_all(c, S->bufs);
+    delete S;
+    c->S = nullptr;
+}
+
+void data_free(sgfim_ctx* c)
+{
+    search_free(c);
+    dev_free_all(c, c->data_bufs);
+    c->d_rows = nullptr;
+    c->d_sup = nullptr;
+    c->d_pair = nullptr;
+    c->d_ok = nullptr;
+    c->d_seg = nullptr;
+    c->d_seg_item_base = nullptr;
+    c->pair_valid = false;
+    c->h_pair.clear();
+    c->loaded = false;
+}
+
+int32_t pick_chunk(const sgfim_ctx* c, int64_t W)
+{
+    if (c->opts.chunk_words == 256 || c->opts.chunk_words == 512 || c->opts.chunk_words == 1024 ||
+        c->opts.chunk_words == 2048)
+        return c->opts.chunk_words;
+    if (W <= 256) return 256;
+    if (W <= 4096) return 512;
+    return 1024;
+}
+
+int ensure_pair(sgfim_ctx* c)
+{
+    if (c->pair_valid) return SGFIM_OK;
+    if (!c->d_pair) {
+        int rc = dev_alloc_t(c, c->data_bufs, &c->d_pair, (size_t)c->dpad * c->dpad);
+        if (rc) return rc;
+    }
+    HIPCHK(c, sg_launch_pair_mfma(c->d_rows, c->dpad, c->Wp, c->W, c->d_pair, c->stream));
+    c->pair_valid = true;
+    c->h_pair.clear();
+    return SGFIM_OK;
+}
+
+int ensure_h_pair(sgfim_ctx* c)
+{
+    if (!c->h_pair.empty()) return SGFIM_OK;
+    int rc = ensure_pair(c);
+    if (rc) return rc;
+    std::vector<int32_t> full((size_t)c->dpad * c->dpad);
+    HIPCHK(c, hipMemcpyAsync(full.data(), c->d_pair, full.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_pair.resize((size_t)c->d * c->d);
+    for (int i = 0; i < c->d; i++)
+        memcpy(&c->h_pair[(size_t)i * c->d], &full[(size_t)i * c->dpad], (size_t)c->d * 4);
+    return SGFIM_OK;
+}
+
+double now_s(const SearchState* S)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t0).count();
+}
+
+// Copy MFIs [ring_read, pos) from the ring into S->pending.
+int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
+{
+    const int64_t n = pos - S->ring_read;
+    if (n <= 0) return SGFIM_OK;
+    const int64_t cap = S->D.ring_cap;
+    if (n > cap) return fail(c, SGFIM_E_CAPACITY, "MFI ring overflow (%lld new entries, capacity %lld)",
+                             (long long)n, (long long)cap);
+    std::vector<uint16_t> it((size_t)n * SG_MAX_DEPTH);
+    std::vector<int32_t> meta((size_t)n * 4);
+    const int64_t a = S->ring_read % cap;
+    const int64_t first = std::min(n, cap - a);
+    HIPCHK(c, hipMemcpyAsync(it.data(), S->D.ring_items + a * SG_MAX_DEPTH, first * SG_MAX_DEPTH * 2,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(meta.data(), S->D.ring_meta + a * 4, first * 16, hipMemcpyDeviceToHost, c->stream));
+    if (first < n) {
+        HIPCHK(c, hipMemcpyAsync(it.data() + first * SG_MAX_DEPTH, S->D.ring_items, (n - first) * SG_MAX_DEPTH * 2,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(meta.data() + first * 4, S->D.ring_meta, (n - first) * 16,
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sgfim_result& R = S->pending;
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t k = meta[i * 4], sup = meta[i * 4 + 1], seg = meta[i * 4 + 2], rnd = meta[i * 4 + 3];
+        for (int j = 0; j < k; j++) R.items.push_back(c->ids[it[i * SG_MAX_DEPTH + j]]);
+        R.off.push_back((int64_t)R.items.size());
+        R.sup.push_back(sup);
+        R.round.push_back(rnd);
+        R.time_s.push_back(t);
+        R.seg.push_back(seg);
+    }
+    S->ring_read = pos;
+    return SGFIM_OK;
+}
+
+int read_ctl(sgfim_ctx* c, SearchState* S)
+{
+    HIPCHK(c, hipMemcpyAsync(S->h_ctl, S->D.ctl, CTL_SLOTS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SGFIM_OK;
+}
+
+void fill_stats(sgfim_ctx* c, SearchState* S)
+{
+    const int64_t* h = S->h_ctl;
+    sgfim_round_stats& st = S->stats;
+    st.round = S->rounds;
+    st.nodes = h[CTL_NODES];
+    st.counts = h[CTL_COUNTS] + S->pair_counts;
+    st.mfis = h[CTL_RING_POS];
+    st.frontier = h[CTL_FRONTIER];
+    st.words = h[CTL_WORDS];
+    st.elapsed_s = now_s(S);
+    st.done = S->done ? 1 : 0;
+    st.status = S->status;
+    c->last = st;
+}
+
+int check_err(sgfim_ctx* c, SearchState* S)
+{
+    const int64_t e = S->h_ctl[CTL_ERR];
+    if (e & 2) return fail(c, SGFIM_E_DEPTH, "an itemset longer than %d items would be needed", SG_MAX_DEPTH);
+    if (e & 1)
+        return fail(c, SGFIM_E_CAPACITY,
+                    "frontier capacity exhausted (cap %lld nodes, newborn cap %lld); raise opts.frontier_cap",
+                    (long long)S->D.cap, (long long)S->D.newborn_cap);
+    return SGFIM_OK;
+}
+
+int launch_round(sgfim_ctx* c, SearchState* S)
+{
+    const int32_t uniform = 1;
+    if (!S->use_graph) {
+        HIPCHK(c, sg_search_round(S->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup, c->d_seg_item_base,
+                                  uniform, S->order_mode, &c->scorer, c->stream));
+        return SGFIM_OK;
+    }
+    if (!S->exec) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = sg_search_round(S->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup,
+                                       c->d_seg_item_base, uniform, S->order_mode, &c->scorer, c->stream);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &S->graph);
+        if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round capture failed: %s", hipGetErrorString(e));
+        if (e2 != hipSuccess) return fail(c, SGFIM_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+        HIPCHK(c, hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(c, hipGraphLaunch(S->exec, c->stream));
+    return SGFIM_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+const char* sgfim_version(void) { return SGFIM_VERSION; }
+
+const char* sgfim_last_error(const sgfim_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out)
+{
+    if (!out) return SGFIM_E_INVAL;
+    *out = nullptr;
+    sgfim_ctx* c = new sgfim_ctx();
+    if (opts) c->opts = *opts;
+    c->device = c->opts.device;
+    c->mem_cap = c->opts.mem_cap_bytes;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        delete c;
+        return SGFIM_E_DEVICE;
+    }
+    if (c->device < 0 || c->device >= n) {
+        delete c;
+        return SGFIM_E_INVAL;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess || strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete c;
+        return SGFIM_E_DEVICE;
+    }
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SGFIM_E_DEVICE;
+    }
+    *out = c;
+    return SGFIM_OK;
+}
+
+void sgfim_destroy(sgfim_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    data_free(c);
+    dev_free_all(c, c->scorer_bufs);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_t* items, int64_t nnz)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (n_txn < 0 || nnz < 0 || (!off && n_txn > 0) || (!items && nnz > 0))
+        return fail(c, SGFIM_E_INVAL, "bad CSR arguments");
+    if (n_txn >= (int64_t)1 << 31) return fail(c, SGFIM_E_INVAL, "at most 2^31-1 transactions are supported");
+    if (n_txn > 0 && (off[0] != 0 || off[n_txn] != nnz))
+        return fail(c, SGFIM_E_INVAL, "txn_off must start at 0 and end at nnz");
+    for (int64_t t = 0; t < n_txn; t++)
+        if (off[t + 1] < off[t]) return fail(c, SGFIM_E_INVAL, "txn_off is not non-decreasing at row %lld", (long long)t);
+    int32_t idmax = 0;
+    for (int64_t p = 0; p < nnz; p++) {
+        const int32_t x = items[p];
+        if (x < 1) return fail(c, SGFIM_E_INVAL, "item id %d at position %lld is not >= 1", x, (long long)p);
+        if (x > idmax) idmax = x;
+    }
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    data_free(c);
+    std::vector<uint8_t> present((size_t)idmax + 1, 0);
+    for (int64_t p = 0; p < nnz; p++) present[items[p]] = 1;
+    c->ids.clear();
+    std::vector<int32_t> idmap((size_t)idmax + 1, -1);
+    for (int32_t x = 1; x <= idmax; x++)
+        if (present[x]) {
+            idmap[x] = (int32_t)c->ids.size();
+            c->ids.push_back(x);
+        }
+    c->n_txn = n_txn;
+    c->d = (int32_t)c->ids.size();
+    if (c->d > 65535) return fail(c, SGFIM_E_INVAL, "at most 65535 distinct items are supported (got %d)", c->d);
+    c->dpad = std::max(128, (c->d + 127) / 128 * 128);
+    c->W = (n_txn + 63) / 64;
+    c->Wp = std::max<int64_t>(64, (c->W + 63) / 64 * 64);
+    int rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_rows, (size_t)c->dpad * c->Wp))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_sup, (size_t)c->dpad))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_rows, 0, (size_t)c->dpad * c->Wp * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_sup, 0, (size_t)c->dpad * 4, c->stream));
+    if (nnz > 0) {
+        std::vector<DevBuf> tmp;
+        int64_t* d_off = nullptr;
+        int32_t* d_items = nullptr;
+        int32_t* d_map = nullptr;
+        if ((rc = dev_alloc_t(c, tmp, &d_off, (size_t)n_txn + 1)) || (rc = dev_alloc_t(c, tmp, &d_items, (size_t)nnz)) ||
+            (rc = dev_alloc_t(c, tmp, &d_map, (size_t)idmax + 1))) {
+            dev_free_all(c, tmp);
+            return rc;
+        }
+        hipError_t e = hipMemcpyAsync(d_off, off, ((size_t)n_txn + 1) * 8, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_items, items, (size_t)nnz * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_map, idmap.data(), ((size_t)idmax + 1) * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = sg_launch_transpose(d_off, d_items, d_map, n_txn, idmax, c->d_rows, c->d, c->dpad, c->Wp, c->d_sup, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dev_free_all(c, tmp);
+        if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "bitmap build failed: %s", hipGetErrorString(e));
+    }
+    std::vector<int32_t> sup32((size_t)std::max(c->d, 1));
+    if (c->d > 0) {
+        HIPCHK(c, hipMemcpyAsync(sup32.data(), c->d_sup, (size_t)c->d * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->h_sup.assign(sup32.begin(), sup32.begin() + c->d);
+    // single segment table
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg, 2))) return rc;  // [0] search, [1] count_batch
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg_item_base, 1))) return rc;
+    int64_t zero = 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->loaded = true;
+    return SGFIM_OK;
+}
+
+int sgfim_info(const sgfim_ctx* c, int64_t* n_txn, int32_t* n_items, int64_t* words)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (n_txn) *n_txn = c->n_txn;
+    if (n_items) *n_items = c->d;
+    if (words) *words = c->W;
+    return SGFIM_OK;
+}
+
+int sgfim_item_ids(const sgfim_ctx* c, int32_t* ids)
+{
+    if (!c || !ids) return SGFIM_E_INVAL;
+    std::copy(c->ids.begin(), c->ids.end(), ids);
+    return SGFIM_OK;
+}
+
+int sgfim_item_supports(sgfim_ctx* c, int64_t* out)
+{
+    if (!c || !out) return SGFIM_E_INVAL;
+    if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    std::copy(c->h_sup.begin(), c->h_sup.end(), out);
+    return SGFIM_OK;
+}
+
+static int pair_out(sgfim_ctx* c, int32_t* out, bool valu)
+{
+    if (!c || !out) return SGFIM_E_INVAL;
+    if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    (void)hipSetDevice(c->device);
+    int rc;
+    if (valu) {
+        std::vector<DevBuf> tmp;
+        int32_t* d = nullptr;
+        if ((rc = dev_alloc_t(c, tmp, &d, (size_t)c->dpad * c->dpad))) return rc;
+        hipError_t e = sg_launch_pair_valu(c->d_rows, c->dpad, c->Wp, c->W, d, c->stream);
+        std::vector<int32_t> full((size_t)c->dpad * c->dpad);
+        if (e == hipSuccess) e = hipMemcpyAsync(full.data(), d, full.size() * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dev_free_all(c, tmp);
+        if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "VALU pair matrix failed: %s", hipGetErrorString(e));
+        for (int i = 0; i < c->d; i++) memcpy(out + (size_t)i * c->d, &full[(size_t)i * c->dpad], (size_t)c->d * 4);
+        return SGFIM_OK;
+    }
+    c->pair_valid = false;  // recompute: this entry point measures the MFMA path
+    if ((rc = ensure_h_pair(c))) return rc;
+    std::copy(c->h_pair.begin(), c->h_pair.end(), out);
+    return SGFIM_OK;
+}
+
+int sgfim_pair_matrix(sgfim_ctx* c, int32_t* out) { return pair_out(c, out, false); }
+int sgfim_pair_matrix_valu(sgfim_ctx* c, int32_t* out) { return pair_out(c, out, true); }
+
+int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
+                      const int32_t* cand, const int64_t* cand_off, int32_t* out)
+{
+    if (!c || n_prefix < 0 || (n_prefix > 0 && (!prefix_off || !cand_off))) return SGFIM_E_INVAL;
+    if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    if (n_prefix == 0) return SGFIM_OK;
+    if (prefix_off[0] != 0 || cand_off[0] != 0) return fail(c, SGFIM_E_INVAL, "offsets must start at 0");
+    // original id -> dense id
+    auto dense = [&](int32_t id) -> int32_t {
+        auto it = std::lower_bound(c->ids.begin(), c->ids.end(), id);
+        return (it != c->ids.end() && *it == id) ? (int32_t)(it - c->ids.begin()) : -1;
+    };
+    const int32_t cw = pick_chunk(c, c->W);
+    const int64_t nchunk = std::max<int64_t>(1, (c->W + cw - 1) / cw);
+    std::vector<SgCountNode> nodes((size_t)n_prefix);
+    const int64_t ncand_total = cand_off[n_prefix];
+    // candidates that occur in the set, compacted; src[j] = output position of compact j
+    std::vector<int32_t> dc;
+    std::vector<int64_t> src;
+    dc.reserve((size_t)ncand_total);
+    src.reserve((size_t)ncand_total);
+    int64_t units = 0;
+    for (int32_t p = 0; p < n_prefix; p++) {
+        const int64_t a = prefix_off[p], b = prefix_off[p + 1];
+        const int64_t ca = cand_off[p], cb = cand_off[p + 1];
+        if (b < a || cb < ca) return fail(c, SGFIM_E_INVAL, "offsets not non-decreasing at prefix %d", p);
+        if (b - a > SG_MAX_DEPTH) return fail(c, SGFIM_E_DEPTH, "prefix %d has %lld > %d items", p, (long long)(b - a), SG_MAX_DEPTH);
+        SgCountNode& nd = nodes[p];
+        memset(&nd, 0, sizeof(nd));
+        nd.k = (int32_t)(b - a);
+        bool absent = false;  // an absent prefix item makes every count of the prefix 0
+        for (int64_t i = a; i < b; i++) {
+            const int32_t x = dense(prefix_ids[i]);
+            if (x < 0) absent = true;
+            nd.items[i - a] = x < 0 ? 0 : x;
+        }
+        nd.cand_off = (int64_t)dc.size();
+        if (!absent && c->W > 0)
+            for (int64_t j = ca; j < cb; j++) {
+                const int32_t x = dense(cand[j]);
+                if (x < 0) continue;  // an absent candidate counts 0
+                dc.push_back(x);
+                src.push_back(j);
+            }
+        nd.ncand = (int32_t)((int64_t)dc.size() - nd.cand_off);
+        nd.seg = 0;
+        nd.nchunk = nd.ncand > 0 ? (int32_t)nchunk : 0;
+        nd.unit_off = units;
+        units += nd.nchunk;
+    }
+    if (dc.empty()) {
+        std::fill(out, out + ncand_total, 0);
+        return SGFIM_OK;
+    }
+    (void)hipSetDevice(c->device);
+    SgSegment seg{};
+    seg.row_base = 0;
+    seg.stride = c->Wp;
+    seg.words = c->W;
+    seg.d = c->d;
+    seg.dpad = c->dpad;
+    seg.dw = (c->d + 63) / 64;
+    seg.n = (int32_t)c->n_txn;
+    HIPCHK(c, hipMemcpyAsync(c->d_seg + 1, &seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
+    std::vector<DevBuf> tmp;
+    SgCountNode* d_nodes = nullptr;
+    int32_t *d_cand = nullptr, *d_out = nullptr, *d_ctr = nullptr;
+    int rc;
+    if ((rc = dev_alloc_t(c, tmp, &d_nodes, nodes.size())) || (rc = dev_alloc_t(c, tmp, &d_cand, dc.size())) ||
+        (rc = dev_alloc_t(c, tmp, &d_out, dc.size())) || (rc = dev_alloc_t(c, tmp, &d_ctr, 1))) {
+        dev_free_all(c, tmp);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_nodes, nodes.data(), nodes.size() * sizeof(SgCountNode), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_cand, dc.data(), dc.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_out, 0, dc.size() * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_ctr, 0, 4, c->stream);
+    if (e == hipSuccess)
+        e = sg_launch_count_nodes(c->d_rows, c->d_seg + 1, d_nodes, n_prefix, units, d_cand, d_out, cw, d_ctr, 2048, c->stream);
+    std::vector<int32_t> res(dc.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(res.data(), d_out, res.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free_all(c, tmp);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "count batch failed: %s", hipGetErrorString(e));
+    std::fill(out, out + ncand_total, 0);
+    for (size_t j = 0; j < src.size(); j++) out[src[j]] = res[j];
+    return SGFIM_OK;
+}
+
+// ------------------------------------------------------------------ search
+static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
+{
+    if (!c || !a) return SGFIM_E_INVAL;
+    if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    if (a->min_count < 1) return fail(c, SGFIM_E_INVAL, "min_count must be >= 1");
+    if ((a->flags & SGFIM_ORDER_LEARNED) && !c->scorer_set)
+        return fail(c, SGFIM_E_STATE, "SGFIM_ORDER_LEARNED needs sgfim_set_scorer first");
+    if ((a->flags & SGFIM_ROOT_STRICT) && (a->flags & SGFIM_ROOT_TOPK))
+        return fail(c, SGFIM_E_INVAL, "SGFIM_ROOT_STRICT and SGFIM_ROOT_TOPK are exclusive");
+    if (c->d > 4096) return fail(c, SGFIM_E_INVAL, "mining supports at most 4096 distinct items (got %d)", c->d);
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    search_free(c);
+    SearchState* S = new SearchState();
+    c->S = S;
+    S->args = *a;
+    S->order_mode = (a->flags & SGFIM_ORDER_LEARNED) ? 1 : 0;
+    S->use_graph = !(a->flags & SGFIM_NO_GRAPH);
+    S->t0 = std::chrono::steady_clock::now();
+    const int64_t thr = a->min_count;
+    const int32_t d = c->d;
+    int rc;
+    if ((rc = (hipHostMalloc((void**)&S->h_ctl, CTL_SLOTS * sizeof(int64_t), hipHostMallocDefault) == hipSuccess
+                   ? SGFIM_OK
+                   : fail(c, SGFIM_E_NOMEM, "pinned control block allocation failed"))))
+        return rc;
+    memset(S->h_ctl, 0, CTL_SLOTS * sizeof(int64_t));
+    if (thr > (int64_t)INT32_MAX) return fail(c, SGFIM_E_INVAL, "min_count too large");
+    // roots (dense ids): frequent items passing the root rule
+    std::vector<int32_t> order(d);
+    for (int32_t i = 0; i < d; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return c->h_sup[x] > c->h_sup[y]; });
+    std::vector<int32_t> roots;
+    for (int32_t r = 0; r < d; r++) {
+        if (c->h_sup[r] < thr) continue;
+        bool ok = true;
+        if (a->flags & SGFIM_ROOT_STRICT) ok = ((double)c->h_sup[r] / (double)c->n_txn) > a->rel_thr;
+        if (!ok) continue;
+        roots.push_back(r);
+    }
+    if (a->flags & SGFIM_ROOT_TOPK) {
+        roots.clear();
+        for (int32_t i = 0; i < std::min<int32_t>(a->root_topk, d); i++)
+            if (c->h_sup[order[i]] >= thr) roots.push_back(order[i]);
+        std::sort(roots.begin(), roots.end());
+    }
+    if (d == 0 || roots.empty() || c->n_txn == 0) {
+        S->active = true;
+        S->done = true;
+        fill_stats(c, S);
+        return SGFIM_OK;
+    }
+    if ((rc = ensure_h_pair(c))) return rc;
+    // capacity needed by the seed: children of every root
+    int64_t seed_children = 0;
+    for (int32_t r : roots)
+        for (int32_t e = 0; e < d; e++) {
+            const int32_t p = c->h_pair[(size_t)r * d + e];
+            if (e != r && e < r && p >= thr) seed_children++;
+        }
+    S->pair_counts = (int64_t)roots.size() * d;
+    // device segment + pair_ok
+    const int32_t dw = (d + 63) / 64;
+    SgSegment seg{};
+    seg.row_base = 0;
+    seg.stride = c->Wp;
+    seg.words = c->W;
+    seg.pair_base = 0;
+    seg.ok_base = 0;
+    seg.d = d;
+    seg.dpad = c->dpad;
+    seg.dw = dw;
+    seg.thr = (int32_t)thr;
+    seg.n = (int32_t)c->n_txn;
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, &seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
+    if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw))) return rc;
+    HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, 1, d, c->d_ok, c->stream));
+    // search state
+    SgSearchDev& D = S->D;
+    D.bmax = c->opts.batch_nodes > 0 ? c->opts.batch_nodes : 4096;
+    if (D.bmax > 4096) D.bmax = 4096;
+    D.d_max = d;
+    D.dw_max = dw;
+    D.cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap : std::max<int64_t>(1 << 22, 2 * seed_children + D.bmax);
+    if (D.cap > ((int64_t)1 << 31) - 1) D.cap = ((int64_t)1 << 31) - 1;
+    D.newborn_cap = std::max<int64_t>(seed_children, std::min<int64_t>(D.cap, (int64_t)1 << 20));
+    D.ring_cap = c->opts.ring_cap > 0 ? c->opts.ring_cap : std::max<int64_t>(1 << 20, (int64_t)roots.size() + D.bmax);
+    if (D.ring_cap < (int64_t)roots.size() + D.bmax)
+        return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)(roots.size() + D.bmax));
+    if (seed_children > D.cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld level-2 nodes",
+                                           (long long)D.cap, (long long)seed_children);
+    D.chunk_words = pick_chunk(c, c->W);
+    D.count_grid = 2048;
+    D.trace_cap = c->trace ? c->trace_cap : 0;
+    auto& B = S->bufs;
+    if ((rc = dev_alloc_t(c, B, &D.items, (size_t)D.cap * SG_MAX_DEPTH)) || (rc = dev_alloc_t(c, B, &D.meta, (size_t)D.cap)) ||
+        (rc = dev_alloc_t(c, B, &D.cand, (size_t)D.cap * dw)) || (rc = dev_alloc_t(c, B, &D.key, (size_t)D.cap)) ||
+        (rc = dev_alloc_t(c, B, &D.pkey, (size_t)D.cap)) || (rc = dev_alloc_t(c, B, &D.free_idx, (size_t)D.cap)) ||
+        (rc = dev_alloc_t(c, B, &D.bucket, (size_t)D.cap * SG_NBUCKETS)) ||
+        (rc = dev_alloc_t(c, B, &D.batch, (size_t)D.bmax + 1)) || (rc = dev_alloc_t(c, B, &D.coff, (size_t)D.bmax + 1)) ||
+        (rc = dev_alloc_t(c, B, &D.unit_off, (size_t)D.bmax + 1)) ||
+        (rc = dev_alloc_t(c, B, &D.clist, (size_t)D.bmax * d)) || (rc = dev_alloc_t(c, B, &D.ccount, (size_t)D.bmax * d)) ||
+        (rc = dev_alloc_t(c, B, &D.newborn, (size_t)D.newborn_cap)) ||
+        (rc = dev_alloc_t(c, B, &D.feat, (size_t)D.newborn_cap * SG_FEAT)) ||
+        (rc = dev_alloc_t(c, B, &D.scores, (size_t)D.newborn_cap)) ||
+        (rc = dev_alloc_t(c, B, &D.ring_items, (size_t)D.ring_cap * SG_MAX_DEPTH)) ||
+        (rc = dev_alloc_t(c, B, &D.ring_meta, (size_t)D.ring_cap * 4)) || (rc = dev_alloc_t(c, B, &D.ctl, CTL_SLOTS)))
+        return rc;
+    if (D.trace_cap) {
+        if ((rc = dev_alloc_t(c, B, &D.trace_key, (size_t)D.trace_cap * 2)) ||
+            (rc = dev_alloc_t(c, B, &D.trace_feat, (size_t)D.trace_cap * SG_FEAT)) ||
+            (rc = dev_alloc_t(c, B, &D.trace_mfi, (size_t)D.trace_cap)))
+            return rc;
+    }
+    HIPCHK(c, sg_search_init(D, c->stream));
+    std::vector<int32_t> rp(roots.size() * 2);
+    for (size_t i = 0; i < roots.size(); i++) {
+        rp[2 * i] = 0;
+        rp[2 * i + 1] = roots[i];
+    }
+    if ((rc = dev_alloc_t(c, B, &S->d_roots, rp.size()))) return rc;
+    S->n_roots = (int32_t)roots.size();
+    HIPCHK(c, hipMemcpyAsync(S->d_roots, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, sg_search_seed(D, c->d_seg, c->d_pair, c->d_ok, S->d_roots, S->n_roots, c->d_sup, c->d_seg_item_base,
+                             S->order_mode, &c->scorer, c->stream));
+    if ((rc = read_ctl(c, S))) return rc;
+    if ((rc = check_err(c, S))) return rc;
+    if ((rc = drain_ring(c, S, S->h_ctl[CTL_RING_POS], now_s(S)))) return rc;
+    S->active = true;
+    S->done = S->h_ctl[CTL_FRONTIER] == 0;
+    fill_stats(c, S);
+    return SGFIM_OK;
+}
+
+int sgfim_anytime_begin(sgfim_ctx* c, const sgfim_mine_args* a)
+{
+    int rc = search_setup(c, a);
+    if (rc && c && c->S) {
+        c->S->status = rc;
+        c->S->done = true;
+    }
+    return rc;
+}
+
+int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
+{
+    if (!c) return SGFIM_E_INVAL;
+    SearchState* S = c->S;
+    if (!S || !S->active) return fail(c, SGFIM_E_STATE, "no anytime search in progress");
+    (void)hipSetDevice(c->device);
+    int rc = S->status;
+    for (int32_t i = 0; rc == SGFIM_OK && !S->done && (max_rounds <= 0 || i < max_rounds); i++) {
+        if (S->cancel) { rc = SGFIM_E_CANCELLED; break; }
+        if (S->args.deadline_s > 0 && now_s(S) > S->args.deadline_s) {
+            rc = fail(c, SGFIM_E_TIMEOUT, "deadline of %.3f s passed", S->args.deadline_s);
+            break;
+        }
+        if ((rc = launch_round(c, S))) break;
+        if ((rc = read_ctl(c, S))) break;
+        S->rounds++;
+        if ((rc = check_err(c, S))) break;
+        if ((rc = drain_ring(c, S, S->h_ctl[CTL_RING_POS], now_s(S)))) break;
+        if (S->h_ctl[CTL_FRONTIER] == 0) S->done = true;
+    }
+    if (rc != SGFIM_OK) {
+        S->status = rc;
+        S->done = true;
+    }
+    fill_stats(c, S);
+    if (st) *st = S->stats;
+    return rc;
+}
+
+int sgfim_anytime_poll(sgfim_ctx* c, sgfim_round_stats* st)
+{
+    if (!c || !st) return SGFIM_E_INVAL;
+    if (!c->S) return fail(c, SGFIM_E_STATE, "no anytime search");
+    *st = c->S->stats;
+    return SGFIM_OK;
+}
+
+int sgfim_anytime_cancel(sgfim_ctx* c)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (!c->S) return fail(c, SGFIM_E_STATE, "no anytime search");
+    c->S->cancel = true;
+    return SGFIM_OK;
+}
+
+int sgfim_anytime_take(sgfim_ctx* c, sgfim_result** out)
+{
+    if (!c || !out) return SGFIM_E_INVAL;
+    if (!c->S) return fail(c, SGFIM_E_STATE, "no anytime search");
+    sgfim_result* r = new sgfim_result();
+    std::swap(*r, c->S->pending);
+    c->S->pending = sgfim_result();
+    *out = r;
+    return SGFIM_OK;
+}
+
+int sgfim_mine(sgfim_ctx* c, const sgfim_mine_args* a, sgfim_result** out)
+{
+    if (!c || !a || !out) return SGFIM_E_INVAL;
+    *out = nullptr;
+    int rc = sgfim_anytime_begin(c, a);
+    if (rc) return rc;
+    sgfim_round_stats st;
+    rc = sgfim_anytime_step(c, 0, &st);
+    if (rc) return rc;
+    return sgfim_anytime_take(c, out);
+}
+
+int sgfim_result_get(const sgfim_result* r, const int32_t** items, const int64_t** off, const int64_t** support,
+                     const int32_t** round, int64_t* n)
+{
+    if (!r) return SGFIM_E_INVAL;
+    if (items) *items = r->items.data();
+    if (off) *off = r->off.data();
+    if (support) *support = r->sup.data();
+    if (round) *round = r->round.data();
+    if (n) *n = (int64_t)r->sup.size();
+    return SGFIM_OK;
+}
+
+int sgfim_result_times(const sgfim_result* r, const double** t)
+{
+    if (!r || !t) return SGFIM_E_INVAL;
+    *t = r->time_s.data();
+    return SGFIM_OK;
+}
+
+void sgfim_result_free(sgfim_result* r) { delete r; }
+
+int sgfim_last_stats(const sgfim_ctx* c, sgfim_round_stats* st)
+{
+    if (!c || !st) return SGFIM_E_INVAL;
+    *st = c->last;
+    return SGFIM_OK;
+}
+
+int sgfim_set_scorer(sgfim_ctx* c, const float* w1, const float* b1, const float* w2, float b2)
+{
+    if (!c || !w1 || !b1 || !w2) return SGFIM_E_INVAL;
+    (void)hipSetDevice(c->device);
+    dev_free_all(c, c->scorer_bufs);
+    c->scorer_set = false;
+    auto bf16 = [](float f) -> uint16_t {
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (uint16_t)(u >> 16);
+    };
+    std::vector<uint16_t> w1t((size_t)SG_HID * 32, 0), w2b(SG_HID);
+    for (int col = 0; col < SG_HID; col++)
+        for (int k = 0; k < SG_FEAT; k++) w1t[(size_t)col * 32 + k] = bf16(w1[(size_t)k * SG_HID + col]);
+    for (int i = 0; i < SG_HID; i++) w2b[i] = bf16(w2[i]);
+    int rc;
+    SgScorer& s = c->scorer;
+    if ((rc = dev_alloc_t(c, c->scorer_bufs, &s.w1, w1t.size())) || (rc = dev_alloc_t(c, c->scorer_bufs, &s.b1, SG_HID)) ||
+        (rc = dev_alloc_t(c, c->scorer_bufs, &s.w2, SG_HID)) || (rc = dev_alloc_t(c, c->scorer_bufs, &s.b2, 1)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(s.w1, w1t.data(), w1t.size() * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.b1, b1, SG_HID * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.w2, w2b.data(), SG_HID * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.b2, &b2, 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->scorer_set = true;
+    return SGFIM_OK;
+}
+
+int sgfim_set_trace(sgfim_ctx* c, int32_t enable, int64_t cap)
+{
+    if (!c || cap < 0) return SGFIM_E_INVAL;
+    c->trace = enable != 0;
+    c->trace_cap = enable ? (cap > 0 ? cap : (int64_t)1 << 22) : 0;
+    return SGFIM_OK;
+}
+
+// Trace of the last run: births (key, parent key, 24 features) and MFI keys.
+int sgfim_trace_get(sgfim_ctx* c, int64_t* n_birth, int64_t* n_mfi, int64_t* keys, float* feat, int64_t* mfi_keys,
+                    int64_t cap)
+{
+    if (!c || !n_birth || !n_mfi) return SGFIM_E_INVAL;
+    SearchState* S = c->S;
+    if (!S || !S->D.trace_cap) return fail(c, SGFIM_E_STATE, "no traced run");
+    (void)hipSetDevice(c->device);
+    int rc;
+    if ((rc = read_ctl(c, S))) return rc;
+    const int64_t nb = std::min(S->h_ctl[CTL_TRACE_BIRTH], S->D.trace_cap);
+    const int64_t nm = std::min(S->h_ctl[CTL_TRACE_MFI], S->D.trace_cap);
+    *n_birth = nb;
+    *n_mfi = nm;
+    if (S->h_ctl[CTL_TRACE_BIRTH] > S->D.trace_cap || S->h_ctl[CTL_TRACE_MFI] > S->D.trace_cap)
+        return fail(c, SGFIM_E_CAPACITY, "trace capacity %lld exceeded", (long long)S->D.trace_cap);
+    if (!keys && !feat && !mfi_keys) return SGFIM_OK;
+    if (cap < std::max(nb, nm)) return fail(c, SGFIM_E_INVAL, "trace output capacity too small");
+    if (keys) HIPCHK(c, hipMemcpyAsync(keys, S->D.trace_key, nb * 16, hipMemcpyDeviceToHost, c->stream));
+    if (feat) HIPCHK(c, hipMemcpyAsync(feat, S->D.trace_feat, nb * SG_FEAT * 4, hipMemcpyDeviceToHost, c->stream));
+    if (mfi_keys) HIPCHK(c, hipMemcpyAsync(mfi_keys, S->D.trace_mfi, nm * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SGFIM_OK;
+}
+
+}  // extern "C"

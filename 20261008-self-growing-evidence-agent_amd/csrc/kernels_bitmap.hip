@@ -1,0 +1,72 @@
+// CSR transactions -> vertical uint64 tidset bitmaps, with the item-support epilogue.
+//
+// Restates buildBitmap (code/engine_tensorfim/code/engine/src/kernel_bitmap.cu:342-380):
+// bit (t & 63) of word (t >> 6) of row x is set iff transaction t holds item x. The
+// reference builds the bitmap on the host, one transaction at a time; here one
+// workgroup owns a tile of TP_DI items x TP_TW words, ORs the bits of its 64*TP_TW
+// transactions into LDS, and writes each item's TP_TW words as one contiguous
+// 128-byte segment. The popcount of every written word is summed per item with a
+// 16-lane shuffle reduction and one atomic per (item, tile), which yields the item
+// supports of getItemsFrequenc (kernel_bitmap.cu:269-298) for free.
+#include "sgfim_impl.h"
+
+#define TP_TW 16
+#define TP_DI 512
+
+__global__ __launch_bounds__(256) void k_transpose(const int64_t* __restrict__ off,
+                                                   const int32_t* __restrict__ items,
+                                                   const int32_t* __restrict__ idmap,
+                                                   int64_t n_txn, int32_t id_max,
+                                                   uint64_t* __restrict__ rows, int32_t d,
+                                                   int64_t Wp, int32_t* __restrict__ sup)
+{
+    __shared__ unsigned long long tile[TP_DI * TP_TW];
+    const int64_t w0 = (int64_t)blockIdx.x * TP_TW;
+    const int32_t i0 = blockIdx.y * TP_DI;
+    const int32_t di = min(TP_DI, d - i0);
+    for (int q = threadIdx.x; q < TP_DI * TP_TW; q += blockDim.x) tile[q] = 0ull;
+    __syncthreads();
+    const int64_t t0 = w0 * 64;
+    for (int q = threadIdx.x; q < TP_TW * 64; q += blockDim.x) {
+        const int64_t t = t0 + q;
+        if (t >= n_txn) break;
+        const int64_t a = off[t], b = off[t + 1];
+        for (int64_t p = a; p < b; p++) {
+            const int32_t id = items[p];
+            if (id < 1 || id > id_max) continue;
+            const int32_t x = idmap[id] - i0;
+            if ((unsigned)x < (unsigned)di)
+                atomicOr(&tile[x * TP_TW + (q >> 6)], 1ull << (q & 63));
+        }
+    }
+    __syncthreads();
+    // 256 threads = 16 items x 16 words per pass; each group of 16 lanes is one item.
+    for (int base = 0; base < TP_DI; base += 16) {
+        const int x = base + (threadIdx.x >> 4);
+        const int wl = threadIdx.x & 15;
+        uint64_t v = 0;
+        const bool live = x < di;
+        if (live) {
+            v = tile[x * TP_TW + wl];
+            rows[(int64_t)(i0 + x) * Wp + w0 + wl] = v;
+        }
+        uint32_t c = (uint32_t)__popcll(v);
+        c += __shfl_xor(c, 8, 16);
+        c += __shfl_xor(c, 4, 16);
+        c += __shfl_xor(c, 2, 16);
+        c += __shfl_xor(c, 1, 16);
+        if (live && wl == 0 && c) atomicAdd(&sup[i0 + x], (int32_t)c);
+    }
+}
+
+hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const int32_t* idmap,
+                               int64_t n_txn, int32_t id_max, uint64_t* rows, int32_t d,
+                               int32_t dpad, int64_t Wp, int32_t* supports, hipStream_t s)
+{
+    (void)dpad;
+    if (d <= 0 || Wp <= 0) return hipSuccess;
+    dim3 grid((unsigned)(Wp / TP_TW), (unsigned)((d + TP_DI - 1) / TP_DI));
+    hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, off, items, idmap, n_txn, id_max,
+                       rows, d, Wp, supports);
+    return hipGetLastError();
+}

@@ -1,0 +1,180 @@
+// sgfim_impl.h -- internal declarations shared by the host driver and the HIP kernels.
+//
+// Data layout in HBM (one context = one transaction set, or one set of windows):
+//   rows      uint64 [d_pad][Wp]   vertical tidset bitmaps; bit (t & 63) of word (t >> 6)
+//                                  of row x is set iff transaction t holds item x.
+//                                  Wp = W rounded up to 64 words (512 B), pad words and
+//                                  rows d..d_pad-1 are zero.
+//   supports  int32  [d]           row popcounts (item supports)
+//   pair      int32  [d_pad][d_pad] pair co-occurrence matrix (int8 MFMA)
+//   pair_ok   uint64 [d][DW]       bit j of row i set iff pair(i, j) >= min_count
+// A "segment" is one transaction set (one window of the batched-window workload);
+// every kernel addresses rows through the segment table, so one launch can mix
+// windows with different lengths.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SG_MAX_DEPTH 32
+#define SG_NBUCKETS 64
+#define SG_WAVE 64
+
+struct SgSegment {
+    int64_t row_base;   // word offset of row 0 of this segment in the rows arena
+    int64_t stride;     // words per row (Wp of the segment)
+    int64_t words;      // W of the segment (ceil(n/64))
+    int64_t pair_base;  // int32 offset of the segment's pair matrix
+    int64_t ok_base;    // uint64 offset of the segment's pair_ok mask
+    int32_t d;          // items in the segment
+    int32_t dpad;       // rows allocated (multiple of 128)
+    int32_t dw;         // 64-bit words per candidate mask: ceil(d / 64)
+    int32_t thr;        // min_count
+    int32_t n;          // transactions
+    int32_t pad_;
+};
+
+// One frontier node. Items are dense ids of the node's segment.
+struct SgNodeMeta {
+    int32_t k;          // |X|
+    int32_t sup;        // support of X
+    int32_t last;       // dense id of the last item added (the enumeration tail bound)
+    int32_t seg;        // segment id
+    int32_t parent_sup; // support of the parent (0 for roots)
+    int32_t parent_nf;  // frequent extensions of the parent
+    int32_t ncand;      // candidates inherited from the parent
+    float score;        // priority score
+};
+
+// Launchers (kernels_*.hip); all launch on the given stream and return hipError_t.
+hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const int32_t* idmap,
+                               int64_t n_txn, int32_t id_max, uint64_t* rows, int32_t d,
+                               int32_t dpad, int64_t Wp, int32_t* supports, hipStream_t s);
+
+hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
+                               int32_t* pair, hipStream_t s);
+hipError_t sg_launch_pair_valu(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
+                               int32_t* pair, hipStream_t s);
+// Batched pair matrices over segments (one launch for all windows).
+hipError_t sg_launch_pair_mfma_segs(const uint64_t* rows, const SgSegment* segs, int32_t nseg,
+                                    int32_t max_dpad, int64_t max_words, int32_t* pair, hipStream_t s);
+hipError_t sg_launch_pair_ok(const int32_t* pair, const SgSegment* segs, int32_t nseg,
+                             int32_t max_d, uint64_t* ok, hipStream_t s);
+
+// Generic batched AND+popcount over explicit node descriptors (used by sgfim_count_batch).
+struct SgCountNode {
+    int32_t k;          // prefix length
+    int32_t ncand;      // candidates
+    int64_t cand_off;   // offset into cand / out
+    int32_t seg;        // segment
+    int32_t nchunk;     // chunks of this node
+    int64_t unit_off;   // first work unit of this node
+    int32_t items[SG_MAX_DEPTH];
+};
+hipError_t sg_launch_count_nodes(const uint64_t* rows, const SgSegment* segs,
+                                 const SgCountNode* nodes, int32_t n_nodes, int64_t n_units,
+                                 const int32_t* cand, int32_t* out, int32_t chunk_words,
+                                 int32_t* unit_counter, int32_t grid, hipStream_t s);
+
+// Search engine device state (kernels_search.hip).
+struct SgSearchDev {
+    // arena
+    uint16_t* items;     // [cap][SG_MAX_DEPTH]
+    SgNodeMeta* meta;    // [cap]
+    uint64_t* cand;      // [cap][dw_max] candidate masks
+    int64_t* key;        // [cap] node key (unique per run)
+    int64_t* pkey;       // [cap] parent key
+    int32_t* free_idx;   // [cap]
+    int32_t* bucket;     // [SG_NBUCKETS][cap]
+    // batch
+    int32_t* batch;      // [bmax + 1]
+    int32_t* coff;       // [bmax + 1] candidate offsets
+    int64_t* unit_off;   // [bmax + 1] work-unit offsets
+    int32_t* clist;      // [bmax * d_max] candidate ids
+    int32_t* ccount;     // [bmax * d_max] counts
+    // children born in the current round
+    int32_t* newborn;    // [newborn_cap]
+    float* feat;         // [newborn_cap][SG_FEAT]
+    float* scores;       // [newborn_cap]
+    // MFI ring
+    uint16_t* ring_items;   // [ring_cap][SG_MAX_DEPTH]
+    int32_t* ring_meta;     // [ring_cap][4]: k, sup, seg, round
+    // optional search trace (training data for the proposal network)
+    int64_t* trace_key;     // [trace_cap][2]: key, parent key
+    float* trace_feat;      // [trace_cap][SG_FEAT]
+    int64_t* trace_mfi;     // [trace_cap] keys of emitted MFIs
+    int64_t trace_cap;
+    // control block (device int64s, see SgCtl)
+    int64_t* ctl;
+    // parameters
+    int64_t cap;
+    int64_t newborn_cap;
+    int64_t ring_cap;
+    int32_t dw_max;
+    int32_t d_max;
+    int32_t bmax;
+    int32_t chunk_words;
+    int32_t count_grid;
+    int32_t pad_;
+};
+
+// Control block layout (int64 slots of SgSearchDev::ctl).
+enum SgCtl {
+    CTL_FREE_TOP = 0,     // free-list size
+    CTL_BATCH_N,          // nodes in the current batch
+    CTL_UNITS,            // counting work units of the current batch
+    CTL_UNIT_NEXT,        // persistent-kernel work counter
+    CTL_RING_POS,         // MFIs written so far (ring position = pos % ring_cap)
+    CTL_NODES,            // nodes processed (cumulative)
+    CTL_COUNTS,           // support counts computed by AND+popcount (cumulative)
+    CTL_WORDS,            // algorithmic words: (candidates + 1) * W per node (cumulative)
+    CTL_ERR,              // error flags: bit 0 capacity, bit 1 depth
+    CTL_ROUND,            // rounds done
+    CTL_NEWBORN,          // children born this round
+    CTL_FRONTIER,         // nodes waiting in buckets
+    CTL_KEY,              // next node key
+    CTL_TRACE_BIRTH,      // trace records written (births)
+    CTL_TRACE_MFI,        // trace records written (MFIs)
+    CTL_BUCKET_TOP0 = 16, // SG_NBUCKETS slots
+    CTL_SLOTS = 16 + SG_NBUCKETS
+};
+
+struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)
+
+hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s);
+hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int32_t* pair,
+                          const uint64_t* ok, const int32_t* roots, int32_t n_roots,
+                          const int32_t* sup_all, const int64_t* seg_item_base, int32_t order_mode,
+                          const SgScorer* scorer, hipStream_t s);
+hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
+                           const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
+                           const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,
+                           const SgScorer* scorer, hipStream_t s);
+
+// Proposal network (kernels_mlp.hip).
+#define SG_FEAT 24
+#define SG_HID 256
+struct SgScorer {
+    uint16_t* w1;   // bf16 [SG_HID][32] (hidden-major, features padded to 32)
+    float* b1;      // [SG_HID]
+    uint16_t* w2;   // bf16 [SG_HID]
+    float* b2;      // [1]
+};
+// Scores out[i] = sigmoid(MLP(feat[i])) for i < *n_dev (n_dev on the device), i < n_max.
+hipError_t sg_launch_score(const SgScorer* sc, const float* feat, int32_t n_max,
+                           const int64_t* n_dev, float* out, hipStream_t s);
+
+// Training state of the proposal network (kernels_mlp.hip).
+// Master weights (FP32): W1 [SG_FEAT][SG_HID], b1 [SG_HID], w2 [SG_HID], b2 [1]; the
+// packed parameter/gradient/moment order is W1, b1, w2, b2.
+struct SgTrainState {
+    float* W1; float* b1; float* w2; float* b2;   // master weights
+    float* m;  float* v;                          // Adam moments (packed)
+    uint16_t* w1t; uint16_t* w2b;                 // BF16 copies: w1t [SG_HID][32], w2b [SG_HID]
+    float* grad;                                  // packed gradient of the last step
+    float* loss;                                  // [1]
+};
+hipError_t sg_mlp_pack(const SgTrainState& T, hipStream_t s);
+hipError_t sg_mlp_train_step(const SgTrainState& T, const float* X, const float* Y, int32_t n, float lr,
+                             float beta1, float beta2, float eps, int32_t step, hipStream_t s);
+hipError_t sg_mlp_forward(const uint16_t* w1t, const float* b1, const uint16_t* w2b, const float* b2,
+                          const float* X, int32_t n, float* out, int32_t logits, hipStream_t s);

@@ -1,0 +1,227 @@
+"""Python host API over the C ABI: one Context per transaction set on one GPU."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def csr_from_rows(rows: Iterable[Iterable[int]]):
+    """CSR arrays (int64 offsets, int32 ids) from an iterable of transactions."""
+    off = [0]
+    items: list[int] = []
+    for r in rows:
+        items.extend(int(x) for x in r)
+        off.append(len(items))
+    return np.asarray(off, dtype=np.int64), np.asarray(items, dtype=np.int32)
+
+
+@dataclass
+class MineResult:
+    """MFIs as tuples of original ids in search insertion order, with exact supports."""
+    itemsets: list = field(default_factory=list)
+    supports: list = field(default_factory=list)
+    rounds: list = field(default_factory=list)
+    times: list = field(default_factory=list)
+    stats: dict = field(default_factory=dict)
+
+    def as_dict(self) -> dict:
+        """{frozenset(ids): support} -- the adapter's comparison form (engine_adapter.py:103-111)."""
+        return {frozenset(x): s for x, s in zip(self.itemsets, self.supports)}
+
+    def __len__(self):
+        return len(self.itemsets)
+
+
+def _take_result(h: C.c_void_p, stats: dict | None = None) -> MineResult:
+    Lb = L.lib()
+    items = C.c_void_p()
+    off = C.c_void_p()
+    sup = C.c_void_p()
+    rnd = C.c_void_p()
+    n = C.c_int64()
+    tms = C.c_void_p()
+    try:
+        Lb.sgfim_result_get(h, C.byref(items), C.byref(off), C.byref(sup), C.byref(rnd), C.byref(n))
+        Lb.sgfim_result_times(h, C.byref(tms))
+        nn = n.value
+        res = MineResult(stats=stats or {})
+        if nn:
+            o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
+            it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
+            s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
+            r = np.ctypeslib.as_array(C.cast(rnd, C.POINTER(C.c_int32)), shape=(nn,)).copy()
+            t = np.ctypeslib.as_array(C.cast(tms, C.POINTER(C.c_double)), shape=(nn,)).copy()
+            res.itemsets = [tuple(int(x) for x in it[o[i]:o[i + 1]]) for i in range(nn)]
+            res.supports = [int(x) for x in s]
+            res.rounds = [int(x) for x in r]
+            res.times = [float(x) for x in t]
+        return res
+    finally:
+        Lb.sgfim_result_free(h)
+
+
+class Context:
+    """A device context holding one loaded transaction set."""
+
+    def __init__(self, device: int = 0, mem_cap_bytes: int = 0, batch_nodes: int = 0, chunk_words: int = 0,
+                 frontier_cap: int = 0, ring_cap: int = 0):
+        self._L = L.lib()
+        o = L.Opts(device, mem_cap_bytes, batch_nodes, chunk_words, frontier_cap, ring_cap)
+        h = C.c_void_p()
+        rc = self._L.sgfim_create(C.byref(o), C.byref(h))
+        if rc != 0:
+            raise L.SgfimError(rc, f"sgfim_create(device={device}) failed (no gfx950 device?)")
+        self._h = h
+        self.n_txn = 0
+        self.n_items = 0
+        self.words = 0
+
+    # ------------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sgfim_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self._L.sgfim_last_error(self._h)
+            raise L.SgfimError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # ------------------------------------------------------------------ data
+    def load_csr(self, offsets: np.ndarray, items: np.ndarray):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        n = int(off.shape[0]) - 1
+        self._check(self._L.sgfim_load_csr(self._h, _p(off), n, _p(it), int(it.shape[0])), "load_csr")
+        nt = C.c_int64()
+        ni = C.c_int32()
+        w = C.c_int64()
+        self._L.sgfim_info(self._h, C.byref(nt), C.byref(ni), C.byref(w))
+        self.n_txn, self.n_items, self.words = nt.value, ni.value, w.value
+
+    def load_rows(self, rows: Iterable[Iterable[int]]):
+        off, it = csr_from_rows(rows)
+        self.load_csr(off, it)
+
+    def item_ids(self) -> np.ndarray:
+        out = np.zeros(max(self.n_items, 1), dtype=np.int32)
+        self._check(self._L.sgfim_item_ids(self._h, _p(out)), "item_ids")
+        return out[: self.n_items]
+
+    def item_supports(self) -> np.ndarray:
+        out = np.zeros(max(self.n_items, 1), dtype=np.int64)
+        self._check(self._L.sgfim_item_supports(self._h, _p(out)), "item_supports")
+        return out[: self.n_items]
+
+    def pair_matrix(self, valu: bool = False) -> np.ndarray:
+        d = self.n_items
+        out = np.zeros((max(d, 1), max(d, 1)), dtype=np.int32)
+        fn = self._L.sgfim_pair_matrix_valu if valu else self._L.sgfim_pair_matrix
+        self._check(fn(self._h, _p(out)), "pair_matrix")
+        return out[:d, :d]
+
+    def count_batch(self, prefixes: Sequence[Sequence[int]], cands: Sequence[Sequence[int]]) -> list:
+        """counts[p][j] = |T(prefixes[p]) & T(cands[p][j])| (original ids)."""
+        assert len(prefixes) == len(cands)
+        poff = np.zeros(len(prefixes) + 1, dtype=np.int64)
+        coff = np.zeros(len(prefixes) + 1, dtype=np.int64)
+        for i, (p, c) in enumerate(zip(prefixes, cands)):
+            poff[i + 1] = poff[i] + len(p)
+            coff[i + 1] = coff[i] + len(c)
+        pid = np.asarray([x for p in prefixes for x in p] or [0], dtype=np.int32)
+        cid = np.asarray([x for c in cands for x in c] or [0], dtype=np.int32)
+        out = np.zeros(max(int(coff[-1]), 1), dtype=np.int32)
+        self._check(self._L.sgfim_count_batch(self._h, _p(pid), _p(poff), len(prefixes), _p(cid), _p(coff),
+                                              _p(out)), "count_batch")
+        return [out[coff[i]:coff[i + 1]].copy() for i in range(len(prefixes))]
+
+    def count_batch_flat(self, prefix_ids: np.ndarray, prefix_off: np.ndarray, cand: np.ndarray,
+                         cand_off: np.ndarray) -> np.ndarray:
+        pid = np.ascontiguousarray(prefix_ids, dtype=np.int32)
+        poff = np.ascontiguousarray(prefix_off, dtype=np.int64)
+        cid = np.ascontiguousarray(cand, dtype=np.int32)
+        coff = np.ascontiguousarray(cand_off, dtype=np.int64)
+        out = np.zeros(max(int(coff[-1]), 1), dtype=np.int32)
+        self._check(self._L.sgfim_count_batch(self._h, _p(pid), _p(poff), int(poff.shape[0]) - 1, _p(cid),
+                                              _p(coff), _p(out)), "count_batch")
+        return out[: int(coff[-1])]
+
+    # ------------------------------------------------------------------ mining
+    @staticmethod
+    def _args(min_count: int, flags: int = 0, rel_thr: float = 0.0, root_topk: int = 0,
+              deadline_s: float = 0.0) -> L.MineArgs:
+        return L.MineArgs(int(min_count), int(flags), float(rel_thr), int(root_topk), float(deadline_s))
+
+    def mine(self, min_count: int, flags: int = 0, rel_thr: float = 0.0, root_topk: int = 0,
+             deadline_s: float = 0.0) -> MineResult:
+        a = self._args(min_count, flags, rel_thr, root_topk, deadline_s)
+        h = C.c_void_p()
+        self._check(self._L.sgfim_mine(self._h, C.byref(a), C.byref(h)), "mine")
+        return _take_result(h, self.last_stats())
+
+    def last_stats(self) -> dict:
+        st = L.RoundStats()
+        self._L.sgfim_last_stats(self._h, C.byref(st))
+        return st.as_dict()
+
+    def anytime_begin(self, min_count: int, flags: int = 0, deadline_s: float = 0.0):
+        a = self._args(min_count, flags, 0.0, 0, deadline_s)
+        self._check(self._L.sgfim_anytime_begin(self._h, C.byref(a)), "anytime_begin")
+
+    def anytime_step(self, max_rounds: int = 1) -> dict:
+        st = L.RoundStats()
+        self._check(self._L.sgfim_anytime_step(self._h, int(max_rounds), C.byref(st)), "anytime_step")
+        return st.as_dict()
+
+    def anytime_cancel(self):
+        self._check(self._L.sgfim_anytime_cancel(self._h), "anytime_cancel")
+
+    def anytime_take(self) -> MineResult:
+        h = C.c_void_p()
+        self._check(self._L.sgfim_anytime_take(self._h, C.byref(h)), "anytime_take")
+        return _take_result(h, self.last_stats())
+
+    # ------------------------------------------------------------------ proposal network
+    def set_scorer(self, w1: np.ndarray, b1: np.ndarray, w2: np.ndarray, b2: float):
+        w1 = np.ascontiguousarray(w1, dtype=np.float32)
+        b1 = np.ascontiguousarray(b1, dtype=np.float32)
+        w2 = np.ascontiguousarray(w2, dtype=np.float32)
+        assert w1.shape == (24, 256) and b1.shape == (256,) and w2.shape == (256,)
+        self._check(self._L.sgfim_set_scorer(self._h, _p(w1), _p(b1), _p(w2), float(b2)), "set_scorer")
+
+    def set_trace(self, enable: bool, cap: int = 0):
+        self._check(self._L.sgfim_set_trace(self._h, 1 if enable else 0, int(cap)), "set_trace")
+
+    def trace(self):
+        """(keys [n,2] = (key, parent key), features [n,24], mfi_keys [m]) of the last traced run."""
+        nb = C.c_int64()
+        nm = C.c_int64()
+        self._check(self._L.sgfim_trace_get(self._h, C.byref(nb), C.byref(nm), None, None, None, 0), "trace_get")
+        cap = max(nb.value, nm.value, 1)
+        keys = np.zeros((cap, 2), dtype=np.int64)
+        feat = np.zeros((cap, 24), dtype=np.float32)
+        mk = np.zeros(cap, dtype=np.int64)
+        self._check(self._L.sgfim_trace_get(self._h, C.byref(nb), C.byref(nm), _p(keys), _p(feat), _p(mk), cap),
+                    "trace_get")
+        return keys[: nb.value], feat[: nb.value], mk[: nm.value]

@@ -1,0 +1,183 @@
+/*
+ * sgfim.h -- C ABI of the MI355X (gfx950) maximal-frequent-itemset support-counting library.
+ *
+ * The reference drives its mining engines as subprocesses over FIMI files
+ * (code/pipeline/engine_adapter.py:70-131). This ABI replaces that process
+ * boundary with a shared library: integer transactions go in as CSR arrays,
+ * maximal frequent itemsets (MFIs) with exact supports come out. Every entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: SGFIM_OK (0) or a negative
+ *     SGFIM_E_* code; sgfim_last_error() holds a message for the last failure.
+ *   - Item ids are the FIMI ids (>= 1). Negative ids are rejected (the
+ *     reference's parser drops them, kernel_bitmap.cu:225-229; the Python
+ *     adapter never writes them, engine_adapter.py:45-48). Id 0 is rejected
+ *     too: the reference counts it in masks but never as a candidate
+ *     (kernel_bitmap.cu:356-357), which no caller relies on.
+ *   - Counts are exact integers; a transaction set must have fewer than 2^31
+ *     transactions (the reference stores counts in int, kernel_bitmap.cu:412).
+ *   - Caller-owned input arrays are copied before the call returns; results
+ *     are library-owned until sgfim_result_free().
+ *   - One context per host thread; a context owns one HIP stream on one
+ *     device and all of its device memory. Contexts share nothing.
+ */
+#ifndef SGFIM_H
+#define SGFIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sgfim_status {
+    SGFIM_OK = 0,
+    SGFIM_E_INVAL = -1,      /* invalid argument                                  */
+    SGFIM_E_NOMEM = -2,      /* device or host allocation failed / memory cap hit */
+    SGFIM_E_DEVICE = -3,     /* HIP runtime error, or no usable gfx950 device     */
+    SGFIM_E_DEPTH = -4,      /* an itemset longer than SGFIM_MAX_DEPTH would be needed */
+    SGFIM_E_CANCELLED = -5,  /* sgfim_anytime_cancel() stopped the run            */
+    SGFIM_E_TIMEOUT = -6,    /* the deadline passed before the search finished    */
+    SGFIM_E_STATE = -7,      /* call out of order (e.g. mining before loading)    */
+    SGFIM_E_CAPACITY = -8    /* a device table (frontier, MFI ring) is too small  */
+};
+
+#define SGFIM_MAX_DEPTH 32   /* longest itemset; the reference stops at 31 (kernel_bmma.cu:798) */
+
+/* Mining flags (sgfim_mine / sgfim_anytime_begin). */
+#define SGFIM_ROOT_STRICT  0x1u  /* TensorFIM root rule: only items with freq/n > rel_thr root a
+                                    subtree (kernel_bitmap.cu:816,841-850); needs opts.rel_thr */
+#define SGFIM_ROOT_TOPK    0x2u  /* AnyFIM stage rule: only the first root_topk items of the
+                                    stable descending-frequency order root a subtree (kernel.cu:850-859) */
+#define SGFIM_ORDER_LEARNED 0x4u /* order the frontier by the proposal network's score instead
+                                    of by support; never changes the result */
+#define SGFIM_NO_GRAPH     0x8u  /* launch each round's kernels directly instead of replaying
+                                    the round's captured hipGraph */
+
+typedef struct sgfim_ctx sgfim_ctx;
+typedef struct sgfim_result sgfim_result;
+
+typedef struct sgfim_opts {
+    int32_t device;          /* HIP device ordinal                                         */
+    int64_t mem_cap_bytes;   /* device-memory cap for this context; 0 = no cap             */
+    int32_t batch_nodes;     /* frontier nodes per search round; 0 = default               */
+    int32_t chunk_words;     /* 64-bit words per counting work unit; 0 = autotuned/default */
+    int64_t frontier_cap;    /* frontier node capacity; 0 = default                        */
+    int64_t ring_cap;        /* MFI output ring capacity (itemsets); 0 = default           */
+} sgfim_opts;
+
+/* Search flags and thresholds for one mining run. */
+typedef struct sgfim_mine_args {
+    int64_t min_count;       /* an itemset is frequent iff support >= min_count (>= 1)     */
+    uint32_t flags;          /* SGFIM_* flags above                                        */
+    double rel_thr;          /* SGFIM_ROOT_STRICT: relative threshold s                    */
+    int32_t root_topk;       /* SGFIM_ROOT_TOPK: number of roots                           */
+    double deadline_s;       /* wall-clock budget in seconds; 0 = none                     */
+} sgfim_mine_args;
+
+/* Round statistics reported by the anytime driver. */
+typedef struct sgfim_round_stats {
+    int64_t round;           /* rounds completed                                           */
+    int64_t nodes;           /* search nodes processed so far                              */
+    int64_t counts;          /* support counts (itemset, item) computed so far             */
+    int64_t mfis;            /* MFIs confirmed so far                                      */
+    int64_t frontier;        /* nodes waiting in the frontier                              */
+    int64_t words;           /* 64-bit words streamed by the counting kernel so far       */
+    double elapsed_s;        /* wall time since sgfim_anytime_begin                        */
+    int32_t done;            /* 1 when the frontier is empty: every MFI is confirmed       */
+    int32_t status;          /* SGFIM_OK or the error that ended the run                   */
+} sgfim_round_stats;
+
+/* ---- context (replaces the engine process: kernel_bitmap.cu:796 main) ---- */
+int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out);
+void sgfim_destroy(sgfim_ctx* ctx);
+const char* sgfim_last_error(const sgfim_ctx* ctx);
+const char* sgfim_version(void);
+
+/* Load integer transactions and build the vertical tidset bitmaps on the GPU.
+ * Replaces readTransSetFile + getItemsFrequenc + buildBitmap + uploadBitmapToGPU
+ * (kernel_bitmap.cu:205-298, 342-388). txn_off has n_txn + 1 entries; row t is
+ * items[txn_off[t] .. txn_off[t+1]); duplicates inside a row are allowed (a row
+ * is a set). An empty row is an empty transaction and counts in n_txn
+ * (kernel_bitmap.cu:219). */
+int sgfim_load_csr(sgfim_ctx* ctx, const int64_t* txn_off, int64_t n_txn,
+                   const int32_t* items, int64_t nnz);
+
+/* Number of transactions and the distinct item ids of the loaded set
+ * (ascending; the library's dense id of ids[k] is k). */
+int sgfim_info(const sgfim_ctx* ctx, int64_t* n_txn, int32_t* n_items, int64_t* words);
+int sgfim_item_ids(const sgfim_ctx* ctx, int32_t* ids /* n_items */);
+
+/* Item supports (row popcounts), one per loaded item id in sgfim_item_ids order.
+ * Replaces the frequency histogram of getItemsFrequenc (kernel_bitmap.cu:269-298). */
+int sgfim_item_supports(sgfim_ctx* ctx, int64_t* out);
+
+/* Pair co-occurrence matrix over all loaded items, out[i*n + j] = |T_i & T_j|
+ * (n = n_items, dense order), computed with int8 MFMA on 0/1 tiles. Replaces the
+ * first BMMA wave over the single-item roots (kernel_bmma.cu:772-872,1224-1236). */
+int sgfim_pair_matrix(sgfim_ctx* ctx, int32_t* out);
+
+/* Same matrix by the VALU AND+popcount kernel (cross-check path). */
+int sgfim_pair_matrix_valu(sgfim_ctx* ctx, int32_t* out);
+
+/* Batched AND+popcount. For prefix p (item ids prefix_ids[prefix_off[p] ..
+ * prefix_off[p+1])) and each candidate id c in cand[cand_off[p] .. cand_off[p+1]),
+ * out[j] = |T(prefix) & T(c)|. An empty prefix stands for all transactions.
+ * Replaces getResFrequencyCuda (kernel_bitmap.cu:451-536) and getResFrequency
+ * (kernel_bitmap.cu:390-408), batched over many prefixes. */
+int sgfim_count_batch(sgfim_ctx* ctx, const int32_t* prefix_ids, const int64_t* prefix_off,
+                      int32_t n_prefix, const int32_t* cand, const int64_t* cand_off,
+                      int32_t* out);
+
+/* Exact MFI mining at one threshold; blocks until the search ends.
+ * Replaces the engine run CoParaCG_<baseline|bmma>.exe <fimi> <rel_thr>
+ * (kernel_bitmap.cu:796-910, kernel_bmma.cu:1179-1295) and one AnyFIM stage
+ * (AnyTime kernel.cu:844-907). Without root flags the result is the exact MFI
+ * family {X : sup(X) >= min_count, no single-item extension frequent}, the
+ * family code/pipeline/skab_spike.py:84-135 returns. */
+int sgfim_mine(sgfim_ctx* ctx, const sgfim_mine_args* args, sgfim_result** out);
+
+/* Anytime search: begin, advance by rounds, poll, cancel. MFIs are confirmed
+ * final when they are reported. Replaces the AnyFIM stage loop
+ * (AnyTime kernel.cu:844-915) with a threshold-free, priority-ordered frontier
+ * that runs down to args->min_count (the support floor). */
+int sgfim_anytime_begin(sgfim_ctx* ctx, const sgfim_mine_args* args);
+int sgfim_anytime_step(sgfim_ctx* ctx, int32_t max_rounds, sgfim_round_stats* stats);
+int sgfim_anytime_poll(sgfim_ctx* ctx, sgfim_round_stats* stats);
+int sgfim_anytime_cancel(sgfim_ctx* ctx);
+/* Take the MFIs confirmed since the last call (or all, after the run ended). */
+int sgfim_anytime_take(sgfim_ctx* ctx, sgfim_result** out);
+
+/* Result access: itemsets as original ids (the root first, then the order the
+ * search added them, as in the reference's Results lines, kernel_bitmap.cu:681-687),
+ * off has n + 1 entries, support has n entries; round[i] is the anytime round in
+ * which set i was confirmed (0 for sgfim_mine). */
+int sgfim_result_get(const sgfim_result* r, const int32_t** items, const int64_t** off,
+                     const int64_t** support, const int32_t** round, int64_t* n);
+/* Wall time (seconds since the run began) at which each set was confirmed. */
+int sgfim_result_times(const sgfim_result* r, const double** t);
+void sgfim_result_free(sgfim_result* r);
+
+/* Cumulative search statistics of the last sgfim_mine / anytime run. */
+int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
+
+/* Proposal network (24 -> 256 ReLU -> 1): load BF16 weights used by
+ * SGFIM_ORDER_LEARNED. w1 is [24][256] row-major, b1 [256], w2 [256], b2 [1], all
+ * float32 here and rounded to BF16 on the device. No reference counterpart: the
+ * reference has no learned component. */
+int sgfim_set_scorer(sgfim_ctx* ctx, const float* w1, const float* b1, const float* w2, float b2);
+
+/* Search trace for training the proposal network. With tracing on, the next
+ * runs record every node born (its key, its parent's key and its 24 features)
+ * and the key of every confirmed MFI, up to cap records each. sgfim_trace_get
+ * returns the counts, and copies the records when the output arrays are given
+ * (keys: 2 per birth; feat: 24 per birth; cap = capacity of the outputs). */
+int sgfim_set_trace(sgfim_ctx* ctx, int32_t enable, int64_t cap);
+int sgfim_trace_get(sgfim_ctx* ctx, int64_t* n_birth, int64_t* n_mfi, int64_t* keys, float* feat,
+                    int64_t* mfi_keys, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGFIM_H */

@@ -1,0 +1,118 @@
+"""ctypes wrapper of the C oracle (fim_ref.c) -- test infrastructure only.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg import this
+module; the product library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libfim_oracle.so")
+SRC = os.path.join(HERE, "fim_ref.c")
+
+ROOTS_ALL, ROOTS_STRICT, ROOTS_TOPK = 0, 1, 2
+COUNT_ALL, COUNT_INHERIT = 0, 1
+
+
+def build(force: bool = True) -> str:
+    """Compile fim_ref.c with gcc -O3 -fopenmp into libfim_oracle.so."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE, "libfim_oracle.so"], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(force=False)
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        L.ref_bitmap.argtypes = [P, P, C.c_int64, C.c_int32, C.c_int64, P]
+        L.ref_count.argtypes = [P, C.c_int64, P, C.c_int32, P, C.c_int32, P]
+        L.ref_row_supports.argtypes = [P, C.c_int32, C.c_int64, P]
+        L.ref_pair_matrix.argtypes = [P, C.c_int32, C.c_int64, P]
+        L.ref_mine.argtypes = [P, C.c_int32, C.c_int64, C.c_double, C.c_int, C.c_int64,
+                               C.c_double, C.c_int32, C.c_int, C.c_int, C.c_int64, P]
+        L.ref_mine.restype = P
+        L.ref_result_n.argtypes = [P]; L.ref_result_n.restype = C.c_int64
+        L.ref_result_nitems.argtypes = [P]; L.ref_result_nitems.restype = C.c_int64
+        L.ref_result_copy.argtypes = [P, P, P, P]
+        L.ref_result_free.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def bitmap(offsets: np.ndarray, items: np.ndarray, n_items: int) -> np.ndarray:
+    """Vertical bitmap rows[id-1, w] (uint64), W = ceil(n_txn / 64)."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    it = np.ascontiguousarray(items, dtype=np.int32)
+    n = off.shape[0] - 1
+    W = (n + 63) // 64
+    rows = np.zeros((n_items, max(W, 0)), dtype=np.uint64)
+    lib().ref_bitmap(_p(off), _p(it), n, n_items, W, _p(rows))
+    return rows
+
+
+def count(rows: np.ndarray, prefix, cands) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    pre = np.ascontiguousarray(prefix, dtype=np.int32)
+    ca = np.ascontiguousarray(cands, dtype=np.int32)
+    out = np.zeros(ca.shape[0], dtype=np.int64)
+    lib().ref_count(_p(rows), rows.shape[1], _p(pre), pre.shape[0], _p(ca), ca.shape[0], _p(out))
+    return out
+
+
+def row_supports(rows: np.ndarray) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    out = np.zeros(rows.shape[0], dtype=np.int64)
+    lib().ref_row_supports(_p(rows), rows.shape[0], rows.shape[1], _p(out))
+    return out
+
+
+def pair_matrix(rows: np.ndarray) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    d = rows.shape[0]
+    out = np.zeros((d, d), dtype=np.int64)
+    lib().ref_pair_matrix(_p(rows), d, rows.shape[1], _p(out))
+    return out
+
+
+def mine(rows: np.ndarray, thr: float, root_mode: int = ROOTS_ALL, n_total: int = 0,
+         s: float = 0.0, topk: int = 0, count_mode: int = COUNT_INHERIT,
+         threads: int = 0, max_nodes: int = 0):
+    """Run the restated engine search over dense rows.
+
+    Returns (list of (tuple(dense ids in insertion order), support), stats dict).
+    """
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    d, W = rows.shape if rows.ndim == 2 else (0, 0)
+    if threads <= 0:
+        threads = len(os.sched_getaffinity(0))
+    stats = np.zeros(3, dtype=np.int64)
+    L = lib()
+    h = L.ref_mine(_p(rows), d, W, float(thr), root_mode, n_total, float(s), topk,
+                   count_mode, threads, max_nodes, _p(stats))
+    try:
+        n = L.ref_result_n(h)
+        ni = L.ref_result_nitems(h)
+        items = np.zeros(max(ni, 1), dtype=np.int32)
+        off = np.zeros(n + 1, dtype=np.int64)
+        sup = np.zeros(max(n, 1), dtype=np.int64)
+        L.ref_result_copy(h, _p(items), _p(off), _p(sup))
+    finally:
+        L.ref_result_free(h)
+    out = [(tuple(int(x) for x in items[off[i]:off[i + 1]]), int(sup[i])) for i in range(n)]
+    return out, {"nodes": int(stats[0]), "counts": int(stats[1]), "overflow": int(stats[2])}

@@ -1,0 +1,111 @@
+"""CPU tests: the oracle against brute force, closed forms and the reference's semantics."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import cref
+from oracle import mfi_oracle as O
+from sgfim import gen
+
+
+def _rand_txns(seed, n, d, p):
+    rng = random.Random(seed)
+    return [tuple(sorted(i for i in range(1, d + 1) if rng.random() < p)) for _ in range(n)]
+
+
+def _dense(T_rows, n_items):
+    off = np.zeros(len(T_rows) + 1, dtype=np.int64)
+    items = []
+    for i, r in enumerate(T_rows):
+        items.extend(r)
+        off[i + 1] = len(items)
+    return off, np.asarray(items or [1], dtype=np.int32)[: len(items)]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_mine_mfi_equals_brute_force(seed):
+    txns = _rand_txns(seed, 60, 9, 0.45)
+    for thr in (3, 8, 15):
+        got = {frozenset(p): c for p, c in O.mine_mfi(txns, min_count=thr)[0]}
+        assert got == O.brute_force_mfi(txns, thr)
+
+
+def test_threshold_rule():
+    # max(1, ceil(min_sup * n)) and max(1, int(min_count)) (skab_spike.py:99-100)
+    assert O.threshold(1000, 0.02) == 20
+    assert O.threshold(999, 0.02) == 20      # 19.98 -> 20
+    assert O.threshold(10, 0.0) == 1
+    assert O.threshold(10, min_count=0) == 1
+    assert O.threshold(10, min_count=7.9) == 7
+
+
+def test_node_cap_raises():
+    txns = [tuple(range(1, 15))] * 4
+    with pytest.raises(RuntimeError):
+        O.mine_mfi(txns, min_count=1, node_cap=100)
+
+
+def test_subsumption_filter_drops_non_maximal_leaves():
+    # {1,3} is a leaf of the forward DFS (no forward extension) but {1,2,3} contains it.
+    txns = [(1, 2, 3), (1, 2, 3), (1, 3), (2, 3)]
+    got = {frozenset(p): c for p, c in O.mine_mfi(txns, min_count=2)[0]}
+    assert got == {frozenset({1, 2, 3}): 2}
+
+
+def test_empty_and_trivial():
+    assert O.mine_mfi([], min_count=1)[0] == []
+    assert O.mine_mfi([(), ()], min_count=1)[0] == []
+    assert O.mine_mfi([(5,)], min_count=1)[0] == [((5,), 1)]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_c_oracle_matches_python(seed):
+    txns = _rand_txns(100 + seed, 200, 14, 0.35)
+    off, items = _dense(txns, 14)
+    rows = cref.bitmap(off, items, 14)
+    sup = cref.row_supports(rows)
+    for thr in (10, 30, 60):
+        keep = np.where(sup >= thr)[0]
+        for mode in (cref.COUNT_ALL, cref.COUNT_INHERIT):
+            out, _ = cref.mine(rows[keep], thr, count_mode=mode, threads=2)
+            got = {frozenset(int(keep[i]) + 1 for i in x): s for x, s in out}
+            ref = {frozenset(p): c for p, c in O.mine_mfi(txns, min_count=thr)[0]}
+            assert got == ref
+
+
+def test_c_counter_closed_forms():
+    T = gen.workload1(3, 3000)
+    rows = cref.bitmap(T.offsets, T.items, T.n_items)
+    sup = cref.row_supports(rows)
+    # one item per sensor: each sensor's bin rows partition the transactions
+    for s in range(8):
+        assert sup[s * 5:(s + 1) * 5].sum() == T.n_txn
+    P = cref.pair_matrix(rows)
+    assert (P == P.T).all() and (np.diag(P) == sup).all()
+    for s in range(8):
+        blk = P[s * 5:(s + 1) * 5, s * 5:(s + 1) * 5]
+        assert (blk[~np.eye(5, dtype=bool)] == 0).all()
+        # sum over the bins of sensor s of C[i][j] equals C[i][i] for i outside s
+        others = [i for i in range(40) if i // 5 != s]
+        assert (P[others][:, s * 5:(s + 1) * 5].sum(axis=1) == sup[others]).all()
+    # the counter against explicit set arithmetic
+    txs = [set(r) for r in T.rows()]
+    got = cref.count(rows, [0, 7], [12, 20, 33])
+    for j, c in enumerate([12, 20, 33]):
+        assert got[j] == sum(1 for t in txs if {1, 8, c + 1} <= t)
+
+
+def test_engine_root_rules_on_tie_cases():
+    # TensorFIM: an MFI whose largest item has count exactly n*s is not emitted
+    # (root rule freq/n > s, kernel_bitmap.cu:816); at s = 1.0 nothing is emitted
+    # (data/fimi/skab_valve2.txt-1.000000=Results.txt reports MFIsNumber: 0).
+    txns = [(1, 2)] * 4 + [(1,)] * 2 + [(3,)] * 4
+    fam = O.tensorfim_family(txns, 0.4)          # thr = 4; counts 1:6, 2:4, 3:4
+    assert fam == {}                             # {1,2} largest item 2 has 4/10 == 0.4, {3} likewise
+    assert O.tensorfim_family([(1,), (1,)], 1.0) == {}
+    exact = {frozenset(p): c for p, c in O.mine_mfi(txns, min_count=4)[0]}
+    assert exact == {frozenset({1, 2}): 4, frozenset({3}): 4}
+    # AnyFIM: items tied with the k-th count but ranked after k are not roots
+    fam2, thr = O.anyfim_stage_family(txns, 2)   # order 1 (6), 2 (4), 3 (4): roots {1, 2}
+    assert thr == 4 and fam2 == {frozenset({1, 2}): 4}
