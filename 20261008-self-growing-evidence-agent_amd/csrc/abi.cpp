@@ -39,6 +39,19 @@ struct sgfim_result {
     std::vector<int32_t> seg;
 };
 
+// Device buffers of the search, cached per context and reused by later runs that fit.
+struct SearchBufs {
+    SgSearchDev D{};
+    std::vector<DevBuf> bufs;
+    hipGraph_t graph[2] = {nullptr, nullptr};       // one captured round per frontier order
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;        // around the counting kernel (direct launches)
+    int64_t* h_ctl = nullptr;                       // pinned copy of the control block
+    int32_t* d_roots = nullptr;
+    int64_t roots_cap = 0;
+};
+
+// One search run.
 struct SearchState {
     bool active = false;
     bool done = false;
@@ -46,11 +59,6 @@ struct SearchState {
     sgfim_mine_args args{};
     int order_mode = 0;
     bool use_graph = true;
-    SgSearchDev D{};
-    std::vector<DevBuf> bufs;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int64_t* h_ctl = nullptr;        // pinned copy of the control block
     int64_t ring_read = 0;
     int64_t rounds = 0;
     int64_t pair_counts = 0;
@@ -58,9 +66,11 @@ struct SearchState {
     sgfim_result pending;            // MFIs confirmed but not yet taken
     sgfim_round_stats stats{};
     bool cancel = false;
-    // device copies used by the rounds
-    int32_t* d_roots = nullptr;
     int32_t n_roots = 0;
+    double count_ms = 0.0;           // device wall-clock duration of the counting kernel
+    double event_ms = 0.0;           // HIP-event duration (direct launches only)
+    int64_t count_launches = 0;
+    SearchBufs* B = nullptr;         // the context's buffers used by this run
 };
 
 struct sgfim_ctx {
@@ -69,6 +79,7 @@ struct sgfim_ctx {
     std::string err;
     int64_t mem_cap = 0;
     int64_t mem_used = 0;
+    int wall_khz = 0;                   // wall_clock64() rate
     sgfim_opts opts{};
     // loaded data
     bool loaded = false;
@@ -95,6 +106,8 @@ struct sgfim_ctx {
     int64_t trace_cap = 0;
     // search
     SearchState* S = nullptr;
+    SearchBufs* SB = nullptr;
+    int64_t cap_hint = 0;               // frontier capacity learned from an earlier overflow
     sgfim_round_stats last{};
 };
 
@@ -155,19 +168,30 @@ void dev_free_all(sgfim_ctx* c, std::vector<DevBuf>& owner)
 
 void search_free(sgfim_ctx* c)
 {
-    if (!c->S) return;
-    SearchState* S = c->S;
-    if (S->exec) (void)hipGraphExecDestroy(S->exec);
-    if (S->graph) (void)hipGraphDestroy(S->graph);
-    if (S->h_ctl) (void)hipHostFree(S->h_ctl);
-    dev_free_all(c, S->bufs);
-    delete S;
+    delete c->S;
     c->S = nullptr;
+}
+
+void bufs_free(sgfim_ctx* c)
+{
+    SearchBufs* B = c->SB;
+    if (!B) return;
+    for (int i = 0; i < 2; i++) {
+        if (B->exec[i]) (void)hipGraphExecDestroy(B->exec[i]);
+        if (B->graph[i]) (void)hipGraphDestroy(B->graph[i]);
+    }
+    if (B->h_ctl) (void)hipHostFree(B->h_ctl);
+    if (B->ev0) (void)hipEventDestroy(B->ev0);
+    if (B->ev1) (void)hipEventDestroy(B->ev1);
+    dev_free_all(c, B->bufs);
+    delete B;
+    c->SB = nullptr;
 }
 
 void data_free(sgfim_ctx* c)
 {
     search_free(c);
+    bufs_free(c);
     dev_free_all(c, c->data_bufs);
     c->d_rows = nullptr;
     c->d_sup = nullptr;
@@ -227,20 +251,20 @@ int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
 {
     const int64_t n = pos - S->ring_read;
     if (n <= 0) return SGFIM_OK;
-    const int64_t cap = S->D.ring_cap;
+    const int64_t cap = S->B->D.ring_cap;
     if (n > cap) return fail(c, SGFIM_E_CAPACITY, "MFI ring overflow (%lld new entries, capacity %lld)",
                              (long long)n, (long long)cap);
     std::vector<uint16_t> it((size_t)n * SG_MAX_DEPTH);
     std::vector<int32_t> meta((size_t)n * 4);
     const int64_t a = S->ring_read % cap;
     const int64_t first = std::min(n, cap - a);
-    HIPCHK(c, hipMemcpyAsync(it.data(), S->D.ring_items + a * SG_MAX_DEPTH, first * SG_MAX_DEPTH * 2,
+    HIPCHK(c, hipMemcpyAsync(it.data(), S->B->D.ring_items + a * SG_MAX_DEPTH, first * SG_MAX_DEPTH * 2,
                              hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(meta.data(), S->D.ring_meta + a * 4, first * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(meta.data(), S->B->D.ring_meta + a * 4, first * 16, hipMemcpyDeviceToHost, c->stream));
     if (first < n) {
-        HIPCHK(c, hipMemcpyAsync(it.data() + first * SG_MAX_DEPTH, S->D.ring_items, (n - first) * SG_MAX_DEPTH * 2,
+        HIPCHK(c, hipMemcpyAsync(it.data() + first * SG_MAX_DEPTH, S->B->D.ring_items, (n - first) * SG_MAX_DEPTH * 2,
                                  hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(meta.data() + first * 4, S->D.ring_meta, (n - first) * 16,
+        HIPCHK(c, hipMemcpyAsync(meta.data() + first * 4, S->B->D.ring_meta, (n - first) * 16,
                                  hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -260,18 +284,22 @@ int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
 
 int read_ctl(sgfim_ctx* c, SearchState* S)
 {
-    HIPCHK(c, hipMemcpyAsync(S->h_ctl, S->D.ctl, CTL_SLOTS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(S->B->h_ctl, S->B->D.ctl, CTL_SLOTS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SGFIM_OK;
 }
 
 void fill_stats(sgfim_ctx* c, SearchState* S)
 {
-    const int64_t* h = S->h_ctl;
+    const int64_t* h = S->B->h_ctl;
     sgfim_round_stats& st = S->stats;
     st.round = S->rounds;
     st.nodes = h[CTL_NODES];
-    st.counts = h[CTL_COUNTS] + S->pair_counts;
+    st.counts = h[CTL_COUNTS];
+    st.pair_counts = S->pair_counts;
+    st.count_launches = S->count_launches;
+    st.count_ms = S->count_ms;
+    st.event_ms = S->event_ms;
     st.mfis = h[CTL_RING_POS];
     st.frontier = h[CTL_FRONTIER];
     st.words = h[CTL_WORDS];
@@ -283,33 +311,100 @@ void fill_stats(sgfim_ctx* c, SearchState* S)
 
 int check_err(sgfim_ctx* c, SearchState* S)
 {
-    const int64_t e = S->h_ctl[CTL_ERR];
+    const int64_t e = S->B->h_ctl[CTL_ERR];
     if (e & 2) return fail(c, SGFIM_E_DEPTH, "an itemset longer than %d items would be needed", SG_MAX_DEPTH);
     if (e & 1)
         return fail(c, SGFIM_E_CAPACITY,
                     "frontier capacity exhausted (cap %lld nodes, newborn cap %lld); raise opts.frontier_cap",
-                    (long long)S->D.cap, (long long)S->D.newborn_cap);
+                    (long long)S->B->D.cap, (long long)S->B->D.newborn_cap);
     return SGFIM_OK;
 }
 
 int launch_round(sgfim_ctx* c, SearchState* S)
 {
     const int32_t uniform = 1;
+    SearchBufs* B = S->B;
+    (void)hipGetLastError();
     if (!S->use_graph) {
-        HIPCHK(c, sg_search_round(S->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup, c->d_seg_item_base,
-                                  uniform, S->order_mode, &c->scorer, c->stream));
+        HIPCHK(c, sg_search_round(B->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup, c->d_seg_item_base,
+                                  uniform, S->order_mode, &c->scorer, B->ev0, B->ev1, c->stream));
         return SGFIM_OK;
     }
-    if (!S->exec) {
+    const int g = S->order_mode;
+    if (!B->exec[g]) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = sg_search_round(S->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup,
-                                       c->d_seg_item_base, uniform, S->order_mode, &c->scorer, c->stream);
-        hipError_t e2 = hipStreamEndCapture(c->stream, &S->graph);
+        hipError_t e = sg_search_round(B->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup,
+                                       c->d_seg_item_base, uniform, S->order_mode, &c->scorer, nullptr, nullptr,
+                                       c->stream);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &B->graph[g]);
         if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round capture failed: %s", hipGetErrorString(e));
         if (e2 != hipSuccess) return fail(c, SGFIM_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
-        HIPCHK(c, hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphInstantiate(&B->exec[g], B->graph[g], nullptr, nullptr, 0));
     }
-    HIPCHK(c, hipGraphLaunch(S->exec, c->stream));
+    HIPCHK(c, hipGraphLaunch(B->exec[g], c->stream));
+    return SGFIM_OK;
+}
+
+// (Re)use the context's search buffers for a run needing the given sizes.
+int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap, int32_t d, int32_t dw,
+                int32_t bmax, int32_t chunk, int64_t trace_cap, int64_t n_roots)
+{
+    SearchBufs* B = c->SB;
+    if (B) {
+        const SgSearchDev& D = B->D;
+        const bool fits = D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
+                          D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.trace_cap == trace_cap &&
+                          B->roots_cap >= n_roots;
+        if (fits) return SGFIM_OK;
+        bufs_free(c);
+    }
+    B = new SearchBufs();
+    c->SB = B;
+    SgSearchDev& D = B->D;
+    D.cap = cap;
+    D.newborn_cap = newborn_cap;
+    D.ring_cap = ring_cap;
+    D.d_max = d;
+    D.dw_max = dw;
+    D.bmax = bmax;
+    D.chunk_words = chunk;
+    D.count_grid = 2048;
+    D.trace_cap = trace_cap;
+    B->roots_cap = std::max<int64_t>(n_roots, 1);
+    auto& V = B->bufs;
+    int rc;
+    if ((rc = dev_alloc_t(c, V, &D.items, (size_t)cap * SG_MAX_DEPTH)) || (rc = dev_alloc_t(c, V, &D.meta, (size_t)cap)) ||
+        (rc = dev_alloc_t(c, V, &D.cand, (size_t)cap * dw)) || (rc = dev_alloc_t(c, V, &D.key, (size_t)cap)) ||
+        (rc = dev_alloc_t(c, V, &D.pkey, (size_t)cap)) || (rc = dev_alloc_t(c, V, &D.free_idx, (size_t)cap)) ||
+        (rc = dev_alloc_t(c, V, &D.bucket, (size_t)cap * SG_NBUCKETS)) ||
+        (rc = dev_alloc_t(c, V, &D.batch, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.coff, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.unit_off, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.clist, (size_t)bmax * d)) || (rc = dev_alloc_t(c, V, &D.ccount, (size_t)bmax * d)) ||
+        (rc = dev_alloc_t(c, V, &D.newborn, (size_t)newborn_cap)) ||
+        (rc = dev_alloc_t(c, V, &D.feat, (size_t)newborn_cap * SG_FEAT)) ||
+        (rc = dev_alloc_t(c, V, &D.scores, (size_t)newborn_cap)) ||
+        (rc = dev_alloc_t(c, V, &D.ring_items, (size_t)ring_cap * SG_MAX_DEPTH)) ||
+        (rc = dev_alloc_t(c, V, &D.ring_meta, (size_t)ring_cap * 4)) || (rc = dev_alloc_t(c, V, &D.ctl, CTL_SLOTS)) ||
+        (rc = dev_alloc_t(c, V, &B->d_roots, (size_t)B->roots_cap * 2))) {
+        bufs_free(c);
+        return rc;
+    }
+    if (trace_cap) {
+        if ((rc = dev_alloc_t(c, V, &D.trace_key, (size_t)trace_cap * 2)) ||
+            (rc = dev_alloc_t(c, V, &D.trace_feat, (size_t)trace_cap * SG_FEAT)) ||
+            (rc = dev_alloc_t(c, V, &D.trace_mfi, (size_t)trace_cap))) {
+            bufs_free(c);
+            return rc;
+        }
+    }
+    if (hipHostMalloc((void**)&B->h_ctl, CTL_SLOTS * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        bufs_free(c);
+        return fail(c, SGFIM_E_NOMEM, "pinned control block allocation failed");
+    }
+    memset(B->h_ctl, 0, CTL_SLOTS * sizeof(int64_t));
+    HIPCHK(c, hipEventCreate(&B->ev0));
+    HIPCHK(c, hipEventCreate(&B->ev1));
     return SGFIM_OK;
 }
 
@@ -352,6 +447,10 @@ int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out)
         delete c;
         return SGFIM_E_DEVICE;
     }
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess) {
+        (void)hipGetLastError();
+        c->wall_khz = 0;
+    }
     *out = c;
     return SGFIM_OK;
 }
@@ -369,6 +468,7 @@ void sgfim_destroy(sgfim_ctx* c)
 
 int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_t* items, int64_t nnz)
 {
+    (void)hipGetLastError();
     if (!c) return SGFIM_E_INVAL;
     if (n_txn < 0 || nnz < 0 || (!off && n_txn > 0) || (!items && nnz > 0))
         return fail(c, SGFIM_E_INVAL, "bad CSR arguments");
@@ -467,6 +567,7 @@ int sgfim_item_supports(sgfim_ctx* c, int64_t* out)
 
 static int pair_out(sgfim_ctx* c, int32_t* out, bool valu)
 {
+    (void)hipGetLastError();
     if (!c || !out) return SGFIM_E_INVAL;
     if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
     (void)hipSetDevice(c->device);
@@ -496,6 +597,7 @@ int sgfim_pair_matrix_valu(sgfim_ctx* c, int32_t* out) { return pair_out(c, out,
 int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
                       const int32_t* cand, const int64_t* cand_off, int32_t* out)
 {
+    (void)hipGetLastError();
     if (!c || n_prefix < 0 || (n_prefix > 0 && (!prefix_off || !cand_off))) return SGFIM_E_INVAL;
     if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
     if (n_prefix == 0) return SGFIM_OK;
@@ -585,6 +687,7 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
 // ------------------------------------------------------------------ search
 static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
 {
+    (void)hipGetLastError();
     if (!c || !a) return SGFIM_E_INVAL;
     if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
     if (a->min_count < 1) return fail(c, SGFIM_E_INVAL, "min_count must be >= 1");
@@ -598,6 +701,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     search_free(c);
     SearchState* S = new SearchState();
     c->S = S;
+    S->B = c->SB;
     S->args = *a;
     S->order_mode = (a->flags & SGFIM_ORDER_LEARNED) ? 1 : 0;
     S->use_graph = !(a->flags & SGFIM_NO_GRAPH);
@@ -605,11 +709,6 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     const int64_t thr = a->min_count;
     const int32_t d = c->d;
     int rc;
-    if ((rc = (hipHostMalloc((void**)&S->h_ctl, CTL_SLOTS * sizeof(int64_t), hipHostMallocDefault) == hipSuccess
-                   ? SGFIM_OK
-                   : fail(c, SGFIM_E_NOMEM, "pinned control block allocation failed"))))
-        return rc;
-    memset(S->h_ctl, 0, CTL_SLOTS * sizeof(int64_t));
     if (thr > (int64_t)INT32_MAX) return fail(c, SGFIM_E_INVAL, "min_count too large");
     // roots (dense ids): frequent items passing the root rule
     std::vector<int32_t> order(d);
@@ -632,7 +731,9 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     if (d == 0 || roots.empty() || c->n_txn == 0) {
         S->active = true;
         S->done = true;
-        fill_stats(c, S);
+        S->stats = sgfim_round_stats{};
+        S->stats.done = 1;
+        c->last = S->stats;
         return SGFIM_OK;
     }
     if ((rc = ensure_h_pair(c))) return rc;
@@ -660,59 +761,37 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     HIPCHK(c, hipMemcpyAsync(c->d_seg, &seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
     if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw))) return rc;
     HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, 1, d, c->d_ok, c->stream));
-    // search state
-    SgSearchDev& D = S->D;
-    D.bmax = c->opts.batch_nodes > 0 ? c->opts.batch_nodes : 4096;
-    if (D.bmax > 4096) D.bmax = 4096;
-    D.d_max = d;
-    D.dw_max = dw;
-    D.cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap : std::max<int64_t>(1 << 22, 2 * seed_children + D.bmax);
-    if (D.cap > ((int64_t)1 << 31) - 1) D.cap = ((int64_t)1 << 31) - 1;
-    D.newborn_cap = std::max<int64_t>(seed_children, std::min<int64_t>(D.cap, (int64_t)1 << 20));
-    D.ring_cap = c->opts.ring_cap > 0 ? c->opts.ring_cap : std::max<int64_t>(1 << 20, (int64_t)roots.size() + D.bmax);
-    if (D.ring_cap < (int64_t)roots.size() + D.bmax)
-        return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)(roots.size() + D.bmax));
-    if (seed_children > D.cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld level-2 nodes",
-                                           (long long)D.cap, (long long)seed_children);
-    D.chunk_words = pick_chunk(c, c->W);
-    D.count_grid = 2048;
-    D.trace_cap = c->trace ? c->trace_cap : 0;
-    auto& B = S->bufs;
-    if ((rc = dev_alloc_t(c, B, &D.items, (size_t)D.cap * SG_MAX_DEPTH)) || (rc = dev_alloc_t(c, B, &D.meta, (size_t)D.cap)) ||
-        (rc = dev_alloc_t(c, B, &D.cand, (size_t)D.cap * dw)) || (rc = dev_alloc_t(c, B, &D.key, (size_t)D.cap)) ||
-        (rc = dev_alloc_t(c, B, &D.pkey, (size_t)D.cap)) || (rc = dev_alloc_t(c, B, &D.free_idx, (size_t)D.cap)) ||
-        (rc = dev_alloc_t(c, B, &D.bucket, (size_t)D.cap * SG_NBUCKETS)) ||
-        (rc = dev_alloc_t(c, B, &D.batch, (size_t)D.bmax + 1)) || (rc = dev_alloc_t(c, B, &D.coff, (size_t)D.bmax + 1)) ||
-        (rc = dev_alloc_t(c, B, &D.unit_off, (size_t)D.bmax + 1)) ||
-        (rc = dev_alloc_t(c, B, &D.clist, (size_t)D.bmax * d)) || (rc = dev_alloc_t(c, B, &D.ccount, (size_t)D.bmax * d)) ||
-        (rc = dev_alloc_t(c, B, &D.newborn, (size_t)D.newborn_cap)) ||
-        (rc = dev_alloc_t(c, B, &D.feat, (size_t)D.newborn_cap * SG_FEAT)) ||
-        (rc = dev_alloc_t(c, B, &D.scores, (size_t)D.newborn_cap)) ||
-        (rc = dev_alloc_t(c, B, &D.ring_items, (size_t)D.ring_cap * SG_MAX_DEPTH)) ||
-        (rc = dev_alloc_t(c, B, &D.ring_meta, (size_t)D.ring_cap * 4)) || (rc = dev_alloc_t(c, B, &D.ctl, CTL_SLOTS)))
+    // search buffers (reused across runs when they fit)
+    const int32_t bmax = std::min(4096, c->opts.batch_nodes > 0 ? c->opts.batch_nodes : 4096);
+    int64_t cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap
+                                           : std::max<int64_t>({(int64_t)1 << 16, 4 * seed_children + 4 * bmax, c->cap_hint});
+    if (cap > ((int64_t)1 << 31) - 1) cap = ((int64_t)1 << 31) - 1;
+    if (seed_children > cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld level-2 nodes",
+                                         (long long)cap, (long long)seed_children);
+    const int64_t newborn_cap = std::max<int64_t>(seed_children, std::min<int64_t>(cap, (int64_t)1 << 20));
+    int64_t ring_cap = c->opts.ring_cap > 0 ? c->opts.ring_cap : std::max<int64_t>(1 << 16, (int64_t)roots.size() + bmax);
+    if (ring_cap < (int64_t)roots.size() + bmax)
+        return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)(roots.size() + bmax));
+    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, pick_chunk(c, c->W), c->trace ? c->trace_cap : 0,
+                          (int64_t)roots.size())))
         return rc;
-    if (D.trace_cap) {
-        if ((rc = dev_alloc_t(c, B, &D.trace_key, (size_t)D.trace_cap * 2)) ||
-            (rc = dev_alloc_t(c, B, &D.trace_feat, (size_t)D.trace_cap * SG_FEAT)) ||
-            (rc = dev_alloc_t(c, B, &D.trace_mfi, (size_t)D.trace_cap)))
-            return rc;
-    }
+    S->B = c->SB;
+    SgSearchDev& D = S->B->D;
     HIPCHK(c, sg_search_init(D, c->stream));
     std::vector<int32_t> rp(roots.size() * 2);
     for (size_t i = 0; i < roots.size(); i++) {
         rp[2 * i] = 0;
         rp[2 * i + 1] = roots[i];
     }
-    if ((rc = dev_alloc_t(c, B, &S->d_roots, rp.size()))) return rc;
     S->n_roots = (int32_t)roots.size();
-    HIPCHK(c, hipMemcpyAsync(S->d_roots, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, sg_search_seed(D, c->d_seg, c->d_pair, c->d_ok, S->d_roots, S->n_roots, c->d_sup, c->d_seg_item_base,
+    HIPCHK(c, hipMemcpyAsync(S->B->d_roots, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, sg_search_seed(D, c->d_seg, c->d_pair, c->d_ok, S->B->d_roots, S->n_roots, c->d_sup, c->d_seg_item_base,
                              S->order_mode, &c->scorer, c->stream));
     if ((rc = read_ctl(c, S))) return rc;
     if ((rc = check_err(c, S))) return rc;
-    if ((rc = drain_ring(c, S, S->h_ctl[CTL_RING_POS], now_s(S)))) return rc;
+    if ((rc = drain_ring(c, S, S->B->h_ctl[CTL_RING_POS], now_s(S)))) return rc;
     S->active = true;
-    S->done = S->h_ctl[CTL_FRONTIER] == 0;
+    S->done = S->B->h_ctl[CTL_FRONTIER] == 0;
     fill_stats(c, S);
     return SGFIM_OK;
 }
@@ -743,9 +822,21 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
         if ((rc = launch_round(c, S))) break;
         if ((rc = read_ctl(c, S))) break;
         S->rounds++;
+        if (S->B->h_ctl[CTL_UNITS] > 0) {
+            // device-side duration of the counting kernel: first workgroup start to last
+            // workgroup end, from the constant-rate wall clock
+            const int64_t t0 = S->B->h_ctl[CTL_T0], t1 = S->B->h_ctl[CTL_T1];
+            if (t1 > t0 && c->wall_khz > 0) S->count_ms += (double)(t1 - t0) / (double)c->wall_khz;
+            S->count_launches++;
+            if (!S->use_graph) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, S->B->ev0, S->B->ev1) == hipSuccess) S->event_ms += ms;
+                else (void)hipGetLastError();
+            }
+        }
         if ((rc = check_err(c, S))) break;
-        if ((rc = drain_ring(c, S, S->h_ctl[CTL_RING_POS], now_s(S)))) break;
-        if (S->h_ctl[CTL_FRONTIER] == 0) S->done = true;
+        if ((rc = drain_ring(c, S, S->B->h_ctl[CTL_RING_POS], now_s(S)))) break;
+        if (S->B->h_ctl[CTL_FRONTIER] == 0) S->done = true;
     }
     if (rc != SGFIM_OK) {
         S->status = rc;
@@ -787,12 +878,20 @@ int sgfim_mine(sgfim_ctx* c, const sgfim_mine_args* a, sgfim_result** out)
 {
     if (!c || !a || !out) return SGFIM_E_INVAL;
     *out = nullptr;
-    int rc = sgfim_anytime_begin(c, a);
-    if (rc) return rc;
-    sgfim_round_stats st;
-    rc = sgfim_anytime_step(c, 0, &st);
-    if (rc) return rc;
-    return sgfim_anytime_take(c, out);
+    for (;;) {
+        int rc = sgfim_anytime_begin(c, a);
+        if (rc == SGFIM_OK) {
+            sgfim_round_stats st;
+            rc = sgfim_anytime_step(c, 0, &st);
+        }
+        // with the default frontier capacity, an overflow restarts the search with 4x the room
+        if (rc == SGFIM_E_CAPACITY && c->opts.frontier_cap == 0 && c->SB && c->SB->D.cap < ((int64_t)1 << 29)) {
+            c->cap_hint = 4 * c->SB->D.cap;
+            continue;
+        }
+        if (rc) return rc;
+        return sgfim_anytime_take(c, out);
+    }
 }
 
 int sgfim_result_get(const sgfim_result* r, const int32_t** items, const int64_t** off, const int64_t** support,
@@ -867,21 +966,21 @@ int sgfim_trace_get(sgfim_ctx* c, int64_t* n_birth, int64_t* n_mfi, int64_t* key
 {
     if (!c || !n_birth || !n_mfi) return SGFIM_E_INVAL;
     SearchState* S = c->S;
-    if (!S || !S->D.trace_cap) return fail(c, SGFIM_E_STATE, "no traced run");
+    if (!S || !S->B->D.trace_cap) return fail(c, SGFIM_E_STATE, "no traced run");
     (void)hipSetDevice(c->device);
     int rc;
     if ((rc = read_ctl(c, S))) return rc;
-    const int64_t nb = std::min(S->h_ctl[CTL_TRACE_BIRTH], S->D.trace_cap);
-    const int64_t nm = std::min(S->h_ctl[CTL_TRACE_MFI], S->D.trace_cap);
+    const int64_t nb = std::min(S->B->h_ctl[CTL_TRACE_BIRTH], S->B->D.trace_cap);
+    const int64_t nm = std::min(S->B->h_ctl[CTL_TRACE_MFI], S->B->D.trace_cap);
     *n_birth = nb;
     *n_mfi = nm;
-    if (S->h_ctl[CTL_TRACE_BIRTH] > S->D.trace_cap || S->h_ctl[CTL_TRACE_MFI] > S->D.trace_cap)
-        return fail(c, SGFIM_E_CAPACITY, "trace capacity %lld exceeded", (long long)S->D.trace_cap);
+    if (S->B->h_ctl[CTL_TRACE_BIRTH] > S->B->D.trace_cap || S->B->h_ctl[CTL_TRACE_MFI] > S->B->D.trace_cap)
+        return fail(c, SGFIM_E_CAPACITY, "trace capacity %lld exceeded", (long long)S->B->D.trace_cap);
     if (!keys && !feat && !mfi_keys) return SGFIM_OK;
     if (cap < std::max(nb, nm)) return fail(c, SGFIM_E_INVAL, "trace output capacity too small");
-    if (keys) HIPCHK(c, hipMemcpyAsync(keys, S->D.trace_key, nb * 16, hipMemcpyDeviceToHost, c->stream));
-    if (feat) HIPCHK(c, hipMemcpyAsync(feat, S->D.trace_feat, nb * SG_FEAT * 4, hipMemcpyDeviceToHost, c->stream));
-    if (mfi_keys) HIPCHK(c, hipMemcpyAsync(mfi_keys, S->D.trace_mfi, nm * 8, hipMemcpyDeviceToHost, c->stream));
+    if (keys) HIPCHK(c, hipMemcpyAsync(keys, S->B->D.trace_key, nb * 16, hipMemcpyDeviceToHost, c->stream));
+    if (feat) HIPCHK(c, hipMemcpyAsync(feat, S->B->D.trace_feat, nb * SG_FEAT * 4, hipMemcpyDeviceToHost, c->stream));
+    if (mfi_keys) HIPCHK(c, hipMemcpyAsync(mfi_keys, S->B->D.trace_mfi, nm * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SGFIM_OK;
 }

@@ -62,6 +62,25 @@ __device__ __forceinline__ void sg_load_x(const float* __restrict__ X, int64_t n
     a1 = sg_pack8(u1);
 }
 
+// As sg_load_x, with row r of the tile read from X[idx[r]] (idx may be null).
+__device__ __forceinline__ void sg_load_x_idx(const float* __restrict__ X, const int32_t* __restrict__ idx,
+                                              int64_t n, int64_t r0, v8bf& a0, v8bf& a1)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = r0 + (lane & 31);
+    const int h = lane >> 5;
+    const int64_t src = row < n ? (idx ? (int64_t)idx[row] : row) : 0;
+    uint16_t u0[8], u1[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int k0 = 8 * h + j, k1 = 16 + 8 * h + j;
+        u0[j] = row < n ? sg_bf16_bits(X[src * SG_FEAT + k0]) : 0;
+        u1[j] = (row < n && k1 < SG_FEAT) ? sg_bf16_bits(X[src * SG_FEAT + k1]) : 0;
+    }
+    a0 = sg_pack8(u0);
+    a1 = sg_pack8(u1);
+}
+
 // Pre-activation tile nt (hidden units 32nt..32nt+31) of the 32-node tile.
 // w1t is BF16 [SG_HID][32] (hidden-major, K padded with zeros).
 __device__ __forceinline__ v16f sg_hidden_tile(const uint16_t* __restrict__ w1t, int nt, v8bf a0, v8bf a1)
@@ -141,7 +160,6 @@ hipError_t sg_launch_score(const SgScorer* sc, const float* feat, int32_t n_max,
 }
 
 // ------------------------------------------------------------------ training step
-#define SG_NPARAM (SG_FEAT * SG_HID + SG_HID + SG_HID + 1)
 
 // Write BF16 copies of the master weights.
 __global__ void k_mlp_pack(SgTrainState T)
@@ -157,8 +175,9 @@ __global__ void k_mlp_pack(SgTrainState T)
 // One fused step on a minibatch: forward, BCE loss, backward, and (if lr > 0) Adam.
 // One workgroup of 4 waves; each wave walks 32-node tiles; sums are combined in a
 // fixed order.
-__global__ __launch_bounds__(256) void k_mlp_train_step(SgTrainState T, const float* __restrict__ X,
-                                                        const float* __restrict__ Y, int32_t n,
+__global__ __launch_bounds__(256) void k_mlp_train_step(SgTrainState T, const float* __restrict__ Xall,
+                                                        const float* __restrict__ Yall,
+                                                        const int32_t* __restrict__ idx, int32_t n,
                                                         float lr, float beta1, float beta2, float eps,
                                                         int32_t step)
 {
@@ -182,13 +201,14 @@ __global__ __launch_bounds__(256) void k_mlp_train_step(SgTrainState T, const fl
     float gb2 = 0.0f, lsum = 0.0f;
     for (int64_t r0 = (int64_t)wv * 32; r0 < n; r0 += 128) {
         v8bf a0, a1;
-        sg_load_x(X, n, r0, a0, a1);
+        sg_load_x_idx(Xall, idx, n, r0, a0, a1);
         // stage the X tile (BF16) for the transposed operand of dW1
         {
             const int64_t row = r0 + c32;
+            const int64_t src = row < n ? (idx ? (int64_t)idx[row] : row) : 0;
             for (int j = 0; j < 16; j++) {
                 const int k = 16 * h + j;
-                s_x[wv][c32][k] = (row < n && k < SG_FEAT) ? sg_bf16_bits(X[row * SG_FEAT + k]) : 0;
+                s_x[wv][c32][k] = (row < n && k < SG_FEAT) ? sg_bf16_bits(Xall[src * SG_FEAT + k]) : 0;
             }
         }
         float z[16];
@@ -198,7 +218,7 @@ __global__ __launch_bounds__(256) void k_mlp_train_step(SgTrainState T, const fl
         for (int v = 0; v < 16; v++) {
             const int64_t row = r0 + sg_acc_row(v);
             if (row < n) {
-                const float y = Y[row];
+                const float y = Yall[idx ? (int64_t)idx[row] : row];
                 const float s = sg_sigmoid(z[v]);
                 dz[v] = (s - y) * inv_n;
                 // softplus(z) - y z, stable form
@@ -299,10 +319,10 @@ hipError_t sg_mlp_pack(const SgTrainState& T, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t sg_mlp_train_step(const SgTrainState& T, const float* X, const float* Y, int32_t n, float lr,
-                             float beta1, float beta2, float eps, int32_t step, hipStream_t s)
+hipError_t sg_mlp_train_step(const SgTrainState& T, const float* X, const float* Y, const int32_t* idx, int32_t n,
+                             float lr, float beta1, float beta2, float eps, int32_t step, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mlp_train_step, dim3(1), dim3(256), 0, s, T, X, Y, n, lr, beta1, beta2, eps, step);
+    hipLaunchKernelGGL(k_mlp_train_step, dim3(1), dim3(256), 0, s, T, X, Y, idx, n, lr, beta1, beta2, eps, step);
     return hipGetLastError();
 }
 

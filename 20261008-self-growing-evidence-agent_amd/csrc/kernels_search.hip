@@ -361,6 +361,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
         s_budget = min(S.ctl[CTL_FREE_TOP], S.newborn_cap);
         S.ctl[CTL_NEWBORN] = 0;
         S.ctl[CTL_UNIT_NEXT] = 0;
+        S.ctl[CTL_T0] = 0x7fffffffffffffffll;
+        S.ctl[CTL_T1] = 0;
     }
     __syncthreads();
     const int total = s_total;
@@ -467,6 +469,8 @@ __global__ __launch_bounds__(256) void k_count_search(SgSearchDev S, const uint6
     const int nwave = blockDim.x >> 6;
     const int64_t n_units = S.ctl[CTL_UNITS];
     const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
+    if (threadIdx.x == 0 && n_units > 0)
+        atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
     for (;;) {
         if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_UNIT_NEXT], 1ull);
         __syncthreads();
@@ -504,6 +508,10 @@ __global__ __launch_bounds__(256) void k_count_search(SgSearchDev S, const uint6
             if ((threadIdx.x & 63) == 0 && s) atomicAdd(&S.ccount[c0 + j], (int32_t)s);
         }
         __syncthreads();
+    }
+    if (threadIdx.x == 0 && n_units > 0) {
+        __threadfence();
+        atomicMax((unsigned long long*)&S.ctl[CTL_T1], (unsigned long long)wall_clock64());
     }
 }
 
@@ -644,11 +652,12 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
                            const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,
-                           const SgScorer* scorer, hipStream_t s)
+                           const SgScorer* scorer, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s)
 {
     hipError_t e;
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S, segs);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
+    if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
     switch (S.chunk_words) {
     case 256:
         hipLaunchKernelGGL(k_count_search<256>, dim3(S.count_grid), dim3(256), 0, s, S, rows, segs, uniform);
@@ -663,6 +672,7 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
         hipLaunchKernelGGL(k_count_search<1024>, dim3(S.count_grid), dim3(256), 0, s, S, rows, segs, uniform);
         break;
     }
+    if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_expand, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs, pair, ok,
                        sup_all, seg_item_base);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -134,8 +134,10 @@ enum SgCtl {
     CTL_KEY,              // next node key
     CTL_TRACE_BIRTH,      // trace records written (births)
     CTL_TRACE_MFI,        // trace records written (MFIs)
+    CTL_T0,               // earliest wall_clock64() at which a counting workgroup started (this round)
     CTL_BUCKET_TOP0 = 16, // SG_NBUCKETS slots
-    CTL_SLOTS = 16 + SG_NBUCKETS
+    CTL_T1 = 16 + SG_NBUCKETS, // latest wall_clock64() at which a counting workgroup finished
+    CTL_SLOTS = 17 + SG_NBUCKETS
 };
 
 struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)
@@ -148,7 +150,7 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
                            const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,
-                           const SgScorer* scorer, hipStream_t s);
+                           const SgScorer* scorer, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);
 
 // Proposal network (kernels_mlp.hip).
 #define SG_FEAT 24
@@ -174,7 +176,8 @@ struct SgTrainState {
     float* loss;                                  // [1]
 };
 hipError_t sg_mlp_pack(const SgTrainState& T, hipStream_t s);
-hipError_t sg_mlp_train_step(const SgTrainState& T, const float* X, const float* Y, int32_t n, float lr,
-                             float beta1, float beta2, float eps, int32_t step, hipStream_t s);
+hipError_t sg_mlp_train_step(const SgTrainState& T, const float* X, const float* Y, const int32_t* idx, int32_t n,
+                             float lr, float beta1, float beta2, float eps, int32_t step, hipStream_t s);
+#define SG_NPARAM (SG_FEAT * SG_HID + SG_HID + SG_HID + 1)
 hipError_t sg_mlp_forward(const uint16_t* w1t, const float* b1, const uint16_t* w2b, const float* b2,
                           const float* X, int32_t n, float* out, int32_t logits, hipStream_t s);

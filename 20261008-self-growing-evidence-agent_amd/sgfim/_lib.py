@@ -39,7 +39,9 @@ class MineArgs(C.Structure):
 
 class RoundStats(C.Structure):
     _fields_ = [("round", C.c_int64), ("nodes", C.c_int64), ("counts", C.c_int64), ("mfis", C.c_int64),
-                ("frontier", C.c_int64), ("words", C.c_int64), ("elapsed_s", C.c_double),
+                ("frontier", C.c_int64), ("words", C.c_int64), ("pair_counts", C.c_int64),
+                ("count_launches", C.c_int64), ("count_ms", C.c_double), ("event_ms", C.c_double),
+                ("elapsed_s", C.c_double),
                 ("done", C.c_int32), ("status", C.c_int32)]
 
     def as_dict(self):
@@ -74,6 +76,14 @@ _SIGS = {
     "sgfim_set_scorer": (C.c_int, [P, P, P, P, C.c_float]),
     "sgfim_set_trace": (C.c_int, [P, C.c_int32, C.c_int64]),
     "sgfim_trace_get": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), P, P, P, C.c_int64]),
+    "sgfim_mlp_create": (C.c_int, [C.c_int32, P, P, P, C.c_float, PP]),
+    "sgfim_mlp_destroy": (None, [P]),
+    "sgfim_mlp_last_error": (C.c_char_p, [P]),
+    "sgfim_mlp_step": (C.c_int, [P, P, P, C.c_int32, C.c_float, P, P]),
+    "sgfim_mlp_set_data": (C.c_int, [P, P, P, C.c_int64]),
+    "sgfim_mlp_train": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, P]),
+    "sgfim_mlp_get": (C.c_int, [P, P, P, P, P]),
+    "sgfim_mlp_forward": (C.c_int, [P, P, C.c_int32, C.c_int32, P]),
 }
 
 _lib = None

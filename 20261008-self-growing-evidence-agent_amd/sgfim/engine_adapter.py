@@ -1,0 +1,331 @@
+"""Drop-in engine adapter: the reference pipeline's engine interface, backed by the GPU library.
+
+Mirrors code/pipeline/engine_adapter.py (same function names, arguments, return
+shapes and error convention) but runs the MFI search in-process through the
+sgfim C ABI instead of spawning CoParaCG_<engine>.exe / AnytimeMining.exe:
+
+  write_fimi / load_items_map / decode_itemset     engine_adapter.py:37-66
+  run_tensorfim(engine, fimi, rel_thr, reps, timeout) engine_adapter.py:70-100
+  parse_tensorfim_results                          engine_adapter.py:103-111
+  run_anyfim(fimi, upto_stage, timeout)            engine_adapter.py:115-131
+  parse_anyfim_results                             engine_adapter.py:134-166
+  cumulative_mfis / support_confidence_lift        engine_adapter.py:169-191
+
+Both run_* functions also write the engine's Results file next to the dataset,
+in the engine's line grammar (kernel_bitmap.cu:651-690, AnyTime kernel.cu:615-659),
+so scripts that read those files keep working. Engine semantics are kept:
+TensorFIM roots need freq/n > rel_thr (kernel_bitmap.cu:816); an AnyFIM stage k
+mines at the k-th highest item count with the first k items of the stable
+frequency order as roots (AnyTime kernel.cu:844-882). Pass exact=True to get
+the exact MFI family instead (the Python miner's semantics, skab_spike.py:84-135).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import re
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .core import Context
+
+TF_LINE = re.compile(r"^\d+th:\s+\d+\s+\{\s*([^}]*)\}\s+support:\s+\S+\s+frequency:\s+(\d+)")
+TF_COMP = re.compile(r"computation_time\(s\):\s*([\d.eE+-]+)")
+TF_TOT = re.compile(r"total_time\(s\):\s*([\d.eE+-]+)")
+AF_STAGE = re.compile(r"^=+ (\d+)th stage MFIs Number: (\d+) =+")
+
+_CTX = None
+
+
+def _ctx() -> Context:
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(device=int(os.environ.get("SGFIM_DEVICE", "0")))
+    return _CTX
+
+
+# ---------------------------------------------------------------------------- FIMI I/O
+def write_fimi(txns, fimi_path, items_map_path=None):
+    """Encode 'name=state' transactions as FIMI lines of 1-based ids (first-seen order),
+    each line the sorted distinct ids; returns {item_str: id}."""
+    mapping = {}
+    out = []
+    for txn in txns:
+        ids = set()
+        for it in txn:
+            if it not in mapping:
+                mapping[it] = len(mapping) + 1
+            ids.add(mapping[it])
+        out.append(" ".join(str(i) for i in sorted(ids)))
+    with open(fimi_path, "w", encoding="utf-8") as f:
+        f.write("\n".join(out) + "\n")
+    if items_map_path:
+        with open(items_map_path, "w", encoding="utf-8") as f:
+            json.dump(mapping, f, ensure_ascii=False, indent=1)
+    return mapping
+
+
+def load_items_map(path):
+    with open(path, encoding="utf-8") as f:
+        return json.load(f)
+
+
+def decode_itemset(itemset_ids, inv_map):
+    def one(i):
+        return inv_map[i] if i in inv_map else inv_map[str(i)]
+    return frozenset(one(i) for i in itemset_ids)
+
+
+def read_fimi(path):
+    """CSR (int64 offsets, int32 ids) of a FIMI file with the engines' parse rules
+    (kernel_bitmap.cu:205-232): whitespace-separated integers, one transaction per
+    line, a negative id and its digits are dropped, an empty line is an empty
+    transaction, a last line without newline counts only if it holds an item."""
+    data = open(path, "rb").read()
+    off = [0]
+    items: list[int] = []
+    row: list[int] = []
+    i, n = 0, len(data)
+    while i < n:
+        c = data[i]
+        if c == 0x0A:
+            items.extend(row)
+            off.append(len(items))
+            row = []
+            i += 1
+        elif 0x30 <= c <= 0x39:
+            v = 0
+            while i < n and 0x30 <= data[i] <= 0x39:
+                v = v * 10 + (data[i] - 0x30)
+                i += 1
+            row.append(v)
+        elif c == 0x2D:
+            i += 1
+            while i < n and 0x30 <= data[i] <= 0x39:
+                i += 1
+        else:
+            i += 1
+    if row:
+        items.extend(row)
+        off.append(len(items))
+    return np.asarray(off, dtype=np.int64), np.asarray(items, dtype=np.int32)
+
+
+def _csr_without_zero(off, items):
+    """Id 0 is counted in masks but never a candidate in the engines; the ABI takes ids >= 1,
+    so it is dropped here (it can then neither extend nor be reported)."""
+    if items.size and (items == 0).any():
+        keep = items != 0
+        cs = np.concatenate([[0], np.cumsum(keep)])
+        return cs[off], items[keep]
+    return off, items
+
+
+# ---------------------------------------------------------------------------- Results files
+def _fmt(x: float) -> str:
+    """C++ iostream default formatting (6 significant digits, %g rules)."""
+    s = "%.6g" % x
+    return s
+
+
+def _mfi_lines(sets, n_txn):
+    order = sorted(range(len(sets)), key=lambda i: (-sets[i][1], sorted(sets[i][0])))
+    lines = []
+    for r, i in enumerate(order):
+        items, freq = sets[i]
+        ids = "".join(f"{x} " for x in items)
+        lines.append(f"{r + 1}th: {len(items)} {{ {ids}}}   support: {_fmt(freq / n_txn)}  frequency: {freq}")
+    return lines
+
+
+def write_tensorfim_results(path, fimi_path, sets, n_txn, rel_thr, comp_s, pre_s, extra=None):
+    """Results file of CoParaCG_<engine>.exe (kernel_bitmap.cu:651-690)."""
+    bar = "=" * 66
+    with open(path, "w", encoding="utf-8") as f:
+        f.write(bar + "\n")
+        f.write("GPU 0: AMD Instinct MI355X (gfx950)\n")
+        f.write(bar + "\n")
+        f.write(f"transSetFile: {fimi_path}\n")
+        for k, v in (extra or {}).items():
+            f.write(f"{k}: {v}\n")
+        f.write(f"\nsupportThreshold: {_fmt(rel_thr)}\n\n")
+        f.write(bar + "\n")
+        f.write(f"preprocessing_time(s): {_fmt(pre_s)}\n")
+        f.write(f"computation_time(s): {_fmt(comp_s)}\n")
+        f.write(f"total_time(s): {_fmt(pre_s + comp_s)}\n")
+        f.write(f"\nMFIsNumber: {len(sets)}\n")
+        f.write("============================List of MFIs:=============================\n")
+        for ln in _mfi_lines(sets, n_txn):
+            f.write(ln + "\n")
+        f.write(bar + "\n")
+
+
+def write_anyfim_results(path, fimi_path, stages, thresholds, runtimes, n_txn, pre_s):
+    """Staged Results file of AnytimeMining.exe (AnyTime kernel.cu:615-659)."""
+    bar = "=" * 65
+    K = len(stages)
+    with open(path, "w", encoding="utf-8") as f:
+        f.write(bar + "\n")
+        f.write("GPU 0: AMD Instinct MI355X (gfx950)\n")
+        f.write(bar + "\n")
+        f.write(f"transSetFile: {fimi_path}\n")
+        f.write(f"transNum: {n_txn}\n")
+        f.write(f"\nUptoStage: {K}\n")
+        f.write(f"preprocessing_time(s): {_fmt(pre_s)}\n")
+        f.write(bar + "\n")
+        f.write("runtimePerStage(s):" + "".join(f" {_fmt(v)}," for v in runtimes) + "\n")
+        f.write("\nfrequencyThrePerStage:" + "".join(f" {t}," for t in thresholds))
+        f.write("\nsupportThrePerStage:" + "".join(f" {_fmt(t / n_txn)}," for t in thresholds) + "\n")
+        f.write("\ndimensionPerStage:" + "".join(f" {k}," for k in range(1, K + 1)) + "\n")
+        nums = [len(stages[k]) for k in range(1, K + 1)]
+        its = [sum(len(x) for x, _ in stages[k]) for k in range(1, K + 1)]
+        f.write("\nMFIsNumPerStage:" + "".join(f" {v}," for v in nums))
+        f.write("\nitemsNumPerStage:" + "".join(f" {v}," for v in its) + "\n")
+        f.write("\nMFIsNumPerStage_withRemoveDuplicate:" + "".join(f" {v}," for v in nums))
+        f.write("\nitemsNumPerStage_withRemoveDuplicate:" + "".join(f" {v}," for v in its) + "\n")
+        for k in range(1, K + 1):
+            f.write(f"====================== {k}th stage MFIs Number: {len(stages[k])} ======================\n")
+            for ln in _mfi_lines(stages[k], n_txn):
+                f.write(ln + "\n")
+            f.write(f"\nfrequencyThreshold: {thresholds[k - 1]}\n")
+        f.write(bar + "\n")
+
+
+def parse_tensorfim_results(path):
+    """{frozenset(item ids): frequency} of a TensorFIM Results file."""
+    out = {}
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for ln in f:
+            m = TF_LINE.match(ln.strip())
+            if m:
+                out[frozenset(int(x) for x in m.group(1).split())] = int(m.group(2))
+    return out
+
+
+def parse_anyfim_results(path):
+    """(stages, meta): stages[k] = [(frozenset(ids), frequency)], meta = per-stage lists."""
+    stages, meta, cur = {}, {}, None
+    keys = (("runtimePerStage(s):", float), ("supportThrePerStage:", float), ("frequencyThrePerStage:", float),
+            ("MFIsNumPerStage:", int), ("itemsNumPerStage:", int), ("dimensionPerStage:", int),
+            ("preprocessing_time(s):", float))
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for ln in f:
+            ln = ln.strip()
+            m = AF_STAGE.match(ln)
+            if m:
+                cur = int(m.group(1))
+                stages[cur] = []
+                continue
+            for key, cast in keys:
+                if ln.startswith(key):
+                    body = ln[len(key):].strip()
+                    meta[key.rstrip(":")] = [cast(x.rstrip(",")) for x in body.split() if x.strip(",")]
+            if cur is not None:
+                m = TF_LINE.match(ln)
+                if m:
+                    stages[cur].append((frozenset(int(x) for x in m.group(1).split()), int(m.group(2))))
+    return stages, meta
+
+
+# ---------------------------------------------------------------------------- engines
+def run_tensorfim(engine, fimi_path, rel_thr, reps=1, timeout=1800, exact=False):
+    """engine: 'baseline' | 'bmma' (both map to the same exact GPU search). Returns
+    {'ok', 'mfis': {frozenset: freq}, 'comp_times': [...], 'tot_times': [...]} and writes
+    <fimi>-<rel_thr:.6f>=Results.txt."""
+    if engine not in ("baseline", "bmma"):
+        raise ValueError(f"unknown engine {engine!r}")
+    rp = f"{fimi_path}-{rel_thr:.6f}=Results.txt"
+    comp, tot = [], []
+    ok, mfis = True, {}
+    t_pre0 = time.perf_counter()
+    off, items = _csr_without_zero(*read_fimi(fimi_path))
+    ctx = _ctx()
+    ctx.load_csr(off, items)
+    pre = time.perf_counter() - t_pre0
+    n = ctx.n_txn
+    thr = max(1, math.ceil(n * rel_thr)) if n else 1
+    for _ in range(reps):
+        if os.path.exists(rp):
+            os.remove(rp)
+        t0 = time.perf_counter()
+        try:
+            res = ctx.mine(thr, flags=0 if exact else L.ROOT_STRICT, rel_thr=rel_thr, deadline_s=float(timeout))
+        except L.SgfimError as e:
+            if e.code == L.E_TIMEOUT:
+                ok = False
+                continue
+            raise
+        dt = time.perf_counter() - t0
+        comp.append(dt)
+        tot.append(dt + pre)
+        mfis = res.as_dict()
+        write_tensorfim_results(rp, fimi_path, list(zip(res.itemsets, res.supports)), n, rel_thr, dt, pre,
+                                {"transNum": n, "dimension": ctx.n_items})
+    return {"ok": ok, "mfis": mfis, "comp_times": comp, "tot_times": tot}
+
+
+def anyfim_thresholds(supports: np.ndarray, upto_stage: int):
+    """Stage thresholds: the k-th highest item count, k = 1..upto_stage (kernel.cu:846)."""
+    return [int(x) for x in np.sort(supports)[::-1][:upto_stage]]
+
+
+def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False):
+    """Run stages 1..upto_stage; returns {'ok', 'stages': {k: [(frozenset, freq)]}, 'meta', 'results_path'}
+    and writes <fimi>-<upto_stage>-stages=Results.txt."""
+    rp = f"{fimi_path}-{upto_stage}-stages=Results.txt"
+    if os.path.exists(rp):
+        os.remove(rp)
+    t_pre0 = time.perf_counter()
+    off, items = _csr_without_zero(*read_fimi(fimi_path))
+    ctx = _ctx()
+    ctx.load_csr(off, items)
+    pre = time.perf_counter() - t_pre0
+    if upto_stage > ctx.n_items:
+        # the engine refuses (kernel.cu:801-804) and writes no file
+        return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+    sup = ctx.item_supports()
+    thrs = anyfim_thresholds(sup, upto_stage)
+    stages, runtimes = {}, []
+    t_start = time.perf_counter()
+    for k in range(1, upto_stage + 1):
+        left = timeout - (time.perf_counter() - t_start)
+        if left <= 0:
+            return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+        t0 = time.perf_counter()
+        try:
+            res = ctx.mine(thrs[k - 1], flags=0 if exact else L.ROOT_TOPK, root_topk=k, deadline_s=left)
+        except L.SgfimError as e:
+            if e.code == L.E_TIMEOUT:
+                return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+            raise
+        runtimes.append(time.perf_counter() - t0)
+        stages[k] = list(zip(res.itemsets, res.supports))
+    write_anyfim_results(rp, fimi_path, stages, thrs, runtimes, ctx.n_txn, pre)
+    st, meta = parse_anyfim_results(rp)
+    return {"ok": True, "stages": st, "meta": meta, "results_path": rp}
+
+
+def cumulative_mfis(stages):
+    """{k: set of every itemset seen in stages <= k}."""
+    cum, seen = {}, set()
+    for k in sorted(stages):
+        for itemset, _ in stages[k]:
+            seen.add(itemset)
+        cum[k] = set(seen)
+    return cum
+
+
+def support_confidence_lift(itemset, txns, labels):
+    """(support, P(X | label 1), lift = P(X | 1) / P(X)) over integer transactions."""
+    n = len(txns)
+    s = set(itemset)
+    cnt = sum(1 for t in txns if s <= set(t))
+    pos = sum(1 for t, lab in zip(txns, labels) if lab == 1 and s <= set(t))
+    npos = sum(1 for lab in labels if lab == 1)
+    sup = cnt / n if n else 0.0
+    conf = pos / npos if npos else 0.0
+    lift = (pos / npos) / (cnt / n) if cnt and npos else 0.0
+    return sup, conf, lift

@@ -80,10 +80,17 @@ typedef struct sgfim_mine_args {
 typedef struct sgfim_round_stats {
     int64_t round;           /* rounds completed                                           */
     int64_t nodes;           /* search nodes processed so far                              */
-    int64_t counts;          /* support counts (itemset, item) computed so far             */
+    int64_t counts;          /* support counts (itemset, item) computed by AND+popcount    */
     int64_t mfis;            /* MFIs confirmed so far                                      */
     int64_t frontier;        /* nodes waiting in the frontier                              */
-    int64_t words;           /* 64-bit words streamed by the counting kernel so far       */
+    int64_t words;           /* algorithmic 64-bit words of the counting kernel so far:
+                                (candidates + 1) * W per processed node                    */
+    int64_t pair_counts;     /* counts taken from the pair matrix (roots x items)          */
+    int64_t count_launches;  /* launches of the counting kernel                            */
+    double count_ms;         /* summed duration of the counting kernel, from the device's
+                                constant-rate wall clock (first workgroup start to last end) */
+    double event_ms;         /* the same duration from HIP events around the kernel; only
+                                with SGFIM_NO_GRAPH (0 otherwise)                          */
     double elapsed_s;        /* wall time since sgfim_anytime_begin                        */
     int32_t done;            /* 1 when the frontier is empty: every MFI is confirmed       */
     int32_t status;          /* SGFIM_OK or the error that ended the run                   */
@@ -176,6 +183,26 @@ int sgfim_set_scorer(sgfim_ctx* ctx, const float* w1, const float* b1, const flo
 int sgfim_set_trace(sgfim_ctx* ctx, int32_t enable, int64_t cap);
 int sgfim_trace_get(sgfim_ctx* ctx, int64_t* n_birth, int64_t* n_mfi, int64_t* keys, float* feat,
                     int64_t* mfi_keys, int64_t cap);
+
+/* ---- proposal-network trainer (no reference counterpart) ----
+ * FP32 master weights and Adam state on one device; forward and backward run as
+ * BF16-MFMA HIP kernels with FP32 accumulation. Parameter order of the packed
+ * gradient: W1 [24][256] row-major, b1 [256], w2 [256], b2. */
+typedef struct sgfim_mlp sgfim_mlp;
+int sgfim_mlp_create(int32_t device, const float* W1, const float* b1, const float* w2, float b2,
+                     sgfim_mlp** out);
+void sgfim_mlp_destroy(sgfim_mlp* m);
+const char* sgfim_mlp_last_error(const sgfim_mlp* m);
+/* One step on host rows X [n][24], labels Y [n] (0/1): mean BCE loss and packed gradient
+ * out (either may be null); an Adam update (beta 0.9/0.999, eps 1e-8) when lr > 0. */
+int sgfim_mlp_step(sgfim_mlp* m, const float* X, const float* Y, int32_t n, float lr, float* loss,
+                   float* grad);
+/* Resident training set and minibatch Adam epochs over it. */
+int sgfim_mlp_set_data(sgfim_mlp* m, const float* X, const float* Y, int64_t n);
+int sgfim_mlp_train(sgfim_mlp* m, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* losses);
+int sgfim_mlp_get(sgfim_mlp* m, float* W1, float* b1, float* w2, float* b2);
+/* Logits (probs = 0) or sigmoid scores (probs = 1) of n rows with the current weights. */
+int sgfim_mlp_forward(sgfim_mlp* m, const float* X, int32_t n, int32_t probs, float* out);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,11 @@ def lib():
         L.ref_result_nitems.argtypes = [P]; L.ref_result_nitems.restype = C.c_int64
         L.ref_result_copy.argtypes = [P, P, P, P]
         L.ref_result_free.argtypes = [P]
+        L.kat_generate.argtypes = [C.c_int64, P, P, P, P]
+        L.kat_nnz.argtypes = [P, C.c_int64]; L.kat_nnz.restype = C.c_int64
+        L.kat_csr.argtypes = [P, C.c_int64, P, P]
+        L.kat_levelwise.argtypes = [P, C.c_int64, C.c_int64, P, P, C.c_int64]
+        L.kat_levelwise.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -116,3 +121,40 @@ def mine(rows: np.ndarray, thr: float, root_mode: int = ROOTS_ALL, n_total: int 
         L.ref_result_free(h)
     out = [(tuple(int(x) for x in items[off[i]:off[i + 1]]), int(sup[i])) for i in range(n)]
     return out, {"nodes": int(stats[0]), "counts": int(stats[1]), "overflow": int(stats[2])}
+
+
+# --------------------------------------------------------------------------- batch-prototype KAT
+KAT_N = 1 << 23
+KAT_MINSUP = int(KAT_N * 0.15 / 100.0)        # 12582 (batch_proto.cu:240)
+KAT_LEVELS = [248, 585, 587, 350, 117, 17, 0]  # run_dispatch.txt:5-11
+KAT_TOTAL = 1904
+
+
+def kat_generate(n: int = KAT_N):
+    """(rows [512, n/64] uint64, patterns [(sorted item list, freq)])."""
+    rows = np.zeros((512, n // 64), dtype=np.uint64)
+    pi = np.zeros((80, 6), dtype=np.int32)
+    ps = np.zeros(80, dtype=np.int32)
+    pf = np.zeros(80, dtype=np.float64)
+    lib().kat_generate(n, _p(rows), _p(pi), _p(ps), _p(pf))
+    pats = [([int(x) for x in pi[p, :ps[p]]], float(pf[p])) for p in range(80)]
+    return rows, pats
+
+
+def kat_csr(rows: np.ndarray):
+    n = rows.shape[1] * 64
+    nnz = lib().kat_nnz(_p(rows), n)
+    off = np.zeros(n + 1, dtype=np.int64)
+    items = np.zeros(max(nnz, 1), dtype=np.int32)
+    lib().kat_csr(_p(rows), n, _p(off), _p(items))
+    return off, items[:nnz]
+
+
+def kat_levelwise(rows: np.ndarray, minsup: int = KAT_MINSUP, out_cap: int = 4096):
+    """(level counts [1..7], total, list of frequent itemsets as tuples of 0-based items)."""
+    n = rows.shape[1] * 64
+    lc = np.zeros(8, dtype=np.int64)
+    out = np.zeros((out_cap, 8), dtype=np.int32)
+    tot = lib().kat_levelwise(_p(rows), n, minsup, _p(lc), _p(out), out_cap)
+    sets = [tuple(int(x) for x in out[i, 1:1 + out[i, 0]]) for i in range(min(tot, out_cap))]
+    return [int(x) for x in lc[1:8]], int(tot), sets

@@ -109,3 +109,17 @@ def test_engine_root_rules_on_tie_cases():
     # AnyFIM: items tied with the k-th count but ranked after k are not roots
     fam2, thr = O.anyfim_stage_family(txns, 2)   # order 1 (6), 2 (4), 3 (4): roots {1, 2}
     assert thr == 4 and fam2 == {frozenset({1, 2}): 4}
+
+
+def test_batch_proto_kat_oracle_reproduces_published_levels():
+    # code/engine_tensorfim/code/batch_proto/run_dispatch.txt:5-15
+    rows, pats = cref.kat_generate()
+    levels, total, sets = cref.kat_levelwise(rows)
+    assert levels == cref.KAT_LEVELS and total == cref.KAT_TOTAL
+    fs = set(sets)
+    assert sum(tuple(p) in fs for p, _ in pats) == 80
+    # downward closure
+    for s in sets:
+        if len(s) > 2:
+            for i in range(len(s)):
+                assert s[:i] + s[i + 1:] in fs
