@@ -87,7 +87,12 @@ struct sgfim_ctx {
     int32_t d = 0, dpad = 0;
     int64_t W = 0, Wp = 0;
     std::vector<int32_t> ids;           // dense -> original id
-    std::vector<int64_t> h_sup;         // item supports
+    std::vector<int64_t> h_sup;         // item supports of window 0
+    std::vector<int64_t> h_sup_seg;     // item supports per window [nseg][d]
+    int32_t nseg = 0;                   // windows (segments)
+    std::vector<SgSegment> segs;        // host copy of the segment table
+    int64_t rows_words = 0;             // words in the rows arena
+    std::vector<int64_t> win_thr;       // per-window min_count for the next run (empty: args.min_count)
     uint64_t* d_rows = nullptr;
     int32_t* d_sup = nullptr;
     int32_t* d_pair = nullptr;
@@ -201,6 +206,7 @@ void data_free(sgfim_ctx* c)
     c->d_seg_item_base = nullptr;
     c->pair_valid = false;
     c->h_pair.clear();
+    c->win_thr.clear();
     c->loaded = false;
 }
 
@@ -218,10 +224,17 @@ int ensure_pair(sgfim_ctx* c)
 {
     if (c->pair_valid) return SGFIM_OK;
     if (!c->d_pair) {
-        int rc = dev_alloc_t(c, c->data_bufs, &c->d_pair, (size_t)c->dpad * c->dpad);
+        int rc = dev_alloc_t(c, c->data_bufs, &c->d_pair, (size_t)c->dpad * c->dpad * c->nseg);
         if (rc) return rc;
     }
-    HIPCHK(c, sg_launch_pair_mfma(c->d_rows, c->dpad, c->Wp, c->W, c->d_pair, c->stream));
+    if (c->nseg == 1) {
+        HIPCHK(c, sg_launch_pair_mfma(c->d_rows, c->dpad, c->Wp, c->W, c->d_pair, c->stream));
+    } else {
+        int64_t maxw = 0;
+        for (const SgSegment& g : c->segs) maxw = std::max(maxw, g.words);
+        HIPCHK(c, hipMemsetAsync(c->d_pair, 0, (size_t)c->dpad * c->dpad * c->nseg * 4, c->stream));
+        HIPCHK(c, sg_launch_pair_mfma_segs(c->d_rows, c->d_seg, c->nseg, c->dpad, maxw, c->d_pair, c->stream));
+    }
     c->pair_valid = true;
     c->h_pair.clear();
     return SGFIM_OK;
@@ -291,18 +304,21 @@ int read_ctl(sgfim_ctx* c, SearchState* S)
 
 void fill_stats(sgfim_ctx* c, SearchState* S)
 {
-    const int64_t* h = S->B->h_ctl;
     sgfim_round_stats& st = S->stats;
+    st = sgfim_round_stats{};
+    if (S->B && S->B->h_ctl) {  // a run that found nothing frequent has no buffers
+        const int64_t* h = S->B->h_ctl;
+        st.nodes = h[CTL_NODES];
+        st.counts = h[CTL_COUNTS];
+        st.mfis = h[CTL_RING_POS];
+        st.frontier = h[CTL_FRONTIER];
+        st.words = h[CTL_WORDS];
+    }
     st.round = S->rounds;
-    st.nodes = h[CTL_NODES];
-    st.counts = h[CTL_COUNTS];
     st.pair_counts = S->pair_counts;
     st.count_launches = S->count_launches;
     st.count_ms = S->count_ms;
     st.event_ms = S->event_ms;
-    st.mfis = h[CTL_RING_POS];
-    st.frontier = h[CTL_FRONTIER];
-    st.words = h[CTL_WORDS];
     st.elapsed_s = now_s(S);
     st.done = S->done ? 1 : 0;
     st.status = S->status;
@@ -322,7 +338,7 @@ int check_err(sgfim_ctx* c, SearchState* S)
 
 int launch_round(sgfim_ctx* c, SearchState* S)
 {
-    const int32_t uniform = 1;
+    const int32_t uniform = c->nseg == 1 ? 1 : 0;
     SearchBufs* B = S->B;
     (void)hipGetLastError();
     if (!S->use_graph) {
@@ -466,13 +482,22 @@ void sgfim_destroy(sgfim_ctx* c)
     delete c;
 }
 
-int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_t* items, int64_t nnz)
+// Load windows: window w holds transactions [win_off[w], win_off[w+1]) of the CSR.
+// All windows share one item universe (the distinct ids of the whole CSR); each has
+// its own bitmaps (dpad rows x Wp_w words), supports, pair matrix and threshold.
+int sgfim_load_windows(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, const int64_t* off, int64_t n_txn,
+                       const int32_t* items, int64_t nnz)
 {
     (void)hipGetLastError();
     if (!c) return SGFIM_E_INVAL;
-    if (n_txn < 0 || nnz < 0 || (!off && n_txn > 0) || (!items && nnz > 0))
-        return fail(c, SGFIM_E_INVAL, "bad CSR arguments");
-    if (n_txn >= (int64_t)1 << 31) return fail(c, SGFIM_E_INVAL, "at most 2^31-1 transactions are supported");
+    if (n_windows < 1 || !win_off || n_txn < 0 || nnz < 0 || (!off && n_txn > 0) || (!items && nnz > 0))
+        return fail(c, SGFIM_E_INVAL, "bad CSR or window arguments");
+    if (win_off[0] != 0 || win_off[n_windows] != n_txn) return fail(c, SGFIM_E_INVAL, "win_off must span 0..n_txn");
+    for (int32_t w = 0; w < n_windows; w++) {
+        if (win_off[w + 1] < win_off[w]) return fail(c, SGFIM_E_INVAL, "win_off is not non-decreasing at %d", w);
+        if (win_off[w + 1] - win_off[w] >= (int64_t)1 << 31)
+            return fail(c, SGFIM_E_INVAL, "at most 2^31-1 transactions per window are supported");
+    }
     if (n_txn > 0 && (off[0] != 0 || off[n_txn] != nnz))
         return fail(c, SGFIM_E_INVAL, "txn_off must start at 0 and end at nnz");
     for (int64_t t = 0; t < n_txn; t++)
@@ -499,13 +524,32 @@ int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_
     c->d = (int32_t)c->ids.size();
     if (c->d > 65535) return fail(c, SGFIM_E_INVAL, "at most 65535 distinct items are supported (got %d)", c->d);
     c->dpad = std::max(128, (c->d + 127) / 128 * 128);
-    c->W = (n_txn + 63) / 64;
-    c->Wp = std::max<int64_t>(64, (c->W + 63) / 64 * 64);
+    c->nseg = n_windows;
+    c->segs.assign(n_windows, SgSegment{});
+    int64_t base = 0;
+    for (int32_t w = 0; w < n_windows; w++) {
+        SgSegment& g = c->segs[w];
+        const int64_t n = win_off[w + 1] - win_off[w];
+        g.n = (int32_t)n;
+        g.words = (n + 63) / 64;
+        g.stride = std::max<int64_t>(64, (g.words + 63) / 64 * 64);
+        g.row_base = base;
+        base += (int64_t)c->dpad * g.stride;
+        g.d = c->d;
+        g.dpad = c->dpad;
+        g.dw = (c->d + 63) / 64;
+        g.pair_base = (int64_t)w * c->dpad * c->dpad;
+        g.ok_base = (int64_t)w * c->d * g.dw;
+        g.thr = 1;
+    }
+    c->W = c->segs[0].words;
+    c->Wp = c->segs[0].stride;
+    c->rows_words = base;
     int rc;
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_rows, (size_t)c->dpad * c->Wp))) return rc;
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_sup, (size_t)c->dpad))) return rc;
-    HIPCHK(c, hipMemsetAsync(c->d_rows, 0, (size_t)c->dpad * c->Wp * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_sup, 0, (size_t)c->dpad * 4, c->stream));
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_rows, (size_t)base))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_sup, (size_t)c->dpad * n_windows))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_rows, 0, (size_t)base * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_sup, 0, (size_t)c->dpad * n_windows * 4, c->stream));
     if (nnz > 0) {
         std::vector<DevBuf> tmp;
         int64_t* d_off = nullptr;
@@ -519,25 +563,55 @@ int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_
         hipError_t e = hipMemcpyAsync(d_off, off, ((size_t)n_txn + 1) * 8, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(d_items, items, (size_t)nnz * 4, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(d_map, idmap.data(), ((size_t)idmax + 1) * 4, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess)
-            e = sg_launch_transpose(d_off, d_items, d_map, n_txn, idmax, c->d_rows, c->d, c->dpad, c->Wp, c->d_sup, c->stream);
+        for (int32_t w = 0; w < n_windows && e == hipSuccess; w++) {
+            const SgSegment& g = c->segs[w];
+            if (g.n == 0) continue;
+            e = sg_launch_transpose(d_off + win_off[w], d_items, d_map, g.n, idmax, c->d_rows + g.row_base, c->d,
+                                    c->dpad, g.stride, c->d_sup + (int64_t)w * c->dpad, c->stream);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         dev_free_all(c, tmp);
         if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "bitmap build failed: %s", hipGetErrorString(e));
     }
-    std::vector<int32_t> sup32((size_t)std::max(c->d, 1));
-    if (c->d > 0) {
-        HIPCHK(c, hipMemcpyAsync(sup32.data(), c->d_sup, (size_t)c->d * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-    c->h_sup.assign(sup32.begin(), sup32.begin() + c->d);
-    // single segment table
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg, 2))) return rc;  // [0] search, [1] count_batch
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg_item_base, 1))) return rc;
-    int64_t zero = 0;
-    HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<int32_t> sup32((size_t)c->dpad * n_windows);
+    HIPCHK(c, hipMemcpyAsync(sup32.data(), c->d_sup, sup32.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_sup_seg.assign((size_t)c->d * n_windows, 0);
+    for (int32_t w = 0; w < n_windows; w++)
+        for (int32_t i = 0; i < c->d; i++) c->h_sup_seg[(size_t)w * c->d + i] = sup32[(size_t)w * c->dpad + i];
+    c->h_sup.assign(c->h_sup_seg.begin(), c->h_sup_seg.begin() + c->d);
+    // segment tables: [0, nseg) search, [nseg] count_batch
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg, (size_t)n_windows + 1))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg_item_base, (size_t)n_windows))) return rc;
+    std::vector<int64_t> ib(n_windows);
+    for (int32_t w = 0; w < n_windows; w++) ib[w] = (int64_t)w * c->dpad;
+    HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, ib.data(), ib.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), c->segs.size() * sizeof(SgSegment), hipMemcpyHostToDevice,
+                             c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->loaded = true;
+    return SGFIM_OK;
+}
+
+int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_t* items, int64_t nnz)
+{
+    const int64_t w[2] = {0, n_txn};
+    return sgfim_load_windows(c, 1, w, off, n_txn, items, nnz);
+}
+
+int sgfim_window_supports(sgfim_ctx* c, int32_t window, int64_t* out)
+{
+    if (!c || !out) return SGFIM_E_INVAL;
+    if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    if (window < 0 || window >= c->nseg) return fail(c, SGFIM_E_INVAL, "window %d out of range", window);
+    std::copy(c->h_sup_seg.begin() + (size_t)window * c->d, c->h_sup_seg.begin() + (size_t)(window + 1) * c->d, out);
+    return SGFIM_OK;
+}
+
+int sgfim_num_windows(const sgfim_ctx* c, int32_t* n)
+{
+    if (!c || !n) return SGFIM_E_INVAL;
+    *n = c->nseg;
     return SGFIM_OK;
 }
 
@@ -570,6 +644,7 @@ static int pair_out(sgfim_ctx* c, int32_t* out, bool valu)
     (void)hipGetLastError();
     if (!c || !out) return SGFIM_E_INVAL;
     if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    if (c->nseg != 1) return fail(c, SGFIM_E_STATE, "pair_matrix needs a single loaded transaction set");
     (void)hipSetDevice(c->device);
     int rc;
     if (valu) {
@@ -600,6 +675,7 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
     (void)hipGetLastError();
     if (!c || n_prefix < 0 || (n_prefix > 0 && (!prefix_off || !cand_off))) return SGFIM_E_INVAL;
     if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
+    if (c->nseg != 1) return fail(c, SGFIM_E_STATE, "count_batch needs a single loaded transaction set");
     if (n_prefix == 0) return SGFIM_OK;
     if (prefix_off[0] != 0 || cand_off[0] != 0) return fail(c, SGFIM_E_INVAL, "offsets must start at 0");
     // original id -> dense id
@@ -658,7 +734,7 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
     seg.dpad = c->dpad;
     seg.dw = (c->d + 63) / 64;
     seg.n = (int32_t)c->n_txn;
-    HIPCHK(c, hipMemcpyAsync(c->d_seg + 1, &seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_seg + c->nseg, &seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
     std::vector<DevBuf> tmp;
     SgCountNode* d_nodes = nullptr;
     int32_t *d_cand = nullptr, *d_out = nullptr, *d_ctr = nullptr;
@@ -673,7 +749,7 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
     if (e == hipSuccess) e = hipMemsetAsync(d_out, 0, dc.size() * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d_ctr, 0, 4, c->stream);
     if (e == hipSuccess)
-        e = sg_launch_count_nodes(c->d_rows, c->d_seg + 1, d_nodes, n_prefix, units, d_cand, d_out, cw, d_ctr, 2048, c->stream);
+        e = sg_launch_count_nodes(c->d_rows, c->d_seg + c->nseg, d_nodes, n_prefix, units, d_cand, d_out, cw, d_ctr, 2048, c->stream);
     std::vector<int32_t> res(dc.size());
     if (e == hipSuccess) e = hipMemcpyAsync(res.data(), d_out, res.size() * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -706,61 +782,72 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     S->order_mode = (a->flags & SGFIM_ORDER_LEARNED) ? 1 : 0;
     S->use_graph = !(a->flags & SGFIM_NO_GRAPH);
     S->t0 = std::chrono::steady_clock::now();
-    const int64_t thr = a->min_count;
     const int32_t d = c->d;
+    const int32_t nseg = c->nseg;
     int rc;
-    if (thr > (int64_t)INT32_MAX) return fail(c, SGFIM_E_INVAL, "min_count too large");
-    // roots (dense ids): frequent items passing the root rule
-    std::vector<int32_t> order(d);
-    for (int32_t i = 0; i < d; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return c->h_sup[x] > c->h_sup[y]; });
-    std::vector<int32_t> roots;
-    for (int32_t r = 0; r < d; r++) {
-        if (c->h_sup[r] < thr) continue;
-        bool ok = true;
-        if (a->flags & SGFIM_ROOT_STRICT) ok = ((double)c->h_sup[r] / (double)c->n_txn) > a->rel_thr;
-        if (!ok) continue;
-        roots.push_back(r);
+    // per-window thresholds
+    std::vector<int64_t> thr(nseg, a->min_count);
+    if (!c->win_thr.empty()) {
+        if ((int32_t)c->win_thr.size() != nseg) return fail(c, SGFIM_E_INVAL, "window threshold count mismatch");
+        thr = c->win_thr;
     }
-    if (a->flags & SGFIM_ROOT_TOPK) {
-        roots.clear();
-        for (int32_t i = 0; i < std::min<int32_t>(a->root_topk, d); i++)
-            if (c->h_sup[order[i]] >= thr) roots.push_back(order[i]);
-        std::sort(roots.begin(), roots.end());
+    for (int64_t t : thr)
+        if (t < 1 || t > (int64_t)INT32_MAX) return fail(c, SGFIM_E_INVAL, "min_count out of range");
+    // roots (seg, dense id): frequent items passing the root rule, per window
+    std::vector<int32_t> rp;
+    for (int32_t w = 0; w < nseg; w++) {
+        const int64_t* sup = c->h_sup_seg.data() + (size_t)w * d;
+        const int64_t n = c->segs[w].n;
+        if (a->flags & SGFIM_ROOT_TOPK) {
+            std::vector<int32_t> order(d);
+            for (int32_t i = 0; i < d; i++) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return sup[x] > sup[y]; });
+            std::vector<int32_t> r;
+            for (int32_t i = 0; i < std::min<int32_t>(a->root_topk, d); i++)
+                if (sup[order[i]] >= thr[w]) r.push_back(order[i]);
+            std::sort(r.begin(), r.end());
+            for (int32_t x : r) { rp.push_back(w); rp.push_back(x); }
+            continue;
+        }
+        for (int32_t x = 0; x < d; x++) {
+            if (sup[x] < thr[w]) continue;
+            if ((a->flags & SGFIM_ROOT_STRICT) && !(((double)sup[x] / (double)n) > a->rel_thr)) continue;
+            rp.push_back(w);
+            rp.push_back(x);
+        }
     }
-    if (d == 0 || roots.empty() || c->n_txn == 0) {
+    const int64_t n_roots = (int64_t)rp.size() / 2;
+    if (d == 0 || n_roots == 0 || c->n_txn == 0) {
+        S->B = nullptr;  // nothing frequent: no search buffers are touched
         S->active = true;
         S->done = true;
-        S->stats = sgfim_round_stats{};
-        S->stats.done = 1;
-        c->last = S->stats;
+        fill_stats(c, S);
         return SGFIM_OK;
     }
-    if ((rc = ensure_h_pair(c))) return rc;
-    // capacity needed by the seed: children of every root
-    int64_t seed_children = 0;
-    for (int32_t r : roots)
-        for (int32_t e = 0; e < d; e++) {
-            const int32_t p = c->h_pair[(size_t)r * d + e];
-            if (e != r && e < r && p >= thr) seed_children++;
-        }
-    S->pair_counts = (int64_t)roots.size() * d;
-    // device segment + pair_ok
+    if ((rc = ensure_pair(c))) return rc;
+    S->pair_counts = n_roots * d;
+    // segment thresholds + pair_ok masks
     const int32_t dw = (d + 63) / 64;
-    SgSegment seg{};
-    seg.row_base = 0;
-    seg.stride = c->Wp;
-    seg.words = c->W;
-    seg.pair_base = 0;
-    seg.ok_base = 0;
-    seg.d = d;
-    seg.dpad = c->dpad;
-    seg.dw = dw;
-    seg.thr = (int32_t)thr;
-    seg.n = (int32_t)c->n_txn;
-    HIPCHK(c, hipMemcpyAsync(c->d_seg, &seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
-    if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw))) return rc;
-    HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, 1, d, c->d_ok, c->stream));
+    for (int32_t w = 0; w < nseg; w++) c->segs[w].thr = (int32_t)thr[w];
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), (size_t)nseg * sizeof(SgSegment), hipMemcpyHostToDevice,
+                             c->stream));
+    if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * nseg))) return rc;
+    HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, nseg, d, c->d_ok, c->stream));
+    // level-2 nodes the seed creates: frequent pairs (r, e) with e < r, r a root
+    std::vector<uint64_t> okh((size_t)d * dw * nseg);
+    HIPCHK(c, hipMemcpyAsync(okh.data(), c->d_ok, okh.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t seed_children = 0;
+    for (int64_t q = 0; q < n_roots; q++) {
+        const int32_t w = rp[2 * q], r = rp[2 * q + 1];
+        const uint64_t* o = okh.data() + c->segs[w].ok_base + (int64_t)r * dw;
+        for (int32_t k = 0; k < dw; k++) {
+            uint64_t v = o[k];
+            if (k > (r >> 6)) v = 0;
+            else if (k == (r >> 6)) v &= (1ull << (r & 63)) - 1ull;
+            seed_children += __builtin_popcountll(v);
+        }
+    }
     // search buffers (reused across runs when they fit)
     const int32_t bmax = std::min(4096, c->opts.batch_nodes > 0 ? c->opts.batch_nodes : 4096);
     int64_t cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap
@@ -769,21 +856,18 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     if (seed_children > cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld level-2 nodes",
                                          (long long)cap, (long long)seed_children);
     const int64_t newborn_cap = std::max<int64_t>(seed_children, std::min<int64_t>(cap, (int64_t)1 << 20));
-    int64_t ring_cap = c->opts.ring_cap > 0 ? c->opts.ring_cap : std::max<int64_t>(1 << 16, (int64_t)roots.size() + bmax);
-    if (ring_cap < (int64_t)roots.size() + bmax)
-        return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)(roots.size() + bmax));
-    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, pick_chunk(c, c->W), c->trace ? c->trace_cap : 0,
-                          (int64_t)roots.size())))
+    int64_t ring_cap = c->opts.ring_cap > 0 ? c->opts.ring_cap : std::max<int64_t>(1 << 16, n_roots + bmax);
+    if (ring_cap < n_roots + bmax)
+        return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)(n_roots + bmax));
+    int64_t maxw = 0;
+    for (const SgSegment& g : c->segs) maxw = std::max(maxw, g.words);
+    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, nseg == 1 ? pick_chunk(c, c->W) : pick_chunk(c, std::min<int64_t>(maxw, 256)),
+                          c->trace ? c->trace_cap : 0, n_roots)))
         return rc;
     S->B = c->SB;
     SgSearchDev& D = S->B->D;
     HIPCHK(c, sg_search_init(D, c->stream));
-    std::vector<int32_t> rp(roots.size() * 2);
-    for (size_t i = 0; i < roots.size(); i++) {
-        rp[2 * i] = 0;
-        rp[2 * i + 1] = roots[i];
-    }
-    S->n_roots = (int32_t)roots.size();
+    S->n_roots = (int32_t)n_roots;
     HIPCHK(c, hipMemcpyAsync(S->B->d_roots, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, sg_search_seed(D, c->d_seg, c->d_pair, c->d_ok, S->B->d_roots, S->n_roots, c->d_sup, c->d_seg_item_base,
                              S->order_mode, &c->scorer, c->stream));
@@ -913,7 +997,28 @@ int sgfim_result_times(const sgfim_result* r, const double** t)
     return SGFIM_OK;
 }
 
+int sgfim_result_windows(const sgfim_result* r, const int32_t** win)
+{
+    if (!r || !win) return SGFIM_E_INVAL;
+    *win = r->seg.data();
+    return SGFIM_OK;
+}
+
 void sgfim_result_free(sgfim_result* r) { delete r; }
+
+int sgfim_set_window_thresholds(sgfim_ctx* c, const int64_t* min_counts, int32_t n)
+{
+    if (!c || n < 0 || (n > 0 && !min_counts)) return SGFIM_E_INVAL;
+    if (n == 0) {
+        c->win_thr.clear();
+        return SGFIM_OK;
+    }
+    if (!c->loaded || n != c->nseg) return fail(c, SGFIM_E_INVAL, "need one threshold per loaded window (%d)", c->nseg);
+    for (int32_t w = 0; w < n; w++)
+        if (min_counts[w] < 1) return fail(c, SGFIM_E_INVAL, "min_count of window %d must be >= 1", w);
+    c->win_thr.assign(min_counts, min_counts + n);
+    return SGFIM_OK;
+}
 
 int sgfim_last_stats(const sgfim_ctx* c, sgfim_round_stats* st)
 {
@@ -966,7 +1071,7 @@ int sgfim_trace_get(sgfim_ctx* c, int64_t* n_birth, int64_t* n_mfi, int64_t* key
 {
     if (!c || !n_birth || !n_mfi) return SGFIM_E_INVAL;
     SearchState* S = c->S;
-    if (!S || !S->B->D.trace_cap) return fail(c, SGFIM_E_STATE, "no traced run");
+    if (!S || !S->B || !S->B->D.trace_cap) return fail(c, SGFIM_E_STATE, "no traced run");
     (void)hipSetDevice(c->device);
     int rc;
     if ((rc = read_ctl(c, S))) return rc;

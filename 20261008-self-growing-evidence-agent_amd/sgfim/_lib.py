@@ -57,6 +57,11 @@ _SIGS = {
     "sgfim_destroy": (None, [P]),
     "sgfim_last_error": (C.c_char_p, [P]),
     "sgfim_load_csr": (C.c_int, [P, P, C.c_int64, P, C.c_int64]),
+    "sgfim_load_windows": (C.c_int, [P, C.c_int32, P, P, C.c_int64, P, C.c_int64]),
+    "sgfim_num_windows": (C.c_int, [P, C.POINTER(C.c_int32)]),
+    "sgfim_window_supports": (C.c_int, [P, C.c_int32, P]),
+    "sgfim_set_window_thresholds": (C.c_int, [P, P, C.c_int32]),
+    "sgfim_result_windows": (C.c_int, [P, PP]),
     "sgfim_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "sgfim_item_ids": (C.c_int, [P, P]),
     "sgfim_item_supports": (C.c_int, [P, P]),

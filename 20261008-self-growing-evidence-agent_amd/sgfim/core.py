@@ -31,6 +31,7 @@ class MineResult:
     supports: list = field(default_factory=list)
     rounds: list = field(default_factory=list)
     times: list = field(default_factory=list)
+    windows: list = field(default_factory=list)
     stats: dict = field(default_factory=dict)
 
     def as_dict(self) -> dict:
@@ -49,9 +50,11 @@ def _take_result(h: C.c_void_p, stats: dict | None = None) -> MineResult:
     rnd = C.c_void_p()
     n = C.c_int64()
     tms = C.c_void_p()
+    win = C.c_void_p()
     try:
         Lb.sgfim_result_get(h, C.byref(items), C.byref(off), C.byref(sup), C.byref(rnd), C.byref(n))
         Lb.sgfim_result_times(h, C.byref(tms))
+        Lb.sgfim_result_windows(h, C.byref(win))
         nn = n.value
         res = MineResult(stats=stats or {})
         if nn:
@@ -64,6 +67,8 @@ def _take_result(h: C.c_void_p, stats: dict | None = None) -> MineResult:
             res.supports = [int(x) for x in s]
             res.rounds = [int(x) for x in r]
             res.times = [float(x) for x in t]
+            wv = np.ctypeslib.as_array(C.cast(win, C.POINTER(C.c_int32)), shape=(nn,)).copy()
+            res.windows = [int(x) for x in wv]
         return res
     finally:
         Lb.sgfim_result_free(h)
@@ -119,6 +124,38 @@ class Context:
         w = C.c_int64()
         self._L.sgfim_info(self._h, C.byref(nt), C.byref(ni), C.byref(w))
         self.n_txn, self.n_items, self.words = nt.value, ni.value, w.value
+
+    def load_windows(self, win_off: np.ndarray, offsets: np.ndarray, items: np.ndarray):
+        """Load windows: window w = transactions [win_off[w], win_off[w+1]) of the CSR."""
+        wo = np.ascontiguousarray(win_off, dtype=np.int64)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        n = int(off.shape[0]) - 1
+        self._check(self._L.sgfim_load_windows(self._h, int(wo.shape[0]) - 1, _p(wo), _p(off), n, _p(it),
+                                               int(it.shape[0])), "load_windows")
+        nt = C.c_int64()
+        ni = C.c_int32()
+        w = C.c_int64()
+        self._L.sgfim_info(self._h, C.byref(nt), C.byref(ni), C.byref(w))
+        self.n_txn, self.n_items, self.words = nt.value, ni.value, w.value
+        self.n_windows = int(wo.shape[0]) - 1
+
+    def window_supports(self, w: int) -> np.ndarray:
+        out = np.zeros(max(self.n_items, 1), dtype=np.int64)
+        self._check(self._L.sgfim_window_supports(self._h, int(w), _p(out)), "window_supports")
+        return out[: self.n_items]
+
+    def set_window_thresholds(self, min_counts):
+        mc = np.ascontiguousarray(min_counts, dtype=np.int64)
+        self._check(self._L.sgfim_set_window_thresholds(self._h, _p(mc), int(mc.shape[0])), "set_window_thresholds")
+
+    def mine_windows(self, min_counts, flags: int = 0) -> MineResult:
+        """Exact MFIs of every loaded window in one search (result.windows tags each set)."""
+        self.set_window_thresholds(min_counts)
+        try:
+            return self.mine(1, flags=flags)
+        finally:
+            self._L.sgfim_set_window_thresholds(self._h, None, 0)
 
     def load_rows(self, rows: Iterable[Iterable[int]]):
         off, it = csr_from_rows(rows)

@@ -124,3 +124,19 @@ def workload4(seed: int, n_txn: int = 500_000) -> Transactions:
 def min_count(rel: float, n: int) -> int:
     """Absolute threshold of the reference Python miner: max(1, ceil(rel*n)) (skab_spike.py:99-100)."""
     return max(1, int(np.ceil(rel * n)))
+
+
+def concat_windows(windows):
+    """(win_off, offsets, items, n_items) of a list of Transactions laid end to end."""
+    win_off = np.zeros(len(windows) + 1, dtype=np.int64)
+    offs, its = [np.zeros(1, dtype=np.int64)], []
+    base = 0
+    n_items = 0
+    for w, T in enumerate(windows):
+        win_off[w + 1] = win_off[w] + T.n_txn
+        offs.append(T.offsets[1:] + base)
+        its.append(T.items)
+        base += int(T.items.shape[0])
+        n_items = max(n_items, T.n_items)
+    items = np.concatenate(its) if its else np.zeros(0, dtype=np.int32)
+    return win_off, np.concatenate(offs), items.astype(np.int32), n_items

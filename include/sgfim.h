@@ -111,6 +111,22 @@ const char* sgfim_version(void);
 int sgfim_load_csr(sgfim_ctx* ctx, const int64_t* txn_off, int64_t n_txn,
                    const int32_t* items, int64_t nnz);
 
+/* Load many independent windows in one context: window w holds transactions
+ * [win_off[w], win_off[w+1]) of the CSR (win_off has n_windows + 1 entries). Each
+ * window gets its own bitmaps, supports, pair matrix and threshold, and one search
+ * run mines all windows together: every counting launch mixes candidates of
+ * windows of different lengths. No reference counterpart as one call: the
+ * reference mines each window's FIMI file with its own engine process
+ * (engine_adapter.py:70-131). */
+int sgfim_load_windows(sgfim_ctx* ctx, int32_t n_windows, const int64_t* win_off, const int64_t* txn_off,
+                       int64_t n_txn, const int32_t* items, int64_t nnz);
+int sgfim_num_windows(const sgfim_ctx* ctx, int32_t* n);
+/* Item supports of one window (sgfim_item_ids order). */
+int sgfim_window_supports(sgfim_ctx* ctx, int32_t window, int64_t* out);
+/* Per-window min_count for the next runs (n = number of windows; n = 0 clears, and
+ * sgfim_mine_args.min_count then applies to every window). */
+int sgfim_set_window_thresholds(sgfim_ctx* ctx, const int64_t* min_counts, int32_t n);
+
 /* Number of transactions and the distinct item ids of the loaded set
  * (ascending; the library's dense id of ids[k] is k). */
 int sgfim_info(const sgfim_ctx* ctx, int64_t* n_txn, int32_t* n_items, int64_t* words);
@@ -162,6 +178,8 @@ int sgfim_anytime_take(sgfim_ctx* ctx, sgfim_result** out);
  * which set i was confirmed (0 for sgfim_mine). */
 int sgfim_result_get(const sgfim_result* r, const int32_t** items, const int64_t** off,
                      const int64_t** support, const int32_t** round, int64_t* n);
+/* Window of each result set (0 for a single loaded set). */
+int sgfim_result_windows(const sgfim_result* r, const int32_t** win);
 /* Wall time (seconds since the run began) at which each set was confirmed. */
 int sgfim_result_times(const sgfim_result* r, const double** t);
 void sgfim_result_free(sgfim_result* r);

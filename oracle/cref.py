@@ -47,6 +47,8 @@ def lib():
         L.ref_result_nitems.argtypes = [P]; L.ref_result_nitems.restype = C.c_int64
         L.ref_result_copy.argtypes = [P, P, P, P]
         L.ref_result_free.argtypes = [P]
+        L.ref_trace_set.argtypes = [P, C.c_int64]
+        L.ref_trace_count.restype = C.c_int64
         L.kat_generate.argtypes = [C.c_int64, P, P, P, P]
         L.kat_nnz.argtypes = [P, C.c_int64]; L.kat_nnz.restype = C.c_int64
         L.kat_csr.argtypes = [P, C.c_int64, P, P]
@@ -78,6 +80,16 @@ def count(rows: np.ndarray, prefix, cands) -> np.ndarray:
     out = np.zeros(ca.shape[0], dtype=np.int64)
     lib().ref_count(_p(rows), rows.shape[1], _p(pre), pre.shape[0], _p(ca), ca.shape[0], _p(out))
     return out
+
+
+def mine_trace(rows: np.ndarray, thr: float, cap: int = 1 << 22, **kw):
+    """ref_mine with a node trace: (results, stats, trace [n, 6] = parent, depth, support,
+    candidates counted, is MFI, last item); single-threaded."""
+    tr = np.zeros((cap, 6), dtype=np.int64)
+    lib().ref_trace_set(_p(tr), cap)
+    out, st = mine(rows, thr, threads=1, **kw)
+    n = lib().ref_trace_count()
+    return out, st, tr[:n]
 
 
 def row_supports(rows: np.ndarray) -> np.ndarray:

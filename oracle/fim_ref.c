@@ -157,6 +157,10 @@ typedef struct {
     ref_result out;
     int64_t nodes, counts;
     int overflow;
+    /* optional node trace (single-threaded runs): one record per visited node */
+    int64_t* tr;          /* [cap][6]: parent, depth, support, counted, is_mfi, last item */
+    int64_t tr_n, tr_cap;
+    int64_t tr_parent;    /* index of the node being expanded */
 } ref_worker;
 
 /* Count node x[0..k-1] whose tidset is masks[k-1]; candidates are all survivors
@@ -168,6 +172,12 @@ static void visit(const ref_ctx* c, ref_worker* w, int k, int64_t freq)
     if (k > REF_MAX_DEPTH) { w->overflow = 1; return; }
     if (c->max_nodes && w->nodes >= c->max_nodes) { w->overflow = 2; return; }
     w->nodes++;
+    int64_t me = -1;
+    if (w->tr && w->tr_n < w->tr_cap) {
+        me = w->tr_n++;
+        int64_t* r = w->tr + me * 6;
+        r[0] = w->tr_parent; r[1] = k; r[2] = freq; r[3] = 0; r[4] = 0; r[5] = w->x[k - 1];
+    }
     const uint64_t* m = w->masks + (size_t)(k - 1) * W;
     int64_t* res = w->res + (size_t)k * d;
     int32_t* fl = w->cand + (size_t)k * d;
@@ -189,9 +199,14 @@ static void visit(const ref_ctx* c, ref_worker* w, int k, int64_t freq)
         if ((double)s >= c->thr) fl[nf++] = e;
     }
     w->ncand[k] = nf;
+    if (me >= 0) w->tr[me * 6 + 3] = nsrc;
     /* items of X are frequent extensions of X (res = freq(X)), so the node
      * is maximal iff the frequent-extension count equals |X| (kernel_bitmap.cu:870). */
-    if (nf == k) { res_push(&w->out, w->x, k, freq); return; }
+    if (nf == k) {
+        if (me >= 0) w->tr[me * 6 + 4] = 1;
+        res_push(&w->out, w->x, k, freq);
+        return;
+    }
     int32_t progress = w->x[k - 1];
     /* children: frequent ids below progress; pushed ascending onto a LIFO in the
      * reference, visited here in descending id order -- the same tree. */
@@ -202,7 +217,10 @@ static void visit(const ref_ctx* c, ref_worker* w, int k, int64_t freq)
         const uint64_t* r = c->rows + (size_t)i * W;
         for (int64_t q = 0; q < W; q++) cm[q] = m[q] & r[q];
         w->x[k] = i;
+        const int64_t saved = w->tr_parent;
+        w->tr_parent = me;
         visit(c, w, k + 1, res[i]);
+        w->tr_parent = saved;
         if (w->overflow) return;
         /* the recursion overwrote deeper levels only; level k lists stay valid */
     }
@@ -235,11 +253,21 @@ static void freq_order(const int64_t* f, int32_t d, int32_t* order)
  *            REF_ROOTS_TOPK   -> the first topk survivors of the stable frequency order
  * Returns a result handle (sets in dense ids, insertion order: root first).
  */
+static int64_t* g_trace = 0;
+static int64_t g_trace_cap = 0, g_trace_n = 0;
+
+/* Record the visited nodes of the next ref_mine run (forced single-threaded) into
+ * trace[cap][6] = (parent index or -1, depth, support, candidates counted, is MFI,
+ * last item); ref_trace_count() returns the number recorded. */
+void ref_trace_set(int64_t* trace, int64_t cap) { g_trace = trace; g_trace_cap = cap; g_trace_n = 0; }
+int64_t ref_trace_count(void) { return g_trace_n; }
+
 ref_result* ref_mine(const uint64_t* rows, int32_t d, int64_t W, double thr,
                      int root_mode, int64_t n_total, double s, int32_t topk,
                      int count_mode, int threads, int64_t max_nodes,
                      int64_t* stats /* [nodes, counts, overflow] */)
 {
+    if (g_trace) threads = 1;
     ref_ctx c = { rows, d, W, thr, count_mode, max_nodes };
     int64_t* f = (int64_t*)malloc(sizeof(int64_t) * (d > 0 ? d : 1));
     ref_row_supports(rows, d, W, f);
@@ -262,6 +290,7 @@ ref_result* ref_mine(const uint64_t* rows, int32_t d, int64_t W, double thr,
         ws[t].cand = (int32_t*)malloc(sizeof(int32_t) * (size_t)(REF_MAX_DEPTH + 2) * (d > 0 ? d : 1));
         ws[t].ncand = (int32_t*)malloc(sizeof(int32_t) * (REF_MAX_DEPTH + 2));
         res_init(&ws[t].out);
+        ws[t].tr = g_trace; ws[t].tr_cap = g_trace_cap; ws[t].tr_n = 0;
     }
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
     for (int32_t ri = 0; ri < nroots; ri++) {
@@ -272,6 +301,7 @@ ref_result* ref_mine(const uint64_t* rows, int32_t d, int64_t W, double thr,
         ref_worker* w = &ws[tid];
         if (w->overflow) continue;
         int32_t r = order[ri];
+        w->tr_parent = -1;
         memcpy(w->masks, rows + (size_t)r * W, sizeof(uint64_t) * (size_t)W);
         w->x[0] = r;
         visit(&c, w, 1, f[r]);
@@ -284,11 +314,13 @@ ref_result* ref_mine(const uint64_t* rows, int32_t d, int64_t W, double thr,
             res_push(out, ws[t].out.items + ws[t].out.off[i],
                      (int)(ws[t].out.off[i + 1] - ws[t].out.off[i]), ws[t].out.sup[i]);
         nodes += ws[t].nodes; counts += ws[t].counts;
+        if (g_trace) g_trace_n = ws[t].tr_n;
         if (ws[t].overflow) ovf = ws[t].overflow;
         free(ws[t].masks); free(ws[t].res); free(ws[t].cand); free(ws[t].ncand);
         free(ws[t].out.items); free(ws[t].out.off); free(ws[t].out.sup);
     }
     free(ws); free(order); free(f);
+    g_trace = 0;
     if (stats) { stats[0] = nodes; stats[1] = counts; stats[2] = ovf; }
     return out;
 }
