@@ -47,6 +47,8 @@ struct SearchBufs {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;        // around the counting kernel (direct launches)
     int64_t* h_ctl = nullptr;                       // pinned copy of the control block
+    uint16_t* h_ring_items = nullptr;               // MFI ring, pinned host memory mapped into the
+    int32_t* h_ring_meta = nullptr;                 // device: kernels write MFIs straight to the host
     int32_t* d_roots = nullptr;
     int64_t roots_cap = 0;
 };
@@ -186,6 +188,8 @@ void bufs_free(sgfim_ctx* c)
         if (B->graph[i]) (void)hipGraphDestroy(B->graph[i]);
     }
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
+    if (B->h_ring_items) (void)hipHostFree(B->h_ring_items);
+    if (B->h_ring_meta) (void)hipHostFree(B->h_ring_meta);
     if (B->ev0) (void)hipEventDestroy(B->ev0);
     if (B->ev1) (void)hipEventDestroy(B->ev1);
     dev_free_all(c, B->bufs);
@@ -271,20 +275,21 @@ int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
     std::vector<int32_t> meta((size_t)n * 4);
     const int64_t a = S->ring_read % cap;
     const int64_t first = std::min(n, cap - a);
-    HIPCHK(c, hipMemcpyAsync(it.data(), S->B->D.ring_items + a * SG_MAX_DEPTH, first * SG_MAX_DEPTH * 2,
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(meta.data(), S->B->D.ring_meta + a * 4, first * 16, hipMemcpyDeviceToHost, c->stream));
+    // the ring lives in mapped host memory and the stream has been synchronised
+    memcpy(it.data(), S->B->h_ring_items + a * SG_MAX_DEPTH, (size_t)first * SG_MAX_DEPTH * 2);
+    memcpy(meta.data(), S->B->h_ring_meta + a * 4, (size_t)first * 16);
     if (first < n) {
-        HIPCHK(c, hipMemcpyAsync(it.data() + first * SG_MAX_DEPTH, S->B->D.ring_items, (n - first) * SG_MAX_DEPTH * 2,
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(meta.data() + first * 4, S->B->D.ring_meta, (n - first) * 16,
-                                 hipMemcpyDeviceToHost, c->stream));
+        memcpy(it.data() + first * SG_MAX_DEPTH, S->B->h_ring_items, (size_t)(n - first) * SG_MAX_DEPTH * 2);
+        memcpy(meta.data() + first * 4, S->B->h_ring_meta, (size_t)(n - first) * 16);
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     sgfim_result& R = S->pending;
     for (int64_t i = 0; i < n; i++) {
         const int32_t k = meta[i * 4], sup = meta[i * 4 + 1], seg = meta[i * 4 + 2], rnd = meta[i * 4 + 3];
-        for (int j = 0; j < k; j++) R.items.push_back(c->ids[it[i * SG_MAX_DEPTH + j]]);
+        // descending item ids: the order in which the reference's search adds items (the
+        // root is the largest id, every child a smaller one), hence its Results lines
+        uint16_t* x = it.data() + i * SG_MAX_DEPTH;
+        std::sort(x, x + k, [](uint16_t a, uint16_t b) { return a > b; });
+        for (int j = 0; j < k; j++) R.items.push_back(c->ids[x[j]]);
         R.off.push_back((int64_t)R.items.size());
         R.sup.push_back(sup);
         R.round.push_back(rnd);
@@ -399,8 +404,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.newborn, (size_t)newborn_cap)) ||
         (rc = dev_alloc_t(c, V, &D.feat, (size_t)newborn_cap * SG_FEAT)) ||
         (rc = dev_alloc_t(c, V, &D.scores, (size_t)newborn_cap)) ||
-        (rc = dev_alloc_t(c, V, &D.ring_items, (size_t)ring_cap * SG_MAX_DEPTH)) ||
-        (rc = dev_alloc_t(c, V, &D.ring_meta, (size_t)ring_cap * 4)) || (rc = dev_alloc_t(c, V, &D.ctl, CTL_SLOTS)) ||
+        (rc = dev_alloc_t(c, V, &D.ctl, CTL_SLOTS)) ||
         (rc = dev_alloc_t(c, V, &B->d_roots, (size_t)B->roots_cap * 2))) {
         bufs_free(c);
         return rc;
@@ -419,6 +423,14 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         return fail(c, SGFIM_E_NOMEM, "pinned control block allocation failed");
     }
     memset(B->h_ctl, 0, CTL_SLOTS * sizeof(int64_t));
+    if (hipHostMalloc((void**)&B->h_ring_items, (size_t)ring_cap * SG_MAX_DEPTH * 2, hipHostMallocMapped) != hipSuccess ||
+        hipHostMalloc((void**)&B->h_ring_meta, (size_t)ring_cap * 16, hipHostMallocMapped) != hipSuccess) {
+        (void)hipGetLastError();
+        bufs_free(c);
+        return fail(c, SGFIM_E_NOMEM, "pinned MFI ring allocation failed");
+    }
+    HIPCHK(c, hipHostGetDevicePointer((void**)&D.ring_items, B->h_ring_items, 0));
+    HIPCHK(c, hipHostGetDevicePointer((void**)&D.ring_meta, B->h_ring_meta, 0));
     HIPCHK(c, hipEventCreate(&B->ev0));
     HIPCHK(c, hipEventCreate(&B->ev1));
     return SGFIM_OK;
@@ -833,23 +845,29 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
                              c->stream));
     if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * nseg))) return rc;
     HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, nseg, d, c->d_ok, c->stream));
-    // level-2 nodes the seed creates: frequent pairs (r, e) with e < r, r a root
-    std::vector<uint64_t> okh((size_t)d * dw * nseg);
-    HIPCHK(c, hipMemcpyAsync(okh.data(), c->d_ok, okh.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // level-2 nodes the seed creates: frequent pairs (r, e) with e < r, r a root. For
+    // one set the bound sum(r) over roots avoids a device round trip; for windows the
+    // exact count comes from the pair_ok masks.
     int64_t seed_children = 0;
-    for (int64_t q = 0; q < n_roots; q++) {
-        const int32_t w = rp[2 * q], r = rp[2 * q + 1];
-        const uint64_t* o = okh.data() + c->segs[w].ok_base + (int64_t)r * dw;
-        for (int32_t k = 0; k < dw; k++) {
-            uint64_t v = o[k];
-            if (k > (r >> 6)) v = 0;
-            else if (k == (r >> 6)) v &= (1ull << (r & 63)) - 1ull;
-            seed_children += __builtin_popcountll(v);
+    if (nseg == 1) {
+        for (int64_t q = 0; q < n_roots; q++) seed_children += rp[2 * q + 1];
+    } else {
+        std::vector<uint64_t> okh((size_t)d * dw * nseg);
+        HIPCHK(c, hipMemcpyAsync(okh.data(), c->d_ok, okh.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t q = 0; q < n_roots; q++) {
+            const int32_t w = rp[2 * q], r = rp[2 * q + 1];
+            const uint64_t* o = okh.data() + c->segs[w].ok_base + (int64_t)r * dw;
+            for (int32_t k = 0; k < dw; k++) {
+                uint64_t v = o[k];
+                if (k > (r >> 6)) v = 0;
+                else if (k == (r >> 6)) v &= (1ull << (r & 63)) - 1ull;
+                seed_children += __builtin_popcountll(v);
+            }
         }
     }
     // search buffers (reused across runs when they fit)
-    const int32_t bmax = std::min(4096, c->opts.batch_nodes > 0 ? c->opts.batch_nodes : 4096);
+    const int32_t bmax = std::min(SG_BMAX, c->opts.batch_nodes > 0 ? c->opts.batch_nodes : SG_BMAX);
     int64_t cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap
                                            : std::max<int64_t>({(int64_t)1 << 16, 4 * seed_children + 4 * bmax, c->cap_hint});
     if (cap > ((int64_t)1 << 31) - 1) cap = ((int64_t)1 << 31) - 1;
@@ -866,16 +884,26 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         return rc;
     S->B = c->SB;
     SgSearchDev& D = S->B->D;
+    const int32_t feat_on = (S->order_mode == 1 || D.trace_cap > 0) ? 1 : 0;
+    if (D.feat_on != feat_on) {  // captured rounds hold the old value: capture again
+        for (int i = 0; i < 2; i++) {
+            if (S->B->exec[i]) (void)hipGraphExecDestroy(S->B->exec[i]);
+            if (S->B->graph[i]) (void)hipGraphDestroy(S->B->graph[i]);
+            S->B->exec[i] = nullptr;
+            S->B->graph[i] = nullptr;
+        }
+        D.feat_on = feat_on;
+    }
     HIPCHK(c, sg_search_init(D, c->stream));
     S->n_roots = (int32_t)n_roots;
     HIPCHK(c, hipMemcpyAsync(S->B->d_roots, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, sg_search_seed(D, c->d_seg, c->d_pair, c->d_ok, S->B->d_roots, S->n_roots, c->d_sup, c->d_seg_item_base,
                              S->order_mode, &c->scorer, c->stream));
-    if ((rc = read_ctl(c, S))) return rc;
-    if ((rc = check_err(c, S))) return rc;
-    if ((rc = drain_ring(c, S, S->B->h_ctl[CTL_RING_POS], now_s(S)))) return rc;
+    // no synchronisation here: the seed's MFIs, errors and frontier are read after the
+    // first round (a round over an empty frontier does nothing)
+    memset(S->B->h_ctl, 0, CTL_SLOTS * sizeof(int64_t));
     S->active = true;
-    S->done = S->B->h_ctl[CTL_FRONTIER] == 0;
+    S->done = false;
     fill_stats(c, S);
     return SGFIM_OK;
 }
@@ -1054,6 +1082,38 @@ int sgfim_set_scorer(sgfim_ctx* c, const float* w1, const float* b1, const float
     HIPCHK(c, hipMemcpyAsync(s.b2, &b2, 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->scorer_set = true;
+    return SGFIM_OK;
+}
+
+int sgfim_copy_probe(sgfim_ctx* c, int64_t bytes, int32_t iters, double* gbps)
+{
+    (void)hipGetLastError();
+    if (!c || bytes < 16 || iters < 1 || !gbps) return SGFIM_E_INVAL;
+    (void)hipSetDevice(c->device);
+    bytes = bytes / 16 * 16;
+    std::vector<DevBuf> tmp;
+    void *a = nullptr, *b = nullptr;
+    int rc;
+    if ((rc = dev_alloc(c, tmp, &a, (size_t)bytes)) || (rc = dev_alloc(c, tmp, &b, (size_t)bytes))) {
+        dev_free_all(c, tmp);
+        return rc;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemsetAsync(a, 1, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = sg_launch_copy_probe(a, b, bytes, c->stream);   // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int32_t i = 0; i < iters && e == hipSuccess; i++) e = sg_launch_copy_probe(i & 1 ? b : a, i & 1 ? a : b, bytes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free_all(c, tmp);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "copy probe: %s", hipGetErrorString(e));
+    *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;  // bytes read + bytes written
     return SGFIM_OK;
 }
 

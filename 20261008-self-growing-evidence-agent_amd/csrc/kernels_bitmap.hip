@@ -70,3 +70,18 @@ hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const i
                        rows, d, Wp, supports);
     return hipGetLastError();
 }
+
+// Plain device copy (16-byte loads and stores, grid-stride): the on-box bandwidth probe
+// and the known-size reference for hardware byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t sg_launch_copy_probe(const void* src, void* dst, int64_t bytes, hipStream_t s)
+{
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}

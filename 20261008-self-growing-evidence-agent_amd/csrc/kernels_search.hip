@@ -111,18 +111,22 @@ __device__ void sg_make_child(const SgSearchDev& S, const SgSegment& sg, const u
         else if (w == ew) tail += __popcll(v & ((1ull << eb) - 1ull));
     }
     uint16_t* ci = S.items + (int64_t)slot * SG_MAX_DEPTH;
-    int32_t mn = item_sup[e], mx = item_sup[e], pmin = 0x7fffffff;
-    float sl = __log2f((float)item_sup[e]);
-    const int32_t* pe = pair + sg.pair_base + (int64_t)e * sg.dpad;
-    for (int i = 0; i < k; i++) {
-        const int32_t x = px[i];
-        ci[i] = (uint16_t)x;
-        const int32_t sx = item_sup[x];
-        mn = min(mn, sx);
-        mx = max(mx, sx);
-        sl += __log2f((float)sx);
-        pmin = min(pmin, pe[x]);
+    int32_t mn = 0, mx = 0, pmin = 0x7fffffff;
+    float sl = 0.0f;
+    if (feat_out) {
+        mn = item_sup[e];
+        mx = item_sup[e];
+        sl = __log2f((float)item_sup[e]);
+        const int32_t* pe = pair + sg.pair_base + (int64_t)e * sg.dpad;
+        for (int i = 0; i < k; i++) {
+            const int32_t sx = item_sup[px[i]];
+            mn = min(mn, sx);
+            mx = max(mx, sx);
+            sl += __log2f((float)sx);
+            pmin = min(pmin, pe[px[i]]);
+        }
     }
+    for (int i = 0; i < k; i++) ci[i] = px[i];
     ci[k] = (uint16_t)e;
     SgNodeMeta m;
     m.k = k + 1;
@@ -166,103 +170,120 @@ __device__ __forceinline__ void sg_emit(const SgSearchDev& S, const uint16_t* px
     }
 }
 
-// Shared per-wave expansion: F mask (LDS) is ready; create children for frequent
-// extensions below `last`, or emit the node if F is empty.
+// Shared per-wave expansion. F (LDS) holds the frequent one-item extensions of X,
+// EQ (LDS) the equivalent ones, sup(X u {e}) == sup(X). Parent-equivalence pruning,
+// exact for maximal itemsets because T(X) is contained in T(e) for e in EQ:
+//   * an equivalent item outside the enumeration tail (id >= last) extends every
+//     itemset of X's subtree without losing support, so the subtree holds no MFI;
+//   * the equivalent tail items join every MFI of the subtree, so they are merged
+//     into X (X' = X u EQ, T(X') = T(X): the counts of X serve X' unchanged) and
+//     removed from F before X' is tested for maximality and expanded.
+// Then X' is emitted iff F is empty, otherwise its frequent tail extensions become
+// children (the reference's node step, kernel_bitmap.cu:870-889).
 __device__ void sg_expand_common(const SgSearchDev& S, const SgSegment& sg, const uint64_t* ok,
                                  const int32_t* item_sup, const int32_t* pair, uint64_t* F,
-                                 const uint16_t* px, int k, int32_t sup_x, int32_t last, int seg,
+                                 const uint64_t* EQ, uint16_t* px, int k, int32_t sup_x, int32_t last, int seg,
                                  const int32_t* ext_item, const int32_t* ext_cnt, int next,
                                  int64_t self_key)
 {
     const int lane = threadIdx.x & 63;
-    // frequent-extension count and tail count
-    int nf = 0, nft = 0;
+    const int lw = last >> 6, lb = last & 63;
+    int nf = 0, nft = 0, net = 0, nen = 0;
     for (int w = 0; w < sg.dw; w++) {
-        const uint64_t v = F[w];
+        const uint64_t low = w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
+        const uint64_t v = F[w], q = EQ[w];
         nf += __popcll(v);
-        const int lw = last >> 6, lb = last & 63;
-        if (w < lw) nft += __popcll(v);
-        else if (w == lw) nft += __popcll(v & ((1ull << lb) - 1ull));
+        nft += __popcll(v & low);
+        net += __popcll(q & low);
+        nen += __popcll(q & ~low);
     }
     if (nf == 0) {
         if (lane == 0) sg_emit(S, px, k, sup_x, seg, self_key);
         return;
+    }
+    if (nen > 0) return;
+    if (net > 0) {
+        if (k + net > SG_MAX_DEPTH) {
+            if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 2ull);
+            return;
+        }
+        if (lane == 0) {
+            int kk = k;
+            for (int w = 0; w < sg.dw; w++) {
+                const uint64_t low = w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
+                uint64_t q = EQ[w] & low;
+                F[w] &= ~q;
+                while (q) {
+                    px[kk++] = (uint16_t)(w * 64 + (int)__builtin_ctzll(q));
+                    q &= q - 1ull;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        k += net;
+        nf -= net;
+        nft -= net;
+        if (nf == 0) {
+            if (lane == 0) sg_emit(S, px, k, sup_x, seg, self_key);
+            return;
+        }
     }
     if (nft == 0) return;
     if (k >= SG_MAX_DEPTH) {
         if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 2ull);
         return;
     }
-    // Reserve nft newborn entries, then nft free slots. Both use compare-and-swap so
-    // a counter never passes its bound; on failure the reserved newborn entries are
-    // marked -1 (k_push skips them) and the error flag stops the run.
+    // Reserve nft newborn entries and nft free slots with one atomic each. The round's
+    // node budget (k_select) and the seed's bound (host) keep both within capacity; if
+    // a counter ever overflows, nothing is written out of bounds: the reserved newborn
+    // entries inside the table are marked -1 (k_push skips them) and the error flag
+    // ends the run.
     int64_t base = 0, nb = 0;
-    int okres = 0;
     if (lane == 0) {
-        unsigned long long* pn = (unsigned long long*)&S.ctl[CTL_NEWBORN];
-        unsigned long long cur = *(volatile unsigned long long*)pn;
-        for (;;) {
-            if ((int64_t)cur + nft > S.newborn_cap) { okres = 0; break; }
-            const unsigned long long prev = atomicCAS(pn, cur, cur + (unsigned long long)nft);
-            if (prev == cur) { okres = 1; break; }
-            cur = prev;
-        }
-        nb = (int64_t)cur;
-        if (okres) {
-            unsigned long long* pf = (unsigned long long*)&S.ctl[CTL_FREE_TOP];
-            unsigned long long f = *(volatile unsigned long long*)pf;
-            for (;;) {
-                if ((int64_t)f < nft) { okres = 2; break; }
-                const unsigned long long prev = atomicCAS(pf, f, f - (unsigned long long)nft);
-                if (prev == f) break;
-                f = prev;
-            }
-            base = (int64_t)f;
-        }
+        nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], (unsigned long long)nft);
+        base = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-(int64_t)nft));
     }
-    okres = __shfl(okres, 0, 64);
     base = __shfl(base, 0, 64);
     nb = __shfl(nb, 0, 64);
-    if (okres != 1) {
-        if (okres == 2)
-            for (int t = lane; t < nft; t += 64) S.newborn[nb + t] = -1;
+    if (nb + nft > S.newborn_cap || base - nft < 0 || base > S.cap) {
+        for (int t = lane; t < nft; t += 64)
+            if (nb + t >= 0 && nb + t < S.newborn_cap) S.newborn[nb + t] = -1;
         if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
         return;
     }
     int64_t kbase = 0;
     if (lane == 0) kbase = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], (unsigned long long)nft);
     kbase = __shfl(kbase, 0, 64);
-    // children in ascending item order: the t-th frequent tail extension
-    for (int t = lane; t < nft; t += 64) {
-        // find the t-th set bit of F below `last`
-        int rem = t, e = -1;
-        for (int w = 0; w < sg.dw && e < 0; w++) {
-            uint64_t v = F[w];
-            if (w == (last >> 6)) v &= (1ull << (last & 63)) - 1ull;
-            else if (w > (last >> 6)) v = 0;
-            const int c = __popcll(v);
-            if (rem < c) {
-                for (int q = 0; q < rem; q++) v &= v - 1ull;
-                e = w * 64 + (int)__builtin_ctzll(v);
-            } else rem -= c;
-        }
-        int32_t sup_y = 0;
-        if (ext_item) {
-            // counts listed per candidate (ascending ids): locate e
-            int lo = 0, hi = next - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (ext_item[mid] < e) lo = mid + 1; else hi = mid;
+    // children in ascending item order: one pass over the tail words, lane j taking bit j
+    int tbase = 0;
+    for (int w = 0; w < sg.dw; w++) {
+        const uint64_t low = w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
+        const uint64_t v = F[w] & low;
+        if (v == 0ull) continue;
+        if ((v >> lane) & 1ull) {
+            const int t = tbase + __popcll(v & ((1ull << lane) - 1ull));
+            const int e = w * 64 + lane;
+            int32_t sup_y;
+            if (ext_item) {
+                // counts listed per candidate (ascending ids): locate e
+                int lo = 0, hi = next - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (ext_item[mid] < e) lo = mid + 1; else hi = mid;
+                }
+                sup_y = ext_cnt[lo];
+            } else {
+                sup_y = ext_cnt[e];  // seed: pair-matrix row of the root
             }
-            sup_y = ext_cnt[lo];
-        } else {
-            sup_y = ext_cnt[e];  // seed: pair-matrix row of the root
+            const int32_t slot = S.free_idx[base - 1 - t];
+            float* fo = S.feat_on ? S.feat + (nb + t) * SG_FEAT : nullptr;
+            sg_make_child(S, sg, ok, item_sup, pair, slot, px, k, sup_x, nf, nft, F, e, sup_y, seg, fo,
+                          kbase + t, self_key);
+            S.newborn[nb + t] = slot;
         }
-        const int32_t slot = S.free_idx[base - 1 - t];
-        float* fo = S.feat ? S.feat + (nb + t) * SG_FEAT : nullptr;
-        sg_make_child(S, sg, ok, item_sup, pair, slot, px, k, sup_x, nf, nft, F, e, sup_y, seg, fo,
-                      kbase + t, self_key);
-        S.newborn[nb + t] = slot;
+        tbase += __popcll(v);
     }
 }
 
@@ -276,11 +297,14 @@ __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __
                                               const int64_t* __restrict__ seg_item_base)
 {
     __shared__ uint64_t sF[4][64];
+    __shared__ uint64_t sE[4][64];
     __shared__ uint16_t sX[4][SG_MAX_DEPTH];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t q = (int64_t)blockIdx.x * 4 + wv;
     const bool live = q < n_roots;
     int seg = 0, r = 0;
+    sE[wv][lane] = 0ull;
+    __syncthreads();
     if (live) {
         seg = roots[2 * q];
         r = roots[2 * q + 1];
@@ -293,6 +317,10 @@ __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __
         }
         sF[wv][lane] = v;
         if (lane == 0) sX[wv][0] = (uint16_t)r;
+        const int32_t* prow = pair + sg.pair_base + (int64_t)r * sg.dpad;
+        const int32_t sr = sup_all[seg_item_base[seg] + r];
+        for (int e = lane; e < sg.d; e += 64)
+            if (e != r && prow[e] == sr) atomicOr((unsigned long long*)&sE[wv][e >> 6], 1ull << (e & 63));
     }
     __syncthreads();
     if (!live) return;
@@ -301,7 +329,7 @@ __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __
     int64_t key = 0;
     if (lane == 0) key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], 1ull);
     key = __shfl(key, 0, 64);
-    sg_expand_common(S, sg, ok, isup, pair, sF[wv], sX[wv], 1, isup[r], r, seg, nullptr,
+    sg_expand_common(S, sg, ok, isup, pair, sF[wv], sE[wv], sX[wv], 1, isup[r], r, seg, nullptr,
                      pair + sg.pair_base + (int64_t)r * sg.dpad, 0, key);
     if (lane == 0) atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], 1ull);
 }
@@ -338,16 +366,18 @@ __device__ __forceinline__ int64_t sg_block_excl_scan(int64_t v, int64_t* wsum, 
 // free-slot and newborn budgets; write candidate offsets and work-unit offsets.
 __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
-    __shared__ int32_t take[SG_NBUCKETS], start[SG_NBUCKETS + 1];
+    __shared__ int32_t take[SG_NBUCKETS], start[SG_NBUCKETS + 1], tops[SG_NBUCKETS];
     __shared__ int64_t wsum[SEL_THREADS / 64];
     __shared__ int64_t s_budget, s_tot;
     __shared__ unsigned long long s_ctot, s_units, s_words;
     __shared__ int32_t s_total, s_cut;
     const int tid = threadIdx.x;
+    if (tid < SG_NBUCKETS) tops[tid] = (int32_t)S.ctl[CTL_BUCKET_TOP0 + tid];
+    __syncthreads();
     if (tid == 0) {
         int32_t taken = 0;
         for (int b = SG_NBUCKETS - 1; b >= 0; b--) {
-            const int32_t top = (int32_t)S.ctl[CTL_BUCKET_TOP0 + b];
+            const int32_t top = tops[b];
             const int32_t t = min(top, S.bmax - taken);
             take[b] = t;
             taken += t;
@@ -366,24 +396,38 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
     }
     __syncthreads();
     const int total = s_total;
-    constexpr int PER = 4;
-    int64_t nc[PER], nu[PER], nw[PER];
-    int32_t nid[PER];
+    constexpr int PER = SG_BMAX / SEL_THREADS;
+    int32_t nc[PER], nu[PER], nid[PER];
     int64_t sc = 0, su = 0;
+    // phase 1: arena slots of this thread's positions (independent loads)
+#pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = tid * PER + q;
-        nc[q] = 0; nu[q] = 0; nw[q] = 0; nid[q] = -1;
+        nid[q] = -1;
         if (i < total) {
             int b = SG_NBUCKETS - 1;
             while (!(i >= start[b] && i < start[b] + take[b])) b--;
-            const int32_t top = (int32_t)S.ctl[CTL_BUCKET_TOP0 + b];
-            const int32_t slot = S.bucket[(int64_t)b * S.cap + (top - 1 - (i - start[b]))];
-            const SgNodeMeta m = S.meta[slot];
-            const int64_t words = segs[m.seg].words;
-            nid[q] = slot;
-            nc[q] = m.ncand;
-            nu[q] = (words + S.chunk_words - 1) / S.chunk_words;
-            nw[q] = (int64_t)(m.ncand + 1) * words;
+            nid[q] = S.bucket[(int64_t)b * S.cap + (tops[b] - 1 - (i - start[b]))];
+        }
+    }
+    // phase 2: their candidate counts and segments
+    int32_t sgi[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        nc[q] = 0;
+        sgi[q] = 0;
+        if (nid[q] >= 0) {
+            nc[q] = S.meta[nid[q]].ncand;
+            sgi[q] = S.meta[nid[q]].seg;
+        }
+    }
+    // phase 3: work units
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        nu[q] = 0;
+        if (nid[q] >= 0) {
+            const int64_t words = segs[sgi[q]].words;
+            nu[q] = (int32_t)((words + S.chunk_words - 1) / S.chunk_words);
         }
         sc += nc[q];
         su += nu[q];
@@ -404,14 +448,18 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
     }
     __syncthreads();
     const int cut = s_cut;
+    unsigned long long mc = 0, mu = 0, mw = 0;
     for (int q = 0; q < PER; q++) {
         const int i = tid * PER + q;
         if (i < cut) {
-            atomicAdd(&s_ctot, (unsigned long long)nc[q]);
-            atomicAdd(&s_units, (unsigned long long)nu[q]);
-            atomicAdd(&s_words, (unsigned long long)nw[q]);
+            mc += (unsigned long long)nc[q];
+            mu += (unsigned long long)nu[q];
+            mw += (unsigned long long)(nc[q] + 1) * (unsigned long long)segs[sgi[q]].words;
         }
     }
+    if (mc) atomicAdd(&s_ctot, mc);
+    if (mu) atomicAdd(&s_units, mu);
+    if (mw) atomicAdd(&s_words, mw);
     __syncthreads();
     if (tid == 0) {
         if (cut == 0 && total > 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
@@ -523,6 +571,7 @@ __global__ __launch_bounds__(256) void k_expand(SgSearchDev S, const SgSegment* 
                                                 const int64_t* __restrict__ seg_item_base)
 {
     __shared__ uint64_t sF[4][64];
+    __shared__ uint64_t sE[4][64];
     __shared__ uint16_t sX[4][SG_MAX_DEPTH];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x * 4 + wv;
@@ -533,6 +582,7 @@ __global__ __launch_bounds__(256) void k_expand(SgSearchDev S, const SgSegment* 
         slot = S.batch[b];
         m = S.meta[slot];
         sF[wv][lane] = 0ull;
+        sE[wv][lane] = 0ull;
         if (lane < m.k) sX[wv][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
     }
     __syncthreads();
@@ -540,15 +590,17 @@ __global__ __launch_bounds__(256) void k_expand(SgSearchDev S, const SgSegment* 
     const SgSegment sg = segs[m.seg];
     const int32_t c0 = S.coff[b], nc = S.coff[b + 1] - c0;
     for (int j = lane; j < nc; j += 64) {
-        if (S.ccount[c0 + j] >= sg.thr) {
+        const int32_t cnt = S.ccount[c0 + j];
+        if (cnt >= sg.thr) {
             const int e = S.clist[c0 + j];
             atomicOr((unsigned long long*)&sF[wv][e >> 6], 1ull << (e & 63));
+            if (cnt == m.sup) atomicOr((unsigned long long*)&sE[wv][e >> 6], 1ull << (e & 63));
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
     const int64_t key = S.key[slot];
-    sg_expand_common(S, sg, ok, sup_all + seg_item_base[m.seg], pair, sF[wv], sX[wv], m.k, m.sup,
+    sg_expand_common(S, sg, ok, sup_all + seg_item_base[m.seg], pair, sF[wv], sE[wv], sX[wv], m.k, m.sup,
                      m.last, m.seg, S.clist + c0, S.ccount + c0, nc, key);
 }
 
@@ -591,7 +643,7 @@ __global__ void k_release(SgSearchDev S)
     const int nb = (int)S.ctl[CTL_BATCH_N];
     if (i < nb) {
         const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], 1ull);
-        S.free_idx[pos] = S.batch[i];
+        if (pos >= 0 && pos < S.cap) S.free_idx[pos] = S.batch[i];   // out of range only after an overflow
     }
     if (i == 0) {
         S.ctl[CTL_ROUND] += 1;

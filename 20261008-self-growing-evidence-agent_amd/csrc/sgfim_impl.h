@@ -18,6 +18,7 @@
 #define SG_MAX_DEPTH 32
 #define SG_NBUCKETS 64
 #define SG_WAVE 64
+#define SG_BMAX 16384   // largest batch of frontier nodes per round
 
 struct SgSegment {
     int64_t row_base;   // word offset of row 0 of this segment in the rows arena
@@ -49,6 +50,8 @@ struct SgNodeMeta {
 hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const int32_t* idmap,
                                int64_t n_txn, int32_t id_max, uint64_t* rows, int32_t d,
                                int32_t dpad, int64_t Wp, int32_t* supports, hipStream_t s);
+
+hipError_t sg_launch_copy_probe(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
                                int32_t* pair, hipStream_t s);
@@ -114,7 +117,7 @@ struct SgSearchDev {
     int32_t bmax;
     int32_t chunk_words;
     int32_t count_grid;
-    int32_t pad_;
+    int32_t feat_on;     // compute the 24 features of newborn nodes (learned order or tracing)
 };
 
 // Control block layout (int64 slots of SgSearchDev::ctl).

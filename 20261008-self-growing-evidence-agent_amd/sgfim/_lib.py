@@ -77,6 +77,7 @@ _SIGS = {
     "sgfim_result_get": (C.c_int, [P, PP, PP, PP, PP, C.POINTER(C.c_int64)]),
     "sgfim_result_times": (C.c_int, [P, PP]),
     "sgfim_result_free": (None, [P]),
+    "sgfim_copy_probe": (C.c_int, [P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "sgfim_last_stats": (C.c_int, [P, C.POINTER(RoundStats)]),
     "sgfim_set_scorer": (C.c_int, [P, P, P, P, C.c_float]),
     "sgfim_set_trace": (C.c_int, [P, C.c_int32, C.c_int64]),

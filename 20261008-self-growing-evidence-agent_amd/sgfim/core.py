@@ -33,16 +33,18 @@ class MineResult:
     times: list = field(default_factory=list)
     windows: list = field(default_factory=list)
     stats: dict = field(default_factory=dict)
+    count: int = 0
+    arrays: tuple = ()   # (items int32, off int64, support int64) without per-set tuples
 
     def as_dict(self) -> dict:
         """{frozenset(ids): support} -- the adapter's comparison form (engine_adapter.py:103-111)."""
         return {frozenset(x): s for x, s in zip(self.itemsets, self.supports)}
 
     def __len__(self):
-        return len(self.itemsets)
+        return self.count
 
 
-def _take_result(h: C.c_void_p, stats: dict | None = None) -> MineResult:
+def _take_result(h: C.c_void_p, stats: dict | None = None, convert: bool = True) -> MineResult:
     Lb = L.lib()
     items = C.c_void_p()
     off = C.c_void_p()
@@ -56,8 +58,13 @@ def _take_result(h: C.c_void_p, stats: dict | None = None) -> MineResult:
         Lb.sgfim_result_times(h, C.byref(tms))
         Lb.sgfim_result_windows(h, C.byref(win))
         nn = n.value
-        res = MineResult(stats=stats or {})
-        if nn:
+        res = MineResult(stats=stats or {}, count=nn)
+        if nn and not convert:
+            o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
+            it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
+            s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
+            res.arrays = (it, o, s)
+        elif nn:
             o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
             it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
             s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
@@ -211,11 +218,18 @@ class Context:
         return L.MineArgs(int(min_count), int(flags), float(rel_thr), int(root_topk), float(deadline_s))
 
     def mine(self, min_count: int, flags: int = 0, rel_thr: float = 0.0, root_topk: int = 0,
-             deadline_s: float = 0.0) -> MineResult:
+             deadline_s: float = 0.0, convert: bool = True) -> MineResult:
+        """Exact MFIs; convert=False keeps them as flat arrays (result.arrays)."""
         a = self._args(min_count, flags, rel_thr, root_topk, deadline_s)
         h = C.c_void_p()
         self._check(self._L.sgfim_mine(self._h, C.byref(a), C.byref(h)), "mine")
-        return _take_result(h, self.last_stats())
+        return _take_result(h, self.last_stats(), convert)
+
+    def copy_probe(self, nbytes: int = 1 << 31, iters: int = 5) -> float:
+        """On-device copy bandwidth in GB/s (bytes read + written)."""
+        g = C.c_double()
+        self._check(self._L.sgfim_copy_probe(self._h, int(nbytes), int(iters), C.byref(g)), "copy_probe")
+        return g.value
 
     def last_stats(self) -> dict:
         st = L.RoundStats()

@@ -84,7 +84,7 @@ def main():
         launches = 0
         mfis = []
         for thr in thrs:
-            r = ctx.mine(thr)
+            r = ctx.mine(thr, convert=False)
             st = r.stats
             counts += st["counts"]
             words += st["words"]
@@ -114,6 +114,7 @@ def main():
     rx = ctx.mine(thrs[-1], flags=NO_GRAPH).stats
     event_check = {"device_clock_ms": rx["count_ms"], "hip_event_ms": rx["event_ms"],
                    "launches": rx["count_launches"]}
+    copy_gbs = ctx.copy_probe(1 << 31, 5)   # on-box HBM copy rate, 2 GiB buffers (8x the MALL)
     value = tot_counts / dt
     avg_ms = tot_cms / max(tot_launch, 1)
     bytes_per_launch = 8.0 * tot_words / max(tot_launch, 1)
@@ -136,7 +137,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_count_search", "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": bytes_per_launch, "timing_check": event_check},
+                     "alg_bytes_per_launch": bytes_per_launch, "timing_check": event_check,
+                     "copy_probe_gbs": copy_gbs},
     }
     ctx.close()
     if not args.no_cpu:

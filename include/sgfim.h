@@ -184,6 +184,12 @@ int sgfim_result_windows(const sgfim_result* r, const int32_t** win);
 int sgfim_result_times(const sgfim_result* r, const double** t);
 void sgfim_result_free(sgfim_result* r);
 
+/* On-device copy bandwidth: `iters` copies of `bytes` between two buffers with a plain
+ * 16-byte-load/store kernel, timed with HIP events; gbps = (read + written bytes) / s.
+ * The reference number for the HBM roofline on this box, and the known-size kernel for
+ * calibrating hardware byte counters. No reference counterpart. */
+int sgfim_copy_probe(sgfim_ctx* ctx, int64_t bytes, int32_t iters, double* gbps);
+
 /* Cumulative search statistics of the last sgfim_mine / anytime run. */
 int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
 

@@ -123,3 +123,28 @@ def test_batch_proto_kat_oracle_reproduces_published_levels():
         if len(s) > 2:
             for i in range(len(s)):
                 assert s[:i] + s[i + 1:] in fs
+
+
+def test_oracle_on_equivalence_heavy_data():
+    """Items present everywhere, copied columns and nested tidsets: the DFS oracle, the C
+    oracle and brute force agree (these are the cases the device search prunes)."""
+    rng = random.Random(9)
+    rows = []
+    for _ in range(150):
+        t = {i for i in range(1, 11) if rng.random() < 0.5}
+        t.add(4)
+        t.discard(7)
+        if 2 in t:
+            t.add(7)
+        if 5 in t:
+            t.add(8)
+        rows.append(tuple(sorted(t)))
+    off, items = _dense(rows, 10)
+    R = cref.bitmap(off, items, 10)
+    for thr in (8, 30, 70):
+        bf = O.brute_force_mfi(rows, thr)
+        assert {frozenset(p): c for p, c in O.mine_mfi(rows, min_count=thr)[0]} == bf
+        sup = cref.row_supports(R)
+        keep = np.where(sup >= thr)[0]
+        out, _ = cref.mine(R[keep], thr, count_mode=cref.COUNT_ALL)
+        assert {frozenset(int(keep[i]) + 1 for i in x): s for x, s in out} == bf
