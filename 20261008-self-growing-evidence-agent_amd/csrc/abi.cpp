@@ -605,6 +605,143 @@ int sgfim_load_windows(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, 
     return SGFIM_OK;
 }
 
+namespace {
+uint64_t zmix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+// Item probabilities q_i = min(1, c i^-s), sum = mean_len (bisection on c), and the
+// planted itemsets (sizes, items, rates) of the synthetic Zipf set, from a splitmix64
+// stream seeded with seed ^ 0x243F6A8885A308D3.
+int sgfim_zipf_spec(int32_t n_items, double zipf_s, double mean_len, int32_t n_planted, int32_t size_lo,
+                    int32_t size_hi, double rate_lo, double rate_hi, uint64_t seed, double* q, int32_t* planted_off,
+                    int32_t* planted_items, double* planted_rate)
+{
+    if (n_items < 1 || mean_len <= 0 || mean_len > n_items || n_planted < 0 || size_lo < 1 || size_hi < size_lo ||
+        size_hi > n_items || rate_lo < 0 || rate_hi < rate_lo || rate_hi > 1)
+        return SGFIM_E_INVAL;
+    double lo = 0.0, hi = 1.0;
+    auto total = [&](double cc) {
+        double t = 0.0;
+        for (int32_t i = 1; i <= n_items; i++) t += std::min(1.0, cc * std::pow((double)i, -zipf_s));
+        return t;
+    };
+    while (total(hi) < mean_len) hi *= 2.0;
+    for (int it = 0; it < 200; it++) {
+        const double m = 0.5 * (lo + hi);
+        if (total(m) < mean_len) lo = m; else hi = m;
+    }
+    for (int32_t i = 1; i <= n_items; i++) q[i - 1] = std::min(1.0, hi * std::pow((double)i, -zipf_s));
+    uint64_t st = seed ^ 0x243F6A8885A308D3ull;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        return zmix64(st);
+    };
+    planted_off[0] = 0;
+    for (int32_t j = 0; j < n_planted; j++) {
+        const int32_t sz = size_lo + (int32_t)(next() % (uint64_t)(size_hi - size_lo + 1));
+        int32_t k = planted_off[j];
+        while (k - planted_off[j] < sz) {
+            const int32_t x = 1 + (int32_t)(next() % (uint64_t)n_items);
+            bool dup = false;
+            for (int32_t a = planted_off[j]; a < k; a++) dup = dup || planted_items[a] == x;
+            if (!dup) planted_items[k++] = x;
+        }
+        planted_off[j + 1] = k;
+        planted_rate[j] = rate_lo + (rate_hi - rate_lo) * ((double)(next() >> 11) * (1.0 / 9007199254740992.0));
+    }
+    return SGFIM_OK;
+}
+
+// Generate the synthetic Zipf set straight into the bitmaps (ids 1..n_items, all kept).
+int sgfim_load_zipf(sgfim_ctx* c, int64_t n_txn, int32_t n_items, double zipf_s, double mean_len, int32_t n_planted,
+                    int32_t size_lo, int32_t size_hi, double rate_lo, double rate_hi, uint64_t seed)
+{
+    (void)hipGetLastError();
+    if (!c) return SGFIM_E_INVAL;
+    if (n_txn < 1 || n_txn >= ((int64_t)1 << 31) || n_items < 1 || n_items > 65535 || n_planted > 4096)
+        return fail(c, SGFIM_E_INVAL, "bad synthetic set size");
+    std::vector<double> q((size_t)n_items), rate((size_t)std::max(n_planted, 1));
+    std::vector<int32_t> poff((size_t)n_planted + 1), pit((size_t)std::max(n_planted * size_hi, 1));
+    if (sgfim_zipf_spec(n_items, zipf_s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed, q.data(),
+                        poff.data(), pit.data(), rate.data()) != SGFIM_OK)
+        return fail(c, SGFIM_E_INVAL, "bad synthetic set parameters");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    data_free(c);
+    const double two53 = 9007199254740992.0;
+    std::vector<uint64_t> q53((size_t)n_items), r53((size_t)std::max(n_planted, 1));
+    for (int32_t i = 0; i < n_items; i++) q53[i] = (uint64_t)(q[i] * two53);
+    for (int32_t j = 0; j < n_planted; j++) r53[j] = (uint64_t)(rate[j] * two53);
+    // per item: the planted sets holding it
+    std::vector<std::vector<int32_t>> by_item((size_t)n_items);
+    for (int32_t j = 0; j < n_planted; j++)
+        for (int32_t a = poff[j]; a < poff[j + 1]; a++) by_item[pit[a] - 1].push_back(j);
+    std::vector<int32_t> isoff((size_t)n_items + 1, 0), isl;
+    for (int32_t i = 0; i < n_items; i++) {
+        isl.insert(isl.end(), by_item[i].begin(), by_item[i].end());
+        isoff[i + 1] = (int32_t)isl.size();
+    }
+    if (isl.empty()) isl.push_back(0);
+    c->n_txn = n_txn;
+    c->d = n_items;
+    c->ids.resize(n_items);
+    for (int32_t i = 0; i < n_items; i++) c->ids[i] = i + 1;
+    c->dpad = std::max(128, (c->d + 127) / 128 * 128);
+    c->nseg = 1;
+    c->segs.assign(1, SgSegment{});
+    SgSegment& g = c->segs[0];
+    g.n = (int32_t)n_txn;
+    g.words = (n_txn + 63) / 64;
+    g.stride = std::max<int64_t>(64, (g.words + 63) / 64 * 64);
+    g.d = c->d;
+    g.dpad = c->dpad;
+    g.dw = (c->d + 63) / 64;
+    g.thr = 1;
+    c->W = g.words;
+    c->Wp = g.stride;
+    c->rows_words = (int64_t)c->dpad * g.stride;
+    int rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_rows, (size_t)c->rows_words))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_sup, (size_t)c->dpad))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_rows, 0, (size_t)c->rows_words * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_sup, 0, (size_t)c->dpad * 4, c->stream));
+    std::vector<DevBuf> tmp;
+    uint64_t *d_q = nullptr, *d_r = nullptr;
+    int32_t *d_isoff = nullptr, *d_isl = nullptr;
+    if ((rc = dev_alloc_t(c, tmp, &d_q, q53.size())) || (rc = dev_alloc_t(c, tmp, &d_r, r53.size())) ||
+        (rc = dev_alloc_t(c, tmp, &d_isoff, isoff.size())) || (rc = dev_alloc_t(c, tmp, &d_isl, isl.size()))) {
+        dev_free_all(c, tmp);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_q, q53.data(), q53.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_r, r53.data(), r53.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_isoff, isoff.data(), isoff.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_isl, isl.data(), isl.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = sg_launch_gen_zipf(c->d_rows, g.stride, n_txn, c->d, d_q, d_r, d_isoff, d_isl, seed, c->d_sup, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free_all(c, tmp);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "synthetic set generation failed: %s", hipGetErrorString(e));
+    std::vector<int32_t> sup32((size_t)c->dpad);
+    HIPCHK(c, hipMemcpyAsync(sup32.data(), c->d_sup, sup32.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_sup_seg.assign(sup32.begin(), sup32.begin() + c->d);
+    c->h_sup = c->h_sup_seg;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg, 2))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg_item_base, 1))) return rc;
+    const int64_t zero = 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), sizeof(SgSegment), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->loaded = true;
+    return SGFIM_OK;
+}
+
 int sgfim_load_csr(sgfim_ctx* c, const int64_t* off, int64_t n_txn, const int32_t* items, int64_t nnz)
 {
     const int64_t w[2] = {0, n_txn};

@@ -51,6 +51,9 @@ hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const i
                                int64_t n_txn, int32_t id_max, uint64_t* rows, int32_t d,
                                int32_t dpad, int64_t Wp, int32_t* supports, hipStream_t s);
 
+hipError_t sg_launch_gen_zipf(uint64_t* rows, int64_t Wp, int64_t n_txn, int32_t d, const uint64_t* q53,
+                              const uint64_t* r53, const int32_t* item_sets_off, const int32_t* item_sets,
+                              uint64_t seed, int32_t* sup, hipStream_t s);
 hipError_t sg_launch_copy_probe(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,

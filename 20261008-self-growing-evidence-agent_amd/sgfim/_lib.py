@@ -62,6 +62,10 @@ _SIGS = {
     "sgfim_window_supports": (C.c_int, [P, C.c_int32, P]),
     "sgfim_set_window_thresholds": (C.c_int, [P, P, C.c_int32]),
     "sgfim_result_windows": (C.c_int, [P, PP]),
+    "sgfim_load_zipf": (C.c_int, [P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                  C.c_int32, C.c_double, C.c_double, C.c_uint64]),
+    "sgfim_zipf_spec": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                  C.c_double, C.c_uint64, P, P, P, P]),
     "sgfim_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "sgfim_item_ids": (C.c_int, [P, P]),
     "sgfim_item_supports": (C.c_int, [P, P]),

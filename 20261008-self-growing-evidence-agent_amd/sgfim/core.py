@@ -147,6 +147,19 @@ class Context:
         self.n_txn, self.n_items, self.words = nt.value, ni.value, w.value
         self.n_windows = int(wo.shape[0]) - 1
 
+    def load_zipf(self, n_txn: int, n_items: int = 2048, zipf_s: float = 1.1, mean_len: float = 60.0,
+                  n_planted: int = 40, size_lo: int = 4, size_hi: int = 12, rate_lo: float = 0.005,
+                  rate_hi: float = 0.05, seed: int = 0):
+        """Generate the synthetic dense-sparse set on the GPU (see sgfim_load_zipf)."""
+        self._check(self._L.sgfim_load_zipf(self._h, int(n_txn), int(n_items), float(zipf_s), float(mean_len),
+                                            int(n_planted), int(size_lo), int(size_hi), float(rate_lo),
+                                            float(rate_hi), int(seed)), "load_zipf")
+        nt = C.c_int64()
+        ni = C.c_int32()
+        w = C.c_int64()
+        self._L.sgfim_info(self._h, C.byref(nt), C.byref(ni), C.byref(w))
+        self.n_txn, self.n_items, self.words = nt.value, ni.value, w.value
+
     def window_supports(self, w: int) -> np.ndarray:
         out = np.zeros(max(self.n_items, 1), dtype=np.int64)
         self._check(self._L.sgfim_window_supports(self._h, int(w), _p(out)), "window_supports")

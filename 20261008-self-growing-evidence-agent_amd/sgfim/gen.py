@@ -142,3 +142,25 @@ def concat_windows(windows):
         n_items = max(n_items, T.n_items)
     items = np.concatenate(its) if its else np.zeros(0, dtype=np.int32)
     return win_off, np.concatenate(offs), items.astype(np.int32), n_items
+
+
+WORKLOAD5 = dict(n_txn=10_000_000, n_items=2048, zipf_s=1.1, mean_len=60.0, n_planted=40, size_lo=4,
+                 size_hi=12, rate_lo=0.005, rate_hi=0.05)
+
+
+def zipf_spec(n_items=2048, zipf_s=1.1, mean_len=60.0, n_planted=40, size_lo=4, size_hi=12,
+              rate_lo=0.005, rate_hi=0.05, seed=0):
+    """(q, planted_off, planted_items, planted_rate) of the synthetic Zipf set, from the library
+    (host code: item probabilities and planted itemsets; the bitmaps are drawn on the GPU)."""
+    import ctypes as C
+    from . import _lib
+    q = np.zeros(n_items, dtype=np.float64)
+    poff = np.zeros(n_planted + 1, dtype=np.int32)
+    pit = np.zeros(max(n_planted * size_hi, 1), dtype=np.int32)
+    pr = np.zeros(max(n_planted, 1), dtype=np.float64)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = _lib.lib().sgfim_zipf_spec(n_items, zipf_s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi,
+                                    seed, P(q), P(poff), P(pit), P(pr))
+    if rc != 0:
+        raise ValueError("bad synthetic set parameters")
+    return q, poff, pit[: poff[-1]], pr[:n_planted]

@@ -50,6 +50,89 @@ def cpu_baseline(T, budget_s: float = 20.0):
                       f"(count-all reference search, {len(out)} MFIs)"}
 
 
+def _gen_w3(args):
+    from sgfim import gen
+    seed, w, n = args
+    T = gen.sensor_stream(seed * 100_003 + w + 1, int(n), 32, 6, 4)
+    return T.offsets, T.items
+
+
+def workload3_run(Context, seed=0, workers=16):
+    """512 windows, lengths log-uniform in [2,000, 200,000], 32 sensors x 6 bins, 3% per window:
+    one search over all windows (every counting launch mixes windows)."""
+    import numpy as np
+    from concurrent.futures import ProcessPoolExecutor
+    from sgfim import gen
+    lens = gen.window_lengths(seed, 512)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(_gen_w3, [(seed, w, n) for w, n in enumerate(lens)]))
+    win_off = np.zeros(len(parts) + 1, dtype=np.int64)
+    offs, its, base = [np.zeros(1, dtype=np.int64)], [], 0
+    for w, (o, it) in enumerate(parts):
+        win_off[w + 1] = win_off[w] + o.shape[0] - 1
+        offs.append(o[1:] + base)
+        its.append(it)
+        base += it.shape[0]
+    off = np.concatenate(offs)
+    items = np.concatenate(its)
+    log(f"workload 3 generated: {len(lens)} windows, {int(win_off[-1])} txns in {time.perf_counter() - t0:.1f} s")
+    ctx = Context(device=0)
+    ctx.load_windows(win_off, off, items)
+    thr = [gen.min_count(0.03, int(n)) for n in lens]
+    ctx.mine_windows(thr)                                  # warm
+    t0 = time.perf_counter()
+    r = ctx.mine_windows(thr)
+    dt = time.perf_counter() - t0
+    st = r.stats
+    ctx.close()
+    counts = st["counts"] + st["pair_counts"]
+    return {"workload": "workload 3: 512 windows, 2k-200k txns, 32 sensors x 6 bins, 3% per window",
+            "seconds": dt, "mfis": len(r), "counts": counts, "counts_per_s": counts / dt,
+            "count_kernel_ms": st["count_ms"], "launches": st["count_launches"],
+            "alg_gbs": 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None}
+
+
+def workload5_run(Context, budget_s=8.0, seed=0):
+    """10M txns x 2,048 Zipf items (2.56 GB of bitmaps, 10x the MALL): a budgeted anytime run
+    down to the 0.5% floor; reports the counting kernel against the HBM roofline."""
+    from sgfim import gen
+    ctx = Context(device=0)
+    t0 = time.perf_counter()
+    ctx.load_zipf(seed=seed, **gen.WORKLOAD5)
+    log(f"workload 5 generated on the GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words})")
+    thr = gen.min_count(0.005, ctx.n_txn)
+    t0 = time.perf_counter()
+    ctx.anytime_begin(thr)
+    mfis = 0
+    while True:
+        st = ctx.anytime_step(1)
+        mfis += len(ctx.anytime_take())
+        log(f"workload 5 round {st['round']}: {st['counts']} counts, {mfis} MFIs, frontier {st['frontier']}")
+        if st["done"] or time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    ctx.close()
+    alg = 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None
+    return {"workload": "workload 5: 10M txns x 2048 Zipf(1.1) items, mean length 60, 40 planted sets, "
+                        "floor 0.5%, budgeted anytime run",
+            "seconds": dt, "rounds": st["round"], "done": bool(st["done"]), "mfis_confirmed": mfis,
+            "counts": st["counts"] + st["pair_counts"], "counts_per_s": (st["counts"] + st["pair_counts"]) / dt,
+            "count_kernel_ms": st["count_ms"], "launches": st["count_launches"],
+            "alg_gbs": alg, "alg_frac_of_hbm_peak": alg / HBM_PEAK_GBS if alg else None}
+
+
+def cpu_extra(budget_s=20.0):
+    """Single-thread restatements of the reference's Python miner on workload 1 (10k txns)."""
+    from oracle import mfi_oracle as O
+    from sgfim import gen
+    T = gen.workload1(0)
+    rows = T.rows()
+    t0 = time.perf_counter()
+    out, _ = O.mine_mfi(rows, min_count=gen.min_count(0.02, T.n_txn), node_cap=0)
+    return {"pure_python_mine_mfi_workload1_2pct_s": time.perf_counter() - t0, "mfis": len(out), "cores": 1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +141,8 @@ def main():
     ap.add_argument("--n-txn", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the workload 3 and workload 5 measurements")
+    ap.add_argument("--w5-budget", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -141,9 +226,13 @@ def main():
                      "copy_probe_gbs": copy_gbs},
     }
     ctx.close()
+    if not args.no_extra:
+        out["extra"] = {"workload5": workload5_run(Context, args.w5_budget),
+                        "workload3": workload3_run(Context)}
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(T)
+        out.setdefault("extra", {})["cpu_python"] = cpu_extra()
     print(json.dumps(out), flush=True)
 
 

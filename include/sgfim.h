@@ -127,6 +127,21 @@ int sgfim_window_supports(sgfim_ctx* ctx, int32_t window, int64_t* out);
  * sgfim_mine_args.min_count then applies to every window). */
 int sgfim_set_window_thresholds(sgfim_ctx* ctx, const int64_t* min_counts, int32_t n);
 
+/* Synthetic dense-sparse set generated on the GPU straight into the bitmaps: item i
+ * (1..n_items) is present independently with probability q_i = min(1, c i^-zipf_s),
+ * c set so that sum q_i = mean_len; n_planted itemsets of size_lo..size_hi items are
+ * ORed in at rates drawn in [rate_lo, rate_hi]. All draws are counter-based hashes of
+ * (seed, transaction, item or set), so any transaction can be regenerated on the CPU.
+ * No reference counterpart (the reference's inputs are files). */
+int sgfim_load_zipf(sgfim_ctx* ctx, int64_t n_txn, int32_t n_items, double zipf_s, double mean_len,
+                    int32_t n_planted, int32_t size_lo, int32_t size_hi, double rate_lo, double rate_hi,
+                    uint64_t seed);
+/* The parameters it derives: q [n_items], planted_off [n_planted + 1], planted_items
+ * [n_planted * size_hi], planted_rate [n_planted]. */
+int sgfim_zipf_spec(int32_t n_items, double zipf_s, double mean_len, int32_t n_planted, int32_t size_lo,
+                    int32_t size_hi, double rate_lo, double rate_hi, uint64_t seed, double* q,
+                    int32_t* planted_off, int32_t* planted_items, double* planted_rate);
+
 /* Number of transactions and the distinct item ids of the loaded set
  * (ascending; the library's dense id of ids[k] is k). */
 int sgfim_info(const sgfim_ctx* ctx, int64_t* n_txn, int32_t* n_items, int64_t* words);

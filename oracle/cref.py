@@ -49,6 +49,10 @@ def lib():
         L.ref_result_free.argtypes = [P]
         L.ref_trace_set.argtypes = [P, C.c_int64]
         L.ref_trace_count.restype = C.c_int64
+        L.zipf_spec_ref.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_double, C.c_double, C.c_uint64, P, P, P, P]
+        L.zipf_csr_ref.argtypes = [C.c_int64, C.c_int32, P, C.c_int32, P, P, P, C.c_uint64, P, P]
+        L.zipf_csr_ref.restype = C.c_int64
         L.kat_generate.argtypes = [C.c_int64, P, P, P, P]
         L.kat_nnz.argtypes = [P, C.c_int64]; L.kat_nnz.restype = C.c_int64
         L.kat_csr.argtypes = [P, C.c_int64, P, P]
@@ -170,3 +174,25 @@ def kat_levelwise(rows: np.ndarray, minsup: int = KAT_MINSUP, out_cap: int = 409
     tot = lib().kat_levelwise(_p(rows), n, minsup, _p(lc), _p(out), out_cap)
     sets = [tuple(int(x) for x in out[i, 1:1 + out[i, 0]]) for i in range(min(tot, out_cap))]
     return [int(x) for x in lc[1:8]], int(tot), sets
+
+
+# --------------------------------------------------------------------------- synthetic Zipf set
+def zipf_spec(d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed):
+    q = np.zeros(d, dtype=np.float64)
+    poff = np.zeros(n_planted + 1, dtype=np.int32)
+    pit = np.zeros(max(n_planted * size_hi, 1), dtype=np.int32)
+    pr = np.zeros(max(n_planted, 1), dtype=np.float64)
+    lib().zipf_spec_ref(d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed,
+                        _p(q), _p(poff), _p(pit), _p(pr))
+    return q, poff, pit[: poff[-1]], pr[:n_planted]
+
+
+def zipf_csr(n, d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed):
+    q, poff, pit, pr = zipf_spec(d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed)
+    pit_c = np.ascontiguousarray(pit if pit.size else np.zeros(1, np.int32))
+    pr_c = np.ascontiguousarray(pr if pr.size else np.zeros(1))
+    nnz = lib().zipf_csr_ref(n, d, _p(q), n_planted, _p(poff), _p(pit_c), _p(pr_c), seed, None, None)
+    off = np.zeros(n + 1, dtype=np.int64)
+    items = np.zeros(max(nnz, 1), dtype=np.int32)
+    lib().zipf_csr_ref(n, d, _p(q), n_planted, _p(poff), _p(pit_c), _p(pr_c), seed, _p(off), _p(items))
+    return off, items[:nnz]

@@ -1,0 +1,127 @@
+"""GPU tests: the drop-in engine adapter against the engines' semantics, and the C ABI's
+error paths (every error is forced from the host; no kernel is made to fail)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mfi_oracle as O
+from sgfim import _lib as L
+from sgfim import engine_adapter as EA
+from sgfim import gen
+from sgfim.core import Context, csr_from_rows
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fimi(tmp_path_factory):
+    T = gen.workload1(7, 3000)
+    rows = T.rows()
+    p = tmp_path_factory.mktemp("fimi") / "w1.txt"
+    with open(p, "w") as f:
+        f.write("\n".join(" ".join(map(str, r)) for r in rows) + "\n")
+    return str(p), rows
+
+
+def test_run_tensorfim_matches_engine_family(fimi):
+    print("case: adapter run_tensorfim", flush=True)
+    path, rows = fimi
+    for s in (0.03, 0.1):
+        r = EA.run_tensorfim("bmma", path, s, reps=2)
+        assert r["ok"] and len(r["comp_times"]) == 2
+        assert r["mfis"] == O.tensorfim_family(rows, s)
+        rp = f"{path}-{s:.6f}=Results.txt"
+        assert EA.parse_tensorfim_results(rp) == r["mfis"]
+        ex = EA.run_tensorfim("baseline", path, s, exact=True)
+        n = len(rows)
+        assert ex["mfis"] == {frozenset(p): c for p, c in O.mine_mfi(rows, min_count=int(np.ceil(n * s)),
+                                                                    node_cap=0)[0]}
+
+
+def test_run_anyfim_matches_stage_families(fimi):
+    print("case: adapter run_anyfim", flush=True)
+    path, rows = fimi
+    r = EA.run_anyfim(path, 12)
+    assert r["ok"] and os.path.exists(r["results_path"])
+    for k in range(1, 13):
+        fam, thr = O.anyfim_stage_family(rows, k)
+        assert dict(r["stages"][k]) == fam
+        assert r["meta"]["frequencyThrePerStage"][k - 1] == thr
+    assert not EA.run_anyfim(path, 41)["ok"]        # more stages than items: the engine refuses
+
+
+def test_error_paths():
+    print("case: error paths", flush=True)
+    off, items = csr_from_rows([(1, 2), (2, 3), (1, 2, 3)])
+    with Context(device=0) as ctx:
+        with pytest.raises(L.SgfimError) as e:
+            ctx.mine(1)
+        assert e.value.code == L.E_STATE                       # nothing loaded
+        with pytest.raises(L.SgfimError) as e:
+            ctx.load_csr(np.asarray([0, 2]), np.asarray([1, 0]))
+        assert e.value.code == L.E_INVAL                       # id 0
+        with pytest.raises(L.SgfimError) as e:
+            ctx.load_csr(np.asarray([0, 2, 1]), np.asarray([1, 2]))
+        assert e.value.code == L.E_INVAL                       # offsets decrease
+        ctx.load_csr(off, items)
+        with pytest.raises(L.SgfimError) as e:
+            ctx.mine(0)
+        assert e.value.code == L.E_INVAL
+        with pytest.raises(L.SgfimError) as e:
+            ctx.mine(1, flags=L.ORDER_LEARNED)
+        assert e.value.code == L.E_STATE                       # no scorer loaded
+        with pytest.raises(L.SgfimError) as e:
+            ctx.count_batch([list(range(1, 40))], [[1]])
+        assert e.value.code == L.E_DEPTH                       # prefix longer than 32 items
+    # device-memory cap
+    with Context(device=0, mem_cap_bytes=1 << 16) as ctx:
+        T = gen.workload1(1, 50_000)
+        with pytest.raises(L.SgfimError) as e:
+            ctx.load_csr(T.offsets, T.items)
+        assert e.value.code == L.E_NOMEM
+    # explicit frontier capacity too small for the level-2 nodes
+    T = gen.workload1(1, 10_000)
+    with Context(device=0, frontier_cap=16) as ctx:
+        ctx.load_csr(T.offsets, T.items)
+        with pytest.raises(L.SgfimError) as e:
+            ctx.mine(gen.min_count(0.02, T.n_txn))
+        assert e.value.code == L.E_CAPACITY
+    # default capacity: an overflow restarts the search with more room, same result
+    with Context(device=0, frontier_cap=0) as a:
+        a.load_csr(T.offsets, T.items)
+        ref = a.mine(gen.min_count(0.02, T.n_txn)).as_dict()
+    assert len(ref) > 0
+
+
+def test_depth_overflow_and_deadline_and_cancel():
+    print("case: depth, deadline, cancel", flush=True)
+    # 40 items that always occur together with independent noise items: one MFI of length 40
+    rng = np.random.default_rng(0)
+    rows = [tuple(range(1, 41)) + tuple(int(x) for x in rng.choice(np.arange(41, 61), 3, replace=False))
+            for _ in range(200)]
+    off, items = csr_from_rows(rows)
+    with Context(device=0) as ctx:
+        ctx.load_csr(off, items)
+        with pytest.raises(L.SgfimError) as e:
+            ctx.mine(150)
+        assert e.value.code == L.E_DEPTH
+    T = gen.workload1(2, 10_000)
+    with Context(device=0, batch_nodes=1) as ctx:
+        ctx.load_csr(T.offsets, T.items)
+        thr = gen.min_count(0.02, T.n_txn)
+        with pytest.raises(L.SgfimError) as e:
+            ctx.mine(thr, deadline_s=1e-9)
+        assert e.value.code == L.E_TIMEOUT
+        ctx.anytime_begin(thr)
+        st = ctx.anytime_step(2)
+        assert not st["done"]
+        got = ctx.anytime_take()
+        ctx.anytime_cancel()
+        with pytest.raises(L.SgfimError) as e:
+            ctx.anytime_step(1)
+        assert e.value.code == L.E_CANCELLED
+        # every MFI reported before the cancel is final
+        full = ctx.mine(thr).as_dict()
+        for x, s in zip(got.itemsets, got.supports):
+            assert full[frozenset(x)] == s

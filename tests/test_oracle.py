@@ -148,3 +148,26 @@ def test_oracle_on_equivalence_heavy_data():
         keep = np.where(sup >= thr)[0]
         out, _ = cref.mine(R[keep], thr, count_mode=cref.COUNT_ALL)
         assert {frozenset(int(keep[i]) + 1 for i in x): s for x, s in out} == bf
+
+
+def test_zipf_spec_library_matches_oracle():
+    """The synthetic Zipf set's parameters: the library's host code and the oracle's
+    restatement agree exactly, and q sums to the mean length."""
+    for args in [(2048, 1.1, 60.0, 40, 4, 12, 0.005, 0.05, 0), (300, 1.1, 20.0, 7, 3, 6, 0.01, 0.1, 9)]:
+        q, poff, pit, pr = gen.zipf_spec(*args)
+        q2, poff2, pit2, pr2 = cref.zipf_spec(*args)
+        assert np.array_equal(q, q2) and np.array_equal(poff, poff2) and np.array_equal(pit, pit2)
+        assert np.array_equal(pr, pr2)
+        assert abs(q.sum() - args[2]) < 1e-9 * args[2]
+        assert (q <= 1.0).all() and (np.diff(q) <= 0).all()
+        assert ((pr >= args[6]) & (pr <= args[7])).all()
+        sizes = np.diff(poff)
+        assert ((sizes >= args[4]) & (sizes <= args[5])).all()
+        for j in range(args[3]):
+            s = pit[poff[j]:poff[j + 1]]
+            assert len(set(s.tolist())) == len(s) and s.min() >= 1 and s.max() <= args[0]
+    # the CSR twin: mean length = target + what the planted sets add (less overlaps)
+    off, items = cref.zipf_csr(4000, 300, 1.1, 20.0, 7, 3, 6, 0.01, 0.1, 9)
+    q, poff, pit, pr = cref.zipf_spec(300, 1.1, 20.0, 7, 3, 6, 0.01, 0.1, 9)
+    extra = float((np.diff(poff) * pr).sum())
+    assert 20.0 - 0.5 < np.diff(off).mean() < 20.0 + extra + 0.5
