@@ -82,6 +82,7 @@ struct sgfim_ctx {
     int64_t mem_cap = 0;
     int64_t mem_used = 0;
     int wall_khz = 0;                   // wall_clock64() rate
+    int n_cu = 256;                     // compute units of the device
     sgfim_opts opts{};
     // loaded data
     bool loaded = false;
@@ -222,6 +223,14 @@ int32_t pick_chunk(const sgfim_ctx* c, int64_t W)
     if (W <= 256) return 256;
     if (W <= 4096) return 512;
     return 1024;
+}
+
+// Row-chunk words of one counting tile of the search (the tile kernel holds 17 chunks in LDS;
+// 256 words keep three 512-thread workgroups resident per CU).
+int32_t search_chunk(const sgfim_ctx* c, int64_t W)
+{
+    (void)W;
+    return c->opts.chunk_words == 512 ? 512 : 256;
 }
 
 int ensure_pair(sgfim_ctx* c)
@@ -368,14 +377,14 @@ int launch_round(sgfim_ctx* c, SearchState* S)
 
 // (Re)use the context's search buffers for a run needing the given sizes.
 int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap, int32_t d, int32_t dw,
-                int32_t bmax, int32_t chunk, int64_t trace_cap, int64_t n_roots)
+                int32_t bmax, int32_t chunk, int64_t trace_cap, int64_t n_roots, int64_t cell_cap)
 {
     SearchBufs* B = c->SB;
     if (B) {
         const SgSearchDev& D = B->D;
         const bool fits = D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
                           D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.trace_cap == trace_cap &&
-                          B->roots_cap >= n_roots;
+                          B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + D.newborn_cap;
         if (fits) return SGFIM_OK;
         bufs_free(c);
     }
@@ -389,8 +398,9 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     D.dw_max = dw;
     D.bmax = bmax;
     D.chunk_words = chunk;
-    D.count_grid = 2048;
+    D.count_grid = sg_count_group_blocks_per_cu(chunk) * c->n_cu;   // one persistent wave of workgroups
     D.trace_cap = trace_cap;
+    D.cell_cap = cell_cap;
     B->roots_cap = std::max<int64_t>(n_roots, 1);
     auto& V = B->bufs;
     int rc;
@@ -400,6 +410,8 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.bucket, (size_t)cap * SG_NBUCKETS)) ||
         (rc = dev_alloc_t(c, V, &D.batch, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.coff, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &D.unit_off, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.choff, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.celloff, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.cells, (size_t)cell_cap)) ||
         (rc = dev_alloc_t(c, V, &D.clist, (size_t)bmax * d)) || (rc = dev_alloc_t(c, V, &D.ccount, (size_t)bmax * d)) ||
         (rc = dev_alloc_t(c, V, &D.newborn, (size_t)newborn_cap)) ||
         (rc = dev_alloc_t(c, V, &D.feat, (size_t)newborn_cap * SG_FEAT)) ||
@@ -479,6 +491,7 @@ int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out)
         (void)hipGetLastError();
         c->wall_khz = 0;
     }
+    if (prop.multiProcessorCount > 0) c->n_cu = prop.multiProcessorCount;
     *out = c;
     return SGFIM_OK;
 }
@@ -982,42 +995,33 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
                              c->stream));
     if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * nseg))) return rc;
     HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, nseg, d, c->d_ok, c->stream));
-    // level-2 nodes the seed creates: frequent pairs (r, e) with e < r, r a root. For
-    // one set the bound sum(r) over roots avoids a device round trip; for windows the
-    // exact count comes from the pair_ok masks.
-    int64_t seed_children = 0;
-    if (nseg == 1) {
-        for (int64_t q = 0; q < n_roots; q++) seed_children += rp[2 * q + 1];
-    } else {
-        std::vector<uint64_t> okh((size_t)d * dw * nseg);
-        HIPCHK(c, hipMemcpyAsync(okh.data(), c->d_ok, okh.size() * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int64_t q = 0; q < n_roots; q++) {
-            const int32_t w = rp[2 * q], r = rp[2 * q + 1];
-            const uint64_t* o = okh.data() + c->segs[w].ok_base + (int64_t)r * dw;
-            for (int32_t k = 0; k < dw; k++) {
-                uint64_t v = o[k];
-                if (k > (r >> 6)) v = 0;
-                else if (k == (r >> 6)) v &= (1ull << (r & 63)) - 1ull;
-                seed_children += __builtin_popcountll(v);
-            }
-        }
-    }
+    // the seed turns every root into at most one frontier record
+    const int64_t seed_records = n_roots;
     // search buffers (reused across runs when they fit)
     const int32_t bmax = std::min(SG_BMAX, c->opts.batch_nodes > 0 ? c->opts.batch_nodes : SG_BMAX);
     int64_t cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap
-                                           : std::max<int64_t>({(int64_t)1 << 16, 4 * seed_children + 4 * bmax, c->cap_hint});
+                                           : std::max<int64_t>({(int64_t)1 << 16, 4 * seed_records + 4 * bmax + 4 * (int64_t)d,
+                                                                c->cap_hint});
     if (cap > ((int64_t)1 << 31) - 1) cap = ((int64_t)1 << 31) - 1;
-    if (seed_children > cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld level-2 nodes",
-                                         (long long)cap, (long long)seed_children);
-    const int64_t newborn_cap = std::max<int64_t>(seed_children, std::min<int64_t>(cap, (int64_t)1 << 20));
-    int64_t ring_cap = c->opts.ring_cap > 0 ? c->opts.ring_cap : std::max<int64_t>(1 << 16, n_roots + bmax);
-    if (ring_cap < n_roots + bmax)
-        return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)(n_roots + bmax));
+    if (seed_records > cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld root records",
+                                        (long long)cap, (long long)seed_records);
+    // newborn_cap bounds the children of one round (each makes at most one record and emits
+    // at most one MFI), so the MFI ring holds the seed's MFIs plus one round's
+    int64_t newborn_cap = std::max<int64_t>(seed_records, std::min<int64_t>(cap, (int64_t)1 << 20));
+    // count cells of one round; at least one record's full tile (d x d) so every round advances
+    const int64_t cell_cap = std::max<int64_t>((int64_t)1 << 26, (int64_t)d * d);
+    int64_t ring_cap = n_roots + newborn_cap;
+    if (c->opts.ring_cap > 0) {
+        const int64_t need = n_roots + std::max<int64_t>(n_roots, d);
+        if (c->opts.ring_cap < need)
+            return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)need);
+        ring_cap = c->opts.ring_cap;
+        newborn_cap = std::min(newborn_cap, ring_cap - n_roots);
+    }
     int64_t maxw = 0;
     for (const SgSegment& g : c->segs) maxw = std::max(maxw, g.words);
-    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, nseg == 1 ? pick_chunk(c, c->W) : pick_chunk(c, std::min<int64_t>(maxw, 256)),
-                          c->trace ? c->trace_cap : 0, n_roots)))
+    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, search_chunk(c, maxw),
+                          c->trace ? c->trace_cap : 0, n_roots, cell_cap)))
         return rc;
     S->B = c->SB;
     SgSearchDev& D = S->B->D;
@@ -1134,7 +1138,7 @@ int sgfim_mine(sgfim_ctx* c, const sgfim_mine_args* a, sgfim_result** out)
             rc = sgfim_anytime_step(c, 0, &st);
         }
         // with the default frontier capacity, an overflow restarts the search with 4x the room
-        if (rc == SGFIM_E_CAPACITY && c->opts.frontier_cap == 0 && c->SB && c->SB->D.cap < ((int64_t)1 << 29)) {
+        if (rc == SGFIM_E_CAPACITY && c->opts.frontier_cap == 0 && c->SB && c->SB->D.cap < ((int64_t)1 << 26)) {
             c->cap_hint = 4 * c->SB->D.cap;
             continue;
         }

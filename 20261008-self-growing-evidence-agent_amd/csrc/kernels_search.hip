@@ -5,23 +5,26 @@
 // one counter call per popped node, and the node step of kernel_bitmap.cu:866-890:
 //   emit X iff #{e : res[e] >= thr} == |X|; otherwise push X u {i} for every dense
 //   id i < progress with res[i] >= thr, with frequency res[i] and progress i.
-// Here the same tree is searched with the state in HBM:
-//   arena      node records (items, support, tail bound, candidate mask)
-//   buckets    64 priority stacks of arena indices; a round takes up to bmax nodes
+// Here the same tree is searched with the state in HBM, by sibling groups:
+//   record     an itemset X whose frequent one-item extensions U = F(X) are known and
+//              whose children X u {e}, e in U below the tail bound `last`, are not yet
+//              counted (items, support, U mask)
+//   buckets    64 priority stacks of record indices; a round takes up to bmax records
 //              from the highest non-empty buckets (LIFO inside a bucket)
-//   free list  stack of free arena slots
-//   ring       confirmed MFIs, drained by the host after every round
-// Candidates: a node counts only the items that can still extend it. For the
-// child Y = X u {e} that set is (F(X) \ {e}) & ok[e], where F(X) holds the
-// frequent one-item extensions of X and ok[e] the items whose pair support with e
-// reaches the threshold; by downward closure every frequent extension of Y lies
-// in it, so the frequent-extension count -- and with it the reference's local,
-// exact maximality test -- is unchanged while most infeasible counts are skipped.
-// Roots are processed straight from the pair matrix (their counts are its rows).
+//   free list  stack of free record slots
+//   ring       confirmed MFIs, written to mapped host memory
+// Processing a record counts all its children against all of U at once: the count
+// matrix C[e][f] = |T(X) & T(e) & T(f)| (e a child, f in U) is a tile in which every
+// candidate row chunk is loaded once and ANDed with up to 16 child prefixes held in
+// LDS. By downward closure every frequent extension of X u {e} lies in U, so the
+// reference's local maximality test (frequent-extension count == |X u {e}|) is
+// exact on the tile's row; cells whose pair (e, f) is infrequent are skipped and
+// stay 0 (no superset of an infrequent pair is frequent).
+// Parent-equivalence pruning (exact): see sg_expand_node.
 //
-// One round = k_select -> k_candlist -> k_count_search -> k_expand -> [score]
-// -> k_push -> k_release; every launch has a fixed shape and reads its sizes from
-// the control block, so the whole round can be captured once as a hipGraph.
+// One round = k_select -> k_candlist -> k_zero_cells -> k_count_group ->
+// k_expand_children -> [score] -> k_push -> k_release; every launch has a fixed shape
+// and reads its sizes from the control block, so the round is captured as one hipGraph.
 #include "count_core.h"
 
 #define SEL_THREADS 1024
@@ -87,75 +90,83 @@ __device__ __forceinline__ void sg_features(const SgFeatIn& f, float* out)
     out[23] = 1.0f;
 }
 
-// ------------------------------------------------------------------ child construction
-// Writes child Y = X u {e} into arena slot `slot`. F is the parent's frequent-extension
-// mask (dw words, LDS), items the parent's items (LDS).
-__device__ void sg_make_child(const SgSearchDev& S, const SgSegment& sg, const uint64_t* __restrict__ ok,
-                              const int32_t* __restrict__ item_sup, const int32_t* __restrict__ pair,
-                              int32_t slot, const uint16_t* px, int k, int32_t sup_x, int nf_x,
-                              int nf_x_tail, const uint64_t* F, int32_t e, int32_t sup_y, int seg,
-                              float* feat_out, int64_t key, int64_t parent_key)
+
+// ------------------------------------------------------------------ records
+#define SG_CB 16      // children per counting tile
+#define SG_CR 8       // row chunks per counting unit
+#define SG_DMAX 4096  // most items a search handles (the host refuses more)
+
+__device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
 {
-    const int dw = sg.dw;
+    const int lw = last >> 6, lb = last & 63;
+    return w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
+}
+
+// Counting units of a record: blocks of SG_CB children x ranges of SG_CR row chunks.
+__device__ __forceinline__ int32_t sg_tile_units(int32_t ntail, int64_t words, int32_t chunk)
+{
+    const int64_t nchunk = (words + chunk - 1) / chunk;
+    return (int32_t)(((ntail + SG_CB - 1) / SG_CB) * ((nchunk + SG_CR - 1) / SG_CR));
+}
+
+// Write record X (items px[0..k), support, tail bound `last`, extensions U in LDS) to slot.
+__device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const int32_t* __restrict__ item_sup,
+                               const int32_t* __restrict__ pair, int32_t slot, const uint16_t* px, int k,
+                               int32_t sup, int32_t last, int seg, const uint64_t* U, int32_t sup_parent,
+                               int32_t nf_parent, int32_t ntail_parent, int parent_is_root, float* feat_out,
+                               int64_t key, int64_t parent_key)
+{
     uint64_t* cm = S.cand + (int64_t)slot * S.dw_max;
-    const uint64_t* oke = ok + sg.ok_base + (int64_t)e * dw;
-    int ncand = 0, tail = 0;
-    const int ew = e >> 6, eb = e & 63;
-    for (int w = 0; w < dw; w++) {
-        uint64_t v = F[w] & oke[w];
-        if (w == ew) v &= ~(1ull << eb);
+    int nu = 0, nt = 0;
+    for (int w = 0; w < sg.dw; w++) {
+        const uint64_t v = U[w];
         cm[w] = v;
-        const int c = __popcll(v);
-        ncand += c;
-        if (w < ew) tail += c;
-        else if (w == ew) tail += __popcll(v & ((1ull << eb) - 1ull));
+        nu += __popcll(v);
+        nt += __popcll(v & sg_low_mask(w, last));
     }
     uint16_t* ci = S.items + (int64_t)slot * SG_MAX_DEPTH;
-    int32_t mn = 0, mx = 0, pmin = 0x7fffffff;
-    float sl = 0.0f;
-    if (feat_out) {
-        mn = item_sup[e];
-        mx = item_sup[e];
-        sl = __log2f((float)item_sup[e]);
-        const int32_t* pe = pair + sg.pair_base + (int64_t)e * sg.dpad;
-        for (int i = 0; i < k; i++) {
-            const int32_t sx = item_sup[px[i]];
-            mn = min(mn, sx);
-            mx = max(mx, sx);
-            sl += __log2f((float)sx);
-            pmin = min(pmin, pe[px[i]]);
-        }
-    }
     for (int i = 0; i < k; i++) ci[i] = px[i];
-    ci[k] = (uint16_t)e;
     SgNodeMeta m;
-    m.k = k + 1;
-    m.sup = sup_y;
-    m.last = e;
+    m.k = k;
+    m.sup = sup;
+    m.last = last;
     m.seg = seg;
-    m.parent_sup = sup_x;
-    m.parent_nf = nf_x;
-    m.ncand = ncand;
+    m.parent_sup = sup_parent;
+    m.parent_nf = nf_parent;
+    m.ncand = nu;
+    m.ntail = nt;
+    m.nunits = sg_tile_units(nt, sg.words, S.chunk_words);
     m.score = 0.0f;
     S.meta[slot] = m;
     S.key[slot] = key;
     S.pkey[slot] = parent_key;
     if (feat_out) {
+        const int e = last;
+        int32_t mn = item_sup[e], mx = item_sup[e], pmin = 0x7fffffff;
+        float sl = 0.0f;
+        const int32_t* pe = pair + sg.pair_base + (int64_t)e * sg.dpad;
+        for (int i = 0; i < k; i++) {
+            const int32_t x = px[i];
+            const int32_t sx = item_sup[x];
+            mn = min(mn, sx);
+            mx = max(mx, sx);
+            sl += __log2f((float)sx);
+            if (x != e) pmin = min(pmin, pe[x]);
+        }
         SgFeatIn f;
-        f.sup_y = sup_y; f.sup_x = sup_x; f.k_y = k + 1; f.nf_x = nf_x; f.ncand_y = ncand;
-        f.tail_y = tail; f.thr = sg.thr; f.n = sg.n; f.d = sg.d; f.e = e; f.sup_e = item_sup[e];
+        f.sup_y = sup; f.sup_x = sup_parent > 0 ? sup_parent : sup; f.k_y = k; f.nf_x = nf_parent; f.ncand_y = nu;
+        f.tail_y = nt; f.thr = sg.thr; f.n = sg.n; f.d = sg.d; f.e = e; f.sup_e = item_sup[e];
         f.min_item = mn; f.max_item = mx; f.pair_min = (pmin == 0x7fffffff ? item_sup[e] : pmin);
-        f.nf_x_tail = nf_x_tail; f.root = (k == 1);
-        f.sum_log_item = sl; f.mean_log_item = sl / (float)(k + 1);
+        f.nf_x_tail = ntail_parent; f.root = parent_is_root;
+        f.sum_log_item = sl; f.mean_log_item = sl / (float)k;
         sg_features(f, feat_out);
     }
 }
 
-// Emit X into the MFI ring.
-__device__ __forceinline__ void sg_emit(const SgSearchDev& S, const uint16_t* px, int k, int32_t sup,
-                                        int seg, int64_t key)
+// Write MFI X at ring position pos (pinned host memory mapped into the device).
+__device__ __forceinline__ void sg_emit_at(const SgSearchDev& S, int64_t pos, const uint16_t* px, int k, int32_t sup,
+                                           int seg, int64_t parent_key)
 {
-    const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], 1ull);
     const int64_t r = pos % S.ring_cap;
     uint16_t* ri = S.ring_items + r * SG_MAX_DEPTH;
     for (int i = 0; i < k; i++) ri[i] = px[i];
@@ -166,52 +177,47 @@ __device__ __forceinline__ void sg_emit(const SgSearchDev& S, const uint16_t* px
     rm[3] = (int32_t)S.ctl[CTL_ROUND];
     if (S.trace_cap) {
         const int64_t t = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_TRACE_MFI], 1ull);
-        if (t < S.trace_cap) S.trace_mfi[t] = key;
+        if (t < S.trace_cap) S.trace_mfi[t] = parent_key;
     }
 }
 
-// Shared per-wave expansion. F (LDS) holds the frequent one-item extensions of X,
-// EQ (LDS) the equivalent ones, sup(X u {e}) == sup(X). Parent-equivalence pruning,
-// exact for maximal itemsets because T(X) is contained in T(e) for e in EQ:
+enum { SG_ACT_NONE = 0, SG_ACT_EMIT = 1, SG_ACT_RECORD = 2 };
+
+// Node step for one wave: X = px[0..k) with tail bound `last`, frequent one-item
+// extensions F and equivalent ones EQ (sup(X u {e}) == sup(X)) in LDS.
+// Parent-equivalence pruning, exact for maximal itemsets because T(X) is contained in
+// T(e) for e in EQ:
 //   * an equivalent item outside the enumeration tail (id >= last) extends every
 //     itemset of X's subtree without losing support, so the subtree holds no MFI;
-//   * the equivalent tail items join every MFI of the subtree, so they are merged
-//     into X (X' = X u EQ, T(X') = T(X): the counts of X serve X' unchanged) and
-//     removed from F before X' is tested for maximality and expanded.
-// Then X' is emitted iff F is empty, otherwise its frequent tail extensions become
-// children (the reference's node step, kernel_bitmap.cu:870-889).
-__device__ void sg_expand_common(const SgSearchDev& S, const SgSegment& sg, const uint64_t* ok,
-                                 const int32_t* item_sup, const int32_t* pair, uint64_t* F,
-                                 const uint64_t* EQ, uint16_t* px, int k, int32_t sup_x, int32_t last, int seg,
-                                 const int32_t* ext_item, const int32_t* ext_cnt, int next,
-                                 int64_t self_key)
+//   * the equivalent tail items join every MFI of the subtree, so they are merged into
+//     X (X' = X u EQ, T(X') = T(X): the counts of X serve X' unchanged; F loses them).
+// Then X' is an MFI iff F is empty (the reference's test, kernel_bitmap.cu:870);
+// otherwise, if it has frequent tail extensions, it becomes one frontier record.
+// Returns the action; k is updated to |X'|.
+__device__ int sg_node_decide(const SgSearchDev& S, const SgSegment& sg, uint64_t* F, const uint64_t* EQ,
+                              uint16_t* px, int& k, int32_t last)
 {
     const int lane = threadIdx.x & 63;
-    const int lw = last >> 6, lb = last & 63;
     int nf = 0, nft = 0, net = 0, nen = 0;
     for (int w = 0; w < sg.dw; w++) {
-        const uint64_t low = w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
+        const uint64_t low = sg_low_mask(w, last);
         const uint64_t v = F[w], q = EQ[w];
         nf += __popcll(v);
         nft += __popcll(v & low);
         net += __popcll(q & low);
         nen += __popcll(q & ~low);
     }
-    if (nf == 0) {
-        if (lane == 0) sg_emit(S, px, k, sup_x, seg, self_key);
-        return;
-    }
-    if (nen > 0) return;
+    if (nf == 0) return SG_ACT_EMIT;
+    if (nen > 0) return SG_ACT_NONE;
     if (net > 0) {
         if (k + net > SG_MAX_DEPTH) {
             if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 2ull);
-            return;
+            return SG_ACT_NONE;
         }
         if (lane == 0) {
             int kk = k;
             for (int w = 0; w < sg.dw; w++) {
-                const uint64_t low = w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
-                uint64_t q = EQ[w] & low;
+                uint64_t q = EQ[w] & sg_low_mask(w, last);
                 F[w] &= ~q;
                 while (q) {
                     px[kk++] = (uint16_t)(w * 64 + (int)__builtin_ctzll(q));
@@ -225,70 +231,32 @@ __device__ void sg_expand_common(const SgSearchDev& S, const SgSegment& sg, cons
         k += net;
         nf -= net;
         nft -= net;
-        if (nf == 0) {
-            if (lane == 0) sg_emit(S, px, k, sup_x, seg, self_key);
-            return;
-        }
+        if (nf == 0) return SG_ACT_EMIT;
     }
-    if (nft == 0) return;
-    if (k >= SG_MAX_DEPTH) {
+    if (nft == 0) return SG_ACT_NONE;
+    if (k >= SG_MAX_DEPTH) {   // its children would exceed the longest supported itemset
         if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 2ull);
-        return;
+        return SG_ACT_NONE;
     }
-    // Reserve nft newborn entries and nft free slots with one atomic each. The round's
-    // node budget (k_select) and the seed's bound (host) keep both within capacity; if
-    // a counter ever overflows, nothing is written out of bounds: the reserved newborn
-    // entries inside the table are marked -1 (k_push skips them) and the error flag
-    // ends the run.
-    int64_t base = 0, nb = 0;
-    if (lane == 0) {
-        nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], (unsigned long long)nft);
-        base = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-(int64_t)nft));
+    return SG_ACT_RECORD;
+}
+
+// Take record slot number `rank` of a reservation: newborn entry nb, free-list top
+// `free_top` (the value before the reservation). The round budget (k_select) and the
+// seed bound (host) keep both in range; an overflow writes nothing out of bounds and
+// stops the run with the error flag. Returns the slot or -1.
+__device__ __forceinline__ int32_t sg_take_slot(const SgSearchDev& S, int64_t nb, int64_t free_top)
+{
+    if (nb >= S.newborn_cap || free_top < 1 || free_top > S.cap) {
+        if (nb < S.newborn_cap) S.newborn[nb] = -1;
+        atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
+        return -1;
     }
-    base = __shfl(base, 0, 64);
-    nb = __shfl(nb, 0, 64);
-    if (nb + nft > S.newborn_cap || base - nft < 0 || base > S.cap) {
-        for (int t = lane; t < nft; t += 64)
-            if (nb + t >= 0 && nb + t < S.newborn_cap) S.newborn[nb + t] = -1;
-        if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
-        return;
-    }
-    int64_t kbase = 0;
-    if (lane == 0) kbase = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], (unsigned long long)nft);
-    kbase = __shfl(kbase, 0, 64);
-    // children in ascending item order: one pass over the tail words, lane j taking bit j
-    int tbase = 0;
-    for (int w = 0; w < sg.dw; w++) {
-        const uint64_t low = w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
-        const uint64_t v = F[w] & low;
-        if (v == 0ull) continue;
-        if ((v >> lane) & 1ull) {
-            const int t = tbase + __popcll(v & ((1ull << lane) - 1ull));
-            const int e = w * 64 + lane;
-            int32_t sup_y;
-            if (ext_item) {
-                // counts listed per candidate (ascending ids): locate e
-                int lo = 0, hi = next - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (ext_item[mid] < e) lo = mid + 1; else hi = mid;
-                }
-                sup_y = ext_cnt[lo];
-            } else {
-                sup_y = ext_cnt[e];  // seed: pair-matrix row of the root
-            }
-            const int32_t slot = S.free_idx[base - 1 - t];
-            float* fo = S.feat_on ? S.feat + (nb + t) * SG_FEAT : nullptr;
-            sg_make_child(S, sg, ok, item_sup, pair, slot, px, k, sup_x, nf, nft, F, e, sup_y, seg, fo,
-                          kbase + t, self_key);
-            S.newborn[nb + t] = slot;
-        }
-        tbase += __popcll(v);
-    }
+    return S.free_idx[free_top - 1];
 }
 
 // ------------------------------------------------------------------ seed (roots from the pair matrix)
-// One wave per root (seg, r): F = ok[r] \ {r}; counts are the pair-matrix row.
+// One wave per root (seg, r): F = ok[r] \ {r}; equivalents from the pair-matrix row.
 __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __restrict__ segs,
                                               const int32_t* __restrict__ pair,
                                               const uint64_t* __restrict__ ok,
@@ -326,12 +294,24 @@ __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __
     if (!live) return;
     const SgSegment sg = segs[seg];
     const int32_t* isup = sup_all + seg_item_base[seg];
-    int64_t key = 0;
-    if (lane == 0) key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], 1ull);
-    key = __shfl(key, 0, 64);
-    sg_expand_common(S, sg, ok, isup, pair, sF[wv], sE[wv], sX[wv], 1, isup[r], r, seg, nullptr,
-                     pair + sg.pair_base + (int64_t)r * sg.dpad, 0, key);
-    if (lane == 0) atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], 1ull);
+    int k = 1;
+    const int act = sg_node_decide(S, sg, sF[wv], sE[wv], sX[wv], k, r);
+    if (lane != 0) return;
+    atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], 1ull);
+    if (act == SG_ACT_EMIT) {
+        const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], 1ull);
+        sg_emit_at(S, pos, sX[wv], k, isup[r], seg, -1);
+    } else if (act == SG_ACT_RECORD) {
+        const int64_t nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], 1ull);
+        const int64_t top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-1ll));
+        const int32_t slot = sg_take_slot(S, nb, top);
+        if (slot >= 0) {
+            const int64_t key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], 1ull);
+            float* fo = S.feat_on ? S.feat + nb * SG_FEAT : nullptr;
+            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, key, -1);
+            S.newborn[nb] = slot;
+        }
+    }
 }
 
 __global__ void k_zero_newborn(SgSearchDev S)
@@ -362,32 +342,32 @@ __device__ __forceinline__ int64_t sg_block_excl_scan(int64_t v, int64_t* wsum, 
     return r;
 }
 
-// Take up to bmax nodes from the highest buckets whose candidate total fits the
-// free-slot and newborn budgets; write candidate offsets and work-unit offsets.
-__global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgSegment* __restrict__ segs)
+// Take up to bmax records from the highest buckets, as many as fit the budgets: new
+// records (at most one per child) within the free slots and newborn table, count
+// cells within the cell table. Writes the U-list, child, cell and unit offsets. The
+// sizes are parked in the offset arrays between the passes (each thread rereads only
+// its own entries), so record metadata is loaded once.
+__global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
 {
-    __shared__ int32_t take[SG_NBUCKETS], start[SG_NBUCKETS + 1], tops[SG_NBUCKETS];
+    __shared__ int32_t take[SG_NBUCKETS], tops[SG_NBUCKETS], ends[SG_NBUCKETS];
     __shared__ int64_t wsum[SEL_THREADS / 64];
-    __shared__ int64_t s_budget, s_tot;
-    __shared__ unsigned long long s_ctot, s_units, s_words;
+    __shared__ int64_t s_budget, tot[4];
     __shared__ int32_t s_total, s_cut;
     const int tid = threadIdx.x;
     if (tid < SG_NBUCKETS) tops[tid] = (int32_t)S.ctl[CTL_BUCKET_TOP0 + tid];
     __syncthreads();
     if (tid == 0) {
+        // order o = 63 - b: the highest bucket first; ends[o] = records taken through o
         int32_t taken = 0;
-        for (int b = SG_NBUCKETS - 1; b >= 0; b--) {
-            const int32_t top = tops[b];
-            const int32_t t = min(top, S.bmax - taken);
+        for (int o = 0; o < SG_NBUCKETS; o++) {
+            const int b = SG_NBUCKETS - 1 - o;
+            const int32_t t = min(tops[b], S.bmax - taken);
             take[b] = t;
             taken += t;
+            ends[o] = taken;
         }
-        int32_t acc = 0;
-        for (int b = SG_NBUCKETS - 1; b >= 0; b--) { start[b] = acc; acc += take[b]; }
-        start[SG_NBUCKETS] = acc;
         s_total = taken;
         s_cut = taken;
-        s_ctot = 0; s_units = 0; s_words = 0;
         s_budget = min(S.ctl[CTL_FREE_TOP], S.newborn_cap);
         S.ctl[CTL_NEWBORN] = 0;
         S.ctl[CTL_UNIT_NEXT] = 0;
@@ -397,88 +377,90 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
     __syncthreads();
     const int total = s_total;
     constexpr int PER = SG_BMAX / SEL_THREADS;
-    int32_t nc[PER], nu[PER], nid[PER];
-    int64_t sc = 0, su = 0;
-    // phase 1: arena slots of this thread's positions (independent loads)
+    // pass 1: record slots, then sizes (independent loads within each phase)
+    int32_t nid[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = tid * PER + q;
         nid[q] = -1;
         if (i < total) {
-            int b = SG_NBUCKETS - 1;
-            while (!(i >= start[b] && i < start[b] + take[b])) b--;
-            nid[q] = S.bucket[(int64_t)b * S.cap + (tops[b] - 1 - (i - start[b]))];
+            int lo = 0, hi = SG_NBUCKETS - 1;   // first order o with ends[o] > i
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ends[mid] > i) hi = mid; else lo = mid + 1;
+            }
+            const int b = SG_NBUCKETS - 1 - lo;
+            const int32_t pos = i - (lo > 0 ? ends[lo - 1] : 0);
+            nid[q] = S.bucket[(int64_t)b * S.cap + (tops[b] - 1 - pos)];
         }
     }
-    // phase 2: their candidate counts and segments
-    int32_t sgi[PER];
+    int64_t su = 0, sc = 0, scell = 0, sunit = 0;
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        nc[q] = 0;
-        sgi[q] = 0;
-        if (nid[q] >= 0) {
-            nc[q] = S.meta[nid[q]].ncand;
-            sgi[q] = S.meta[nid[q]].seg;
-        }
-    }
-    // phase 3: work units
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        nu[q] = 0;
-        if (nid[q] >= 0) {
-            const int64_t words = segs[sgi[q]].words;
-            nu[q] = (int32_t)((words + S.chunk_words - 1) / S.chunk_words);
-        }
-        sc += nc[q];
-        su += nu[q];
-    }
-    int64_t ec = sg_block_excl_scan(sc, wsum, &s_tot);
-    int64_t eu = sg_block_excl_scan(su, wsum, &s_tot);
-    const int64_t budget = s_budget;
     for (int q = 0; q < PER; q++) {
         const int i = tid * PER + q;
-        if (i < total) {
-            if (ec + nc[q] > budget) atomicMin(&s_cut, i);
-            S.coff[i] = (int32_t)ec;
-            S.unit_off[i] = eu;
+        if (nid[q] >= 0) {
+            const SgNodeMeta m = S.meta[nid[q]];
+            su += m.ncand;
+            sc += m.ntail;
+            scell += (int64_t)m.ncand * m.ntail;
+            sunit += m.nunits;
             S.batch[i] = nid[q];
+            S.coff[i] = m.ncand;
+            S.choff[i] = m.ntail;
+            S.unit_off[i] = m.nunits;
         }
-        ec += nc[q];
-        eu += nu[q];
     }
-    __syncthreads();
-    const int cut = s_cut;
-    unsigned long long mc = 0, mu = 0, mw = 0;
+    int64_t eu = sg_block_excl_scan(su, wsum, &tot[0]);
+    int64_t ec = sg_block_excl_scan(sc, wsum, &tot[1]);
+    int64_t ecell = sg_block_excl_scan(scell, wsum, &tot[2]);
+    int64_t eunit = sg_block_excl_scan(sunit, wsum, &tot[3]);
+    // pass 2: offsets; the cut is the first record past a budget (offsets only grow)
+    const int64_t budget = s_budget;
+#pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = tid * PER + q;
-        if (i < cut) {
-            mc += (unsigned long long)nc[q];
-            mu += (unsigned long long)nu[q];
-            mw += (unsigned long long)(nc[q] + 1) * (unsigned long long)segs[sgi[q]].words;
+        if (nid[q] >= 0) {
+            const int32_t nu = S.coff[i], nt = S.choff[i];
+            const int64_t nun = S.unit_off[i];
+            const int64_t cells = (int64_t)nu * nt;
+            if (ec + nt > budget || ecell + cells > S.cell_cap) atomicMin(&s_cut, i);
+            S.coff[i] = (int32_t)eu;
+            S.choff[i] = (int32_t)ec;
+            S.celloff[i] = ecell;
+            S.unit_off[i] = eunit;
+            eu += nu;
+            ec += nt;
+            ecell += cells;
+            eunit += nun;
         }
     }
-    if (mc) atomicAdd(&s_ctot, mc);
-    if (mu) atomicAdd(&s_units, mu);
-    if (mw) atomicAdd(&s_words, mw);
     __syncthreads();
     if (tid == 0) {
+        const int cut = s_cut;
+        if (cut == total) {   // every taken record fits: the end offsets are the totals
+            S.coff[cut] = (int32_t)tot[0];
+            S.choff[cut] = (int32_t)tot[1];
+            S.celloff[cut] = tot[2];
+            S.unit_off[cut] = tot[3];
+        }
         if (cut == 0 && total > 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
-        for (int b = SG_NBUCKETS - 1; b >= 0; b--) {
-            const int32_t lo = start[b], hi = start[b] + take[b];
+        for (int o = 0; o < SG_NBUCKETS; o++) {
+            const int b = SG_NBUCKETS - 1 - o;
+            const int32_t lo = o > 0 ? ends[o - 1] : 0, hi = ends[o];
             S.ctl[CTL_BUCKET_TOP0 + b] -= max(0, min(hi, cut) - lo);
         }
-        S.coff[cut] = (int32_t)s_ctot;
-        S.unit_off[cut] = (int64_t)s_units;
+        const int64_t nch = S.choff[cut];
         S.ctl[CTL_BATCH_N] = cut;
-        S.ctl[CTL_UNITS] = (int64_t)s_units;
-        S.ctl[CTL_COUNTS] += (int64_t)s_ctot;
-        S.ctl[CTL_WORDS] += (int64_t)s_words;
-        S.ctl[CTL_NODES] += cut;
+        S.ctl[CTL_CHILDREN] = nch;
+        S.ctl[CTL_CELLS] = S.celloff[cut];
+        S.ctl[CTL_UNITS] = S.unit_off[cut];
+        S.ctl[CTL_NODES] += nch;
     }
 }
 
-// ------------------------------------------------------------------ candidate lists
-// One wave per batch node: expand the candidate mask to an ascending id list, zero counts.
+// ------------------------------------------------------------------ U lists
+// One wave per record: expand U into an ascending id list; the children are its
+// first ntail entries (U is ascending and children are the items below `last`).
 __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -499,63 +481,196 @@ __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment
     while (t) {
         const int bit = (int)__builtin_ctzll(t);
         t &= t - 1ull;
-        S.clist[pos] = lane * 64 + bit;
-        S.ccount[pos] = 0;
-        pos++;
+        S.clist[pos++] = lane * 64 + bit;
     }
+    for (int j = lane; j < m.ntail; j += 64) S.ccount[S.choff[b] + j] = 0;
 }
 
-// ------------------------------------------------------------------ counting
+__global__ __launch_bounds__(256) void k_zero_cells(SgSearchDev S)
+{
+    const int64_t n = S.ctl[CTL_CELLS];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        S.cells[i] = 0;
+}
+
+// ------------------------------------------------------------------ counting tiles
+// Sum of four per-lane values over the wave, transposed: on return lane 16*r holds the
+// total of v[r] (every lane of row r holds it). Two 32-lane swaps, one 16-lane swap and
+// a 16-lane DPP rotation tree: 10 VALU ops for four sums instead of four full trees.
+__device__ __forceinline__ uint32_t sg_wave_sum4(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3)
+{
+    // permlane32_swap(a, b) -> {a.lo | b.lo, a.hi | b.hi} (halves of 32 lanes)
+    const auto p02 = __builtin_amdgcn_permlane32_swap(v0, v2, false, false);
+    const auto p13 = __builtin_amdgcn_permlane32_swap(v1, v3, false, false);
+    const uint32_t s02 = p02[0] + p02[1];   // lanes 0-31: v0 halves, 32-63: v2 halves
+    const uint32_t s13 = p13[0] + p13[1];   // lanes 0-31: v1 halves, 32-63: v3 halves
+    // permlane16_swap(a, b) -> {a.r0 b.r0 a.r2 b.r2, a.r1 b.r1 a.r3 b.r3} (rows of 16 lanes)
+    const auto q = __builtin_amdgcn_permlane16_swap(s02, s13, false, false);
+    uint32_t t = q[0] + q[1];               // row r holds 16 partial sums of v[r]
+    t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x128, 0xf, 0xf, false);
+    t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x124, 0xf, 0xf, false);
+    t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x122, 0xf, 0xf, false);
+    t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x121, 0xf, 0xf, false);
+    return t;
+}
+
+#define SG_TILE_THREADS 512
+#define SG_RC 4   // candidates held in registers per wave step
+
+// Unit (record g, block of up to SG_CB children, range of up to SG_CR row chunks).
+// Set-up, once per unit: the children's pair-ok rows give each candidate f of U the
+// mask of children e that need it (pair (e, f) frequent, f != e); candidates no child
+// needs are dropped, the others listed in LDS. Then per chunk: the prefix of X and the
+// prefixes of the children go to LDS; each wave takes SG_RC listed candidates at a
+// time, loads their row chunks once into registers (coalesced loads) and, for every
+// child that needs one of them, reads the child prefix from LDS once and ANDs +
+// popcounts it with each candidate that needs it. The four per-lane partial sums of a
+// child reduce together (sg_wave_sum4); one integer atomic per needed (child,
+// candidate, chunk). Skipped cells stay 0, which the expansion reads as infrequent --
+// exact, since an infrequent pair has no frequent superset. The child prefixes'
+// popcounts give the children's supports.
 template <int CW>
-__global__ __launch_bounds__(256) void k_count_search(SgSearchDev S, const uint64_t* __restrict__ rows,
-                                                      const SgSegment* __restrict__ segs, int32_t uniform)
+__global__ __launch_bounds__(SG_TILE_THREADS) void k_count_group(SgSearchDev S, const uint64_t* __restrict__ rows,
+                                                                 const SgSegment* __restrict__ segs,
+                                                                 const uint64_t* __restrict__ ok)
 {
     __shared__ uint64_t pre[CW];
+    __shared__ uint64_t cpbuf[SG_CB * CW];   // child prefixes; the children's ok rows during set-up
+    __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
     __shared__ int64_t s_unit;
     __shared__ int32_t s_items[SG_MAX_DEPTH];
-    const int wave = threadIdx.x >> 6;
-    const int nwave = blockDim.x >> 6;
+    __shared__ int32_t s_child[SG_CB];
+    __shared__ int32_t s_nc;
+    static_assert(SG_CB * 64 <= SG_CB * CW, "ok rows fit the prefix buffer");
+    static_assert(SG_RC * SG_CB == 64, "one lane per (candidate, child) pair of a wave step");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int NW = SG_TILE_THREADS / 64;
+    constexpr int PER = CW / 64;
     const int64_t n_units = S.ctl[CTL_UNITS];
     const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
     if (threadIdx.x == 0 && n_units > 0)
         atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
     for (;;) {
-        if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_UNIT_NEXT], 1ull);
+        if (threadIdx.x == 0) {
+            s_unit = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_UNIT_NEXT], 1ull);
+            s_nc = 0;
+        }
         __syncthreads();
         const int64_t u = s_unit;
         if (u >= n_units) break;
-        int b, c;
-        if (uniform) {
-            // chunk-major order: all nodes' chunk c run together, so the candidate
-            // rows' chunk c is shared in L2 across the batch
-            c = (int)(u / nb);
-            b = (int)(u % nb);
-        } else {
-            int lo = 0, hi = nb - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (S.unit_off[mid] <= u) lo = mid; else hi = mid - 1;
-            }
-            b = lo;
-            c = (int)(u - S.unit_off[b]);
+        int lo = 0, hi = nb - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.unit_off[mid] <= u) lo = mid; else hi = mid - 1;
         }
-        const int32_t slot = S.batch[b];
+        const int g = lo;
+        const int32_t slot = S.batch[g];
         const SgNodeMeta m = S.meta[slot];
         const SgSegment sg = segs[m.seg];
-        const int64_t w0 = (int64_t)c * CW;
-        const int wn = (int)min((int64_t)CW, sg.words - w0);
+        const int nblk = (m.ntail + SG_CB - 1) / SG_CB;
+        const int r = (int)(u - S.unit_off[g]);
+        const int cr = r / nblk, blk = r % nblk;
+        const int64_t nchunk = (sg.words + CW - 1) / CW;
+        const int c0 = cr * SG_CR;
+        const int c1 = (int)min((int64_t)(c0 + SG_CR), nchunk);
+        const int j0 = blk * SG_CB;
+        const int ncb = min(SG_CB, m.ntail - j0);
+        const int32_t u0 = S.coff[g];
+        const int nu = m.ncand;
         if (threadIdx.x < m.k) s_items[threadIdx.x] = S.items[(int64_t)slot * SG_MAX_DEPTH + threadIdx.x];
+        if (threadIdx.x < ncb) s_child[threadIdx.x] = S.clist[u0 + j0 + threadIdx.x];
         __syncthreads();
-        const uint64_t* seg_rows = rows + sg.row_base;
-        sg_build_prefix(pre, seg_rows, sg.stride, w0, wn, m.k, s_items);
-        __syncthreads();
-        const int32_t c0 = S.coff[b], nc = S.coff[b + 1] - c0;
-        for (int j = wave; j < nc; j += nwave) {
-            const int32_t e = S.clist[c0 + j];
-            const uint32_t s = sg_count_row<CW>(pre, seg_rows + (int64_t)e * sg.stride + w0, wn);
-            if ((threadIdx.x & 63) == 0 && s) atomicAdd(&S.ccount[c0 + j], (int32_t)s);
+        // set-up: children's ok rows, then the list of needed candidates
+        const uint64_t* okb = ok + sg.ok_base;
+        for (int t = threadIdx.x; t < ncb * sg.dw; t += blockDim.x) {
+            const int j = t / sg.dw, w = t - j * sg.dw;
+            cpbuf[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
         }
         __syncthreads();
+        for (int i = threadIdx.x; i < nu; i += blockDim.x) {
+            const int32_t f = S.clist[u0 + i];
+            uint32_t mask = 0u;
+            for (int j = 0; j < ncb; j++)
+                if (s_child[j] != f && ((cpbuf[j * 64 + (f >> 6)] >> (f & 63)) & 1ull)) mask |= 1u << j;
+            if (mask) s_cand[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
+        }
+        __syncthreads();
+        const int nc = s_nc;
+        const uint64_t* seg_rows = rows + sg.row_base;
+        int32_t* cells = S.cells + S.celloff[g];
+        const int rl = lane >> 4, jl = lane & 15;   // this lane's (candidate, child) of a step
+        for (int c = c0; c < c1; c++) {
+            const int64_t w0 = (int64_t)c * CW;
+            const int wn = (int)min((int64_t)CW, sg.words - w0);
+            const int qn = (wn + 63) >> 6;   // 64-word columns holding row words
+            // prefix of X over the chunk (zero past the last word)
+            for (int w = threadIdx.x; w < CW; w += blockDim.x) {
+                uint64_t v = 0ull;
+                if (w < wn) {
+                    v = ~0ull;
+                    for (int i = 0; i < m.k; i++) v &= seg_rows[(int64_t)s_items[i] * sg.stride + w0 + w];
+                }
+                pre[w] = v;
+            }
+            __syncthreads();
+            // child prefixes and child supports
+            for (int j = wave; j < ncb; j += NW) {
+                uint32_t acc = 0;
+                const uint64_t* row = seg_rows + (int64_t)s_child[j] * sg.stride + w0;
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    if (q < qn) {
+                        const int w = lane + 64 * q;
+                        const uint64_t v = w < wn ? (pre[w] & row[w]) : 0ull;
+                        cpbuf[j * CW + w] = v;
+                        acc += (uint32_t)__popcll(v);
+                    }
+                }
+                const uint32_t s = sg_wave_sum(acc);
+                if (lane == 0 && s) atomicAdd(&S.ccount[S.choff[g] + j0 + j], (int32_t)s);
+            }
+            __syncthreads();
+            for (int i0 = wave * SG_RC; i0 < nc; i0 += NW * SG_RC) {
+                const uint32_t ent = i0 + rl < nc ? s_cand[i0 + rl] : 0u;
+                const int il = (int)(ent & 0xffffu);
+                const int32_t fl = ent ? S.clist[u0 + il] : -1;
+                const uint64_t nm = __ballot((ent >> (16 + jl)) & 1u);   // bit 16*r + j
+                uint64_t x[SG_RC][PER];
+#pragma unroll
+                for (int rr = 0; rr < SG_RC; rr++) {
+                    const bool any = ((nm >> (16 * rr)) & 0xffffull) != 0ull;
+                    const int32_t f = __builtin_amdgcn_readlane(fl, 16 * rr);
+                    const uint64_t* row = seg_rows + (int64_t)(f < 0 ? 0 : f) * sg.stride + w0;
+#pragma unroll
+                    for (int q = 0; q < PER; q++) {
+                        const int w = lane + 64 * q;
+                        x[rr][q] = (any && q < qn && w < wn) ? row[w] : 0ull;
+                    }
+                }
+                for (int j = 0; j < ncb; j++) {
+                    const uint32_t mj = (uint32_t)((nm >> j) & 1ull) | (uint32_t)((nm >> (15 + j)) & 2ull) |
+                                        (uint32_t)((nm >> (30 + j)) & 4ull) | (uint32_t)((nm >> (45 + j)) & 8ull);
+                    if (mj == 0u) continue;
+                    uint64_t cj[PER];
+#pragma unroll
+                    for (int q = 0; q < PER; q++) cj[q] = q < qn ? cpbuf[j * CW + lane + 64 * q] : 0ull;
+                    uint32_t a[SG_RC];
+#pragma unroll
+                    for (int rr = 0; rr < SG_RC; rr++) {
+                        a[rr] = 0u;
+                        if ((mj >> rr) & 1u) {
+#pragma unroll
+                            for (int q = 0; q < PER; q++)
+                                if (q < qn) a[rr] += (uint32_t)__popcll(x[rr][q] & cj[q]);
+                        }
+                    }
+                    const uint32_t t = sg_wave_sum4(a[0], a[1], a[2], a[3]);
+                    if (jl == 0 && ((mj >> rl) & 1u) && t)
+                        atomicAdd(&cells[(int64_t)(j0 + j) * nu + il], (int32_t)t);
+                }
+            }
+            __syncthreads();
+        }
     }
     if (threadIdx.x == 0 && n_units > 0) {
         __threadfence();
@@ -563,75 +678,150 @@ __global__ __launch_bounds__(256) void k_count_search(SgSearchDev S, const uint6
     }
 }
 
-// ------------------------------------------------------------------ expand
-__global__ __launch_bounds__(256) void k_expand(SgSearchDev S, const SgSegment* __restrict__ segs,
-                                                const int32_t* __restrict__ pair,
-                                                const uint64_t* __restrict__ ok,
-                                                const int32_t* __restrict__ sup_all,
-                                                const int64_t* __restrict__ seg_item_base)
+// ------------------------------------------------------------------ expansion
+#define SG_EXP_WAVES 16
+
+// Workgroups of 16 waves stride over the batch's children (static assignment: the
+// per-child work is nearly uniform). Child j of record g is Y = X u {e}; its row of the
+// count matrix gives F(Y) and EQ(Y) over U \ {e}; then the node step. MFI ring
+// positions and record slots are reserved once per workgroup step (one atomic per
+// counter for 16 children), and the cumulative counters once per workgroup.
+__global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchDev S, const SgSegment* __restrict__ segs,
+                                                                       const int32_t* __restrict__ pair,
+                                                                       const uint64_t* __restrict__ ok,
+                                                                       const int32_t* __restrict__ sup_all,
+                                                                       const int64_t* __restrict__ seg_item_base)
 {
-    __shared__ uint64_t sF[4][64];
-    __shared__ uint64_t sE[4][64];
-    __shared__ uint16_t sX[4][SG_MAX_DEPTH];
+    __shared__ uint64_t sF[SG_EXP_WAVES][64];
+    __shared__ uint64_t sE[SG_EXP_WAVES][64];
+    __shared__ uint16_t sX[SG_EXP_WAVES][SG_MAX_DEPTH];
+    __shared__ int32_t s_act[SG_EXP_WAVES];
+    __shared__ int64_t s_ring, s_nb, s_top, s_key;
+    __shared__ unsigned long long s_counts, s_words;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int b = blockIdx.x * 4 + wv;
-    const bool live = b < (int)S.ctl[CTL_BATCH_N];
-    int32_t slot = 0;
-    SgNodeMeta m = {};
-    if (live) {
-        slot = S.batch[b];
-        m = S.meta[slot];
+    const int64_t nchild = S.ctl[CTL_CHILDREN];
+    const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
+    if (threadIdx.x == 0) { s_counts = 0; s_words = 0; }
+    for (int64_t q0 = (int64_t)blockIdx.x * SG_EXP_WAVES; q0 < nchild; q0 += (int64_t)gridDim.x * SG_EXP_WAVES) {
+        const int64_t q = q0 + wv;
+        int act = SG_ACT_NONE, k = 0, g = 0;
+        int32_t e = 0, sup_e = 0;
         sF[wv][lane] = 0ull;
         sE[wv][lane] = 0ull;
-        if (lane < m.k) sX[wv][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
+        __syncthreads();
+        if (q < nchild) {
+            int lo = 0, hi = nb - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (S.choff[mid] <= q) lo = mid; else hi = mid - 1;
+            }
+            g = lo;
+            const int j = (int)(q - S.choff[g]);
+            const int32_t slot = S.batch[g];
+            const SgNodeMeta m = S.meta[slot];
+            const SgSegment sg = segs[m.seg];
+            const int32_t u0 = S.coff[g];
+            const int nu = m.ncand;
+            e = S.clist[u0 + j];
+            sup_e = S.ccount[S.choff[g] + j];
+            const int32_t* crow = S.cells + S.celloff[g] + (int64_t)j * nu;
+            const uint64_t* oke = ok + sg.ok_base + (int64_t)e * sg.dw;
+            if (lane < m.k) sX[wv][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
+            if (lane == 0) sX[wv][m.k] = (uint16_t)e;
+            int needed = 0;
+            for (int i = lane; i < nu; i += 64) {
+                if (i == j) continue;
+                const int32_t f = S.clist[u0 + i];
+                const int32_t cnt = crow[i];
+                if ((oke[f >> 6] >> (f & 63)) & 1ull) needed++;
+                if (cnt >= sg.thr) {
+                    atomicOr((unsigned long long*)&sF[wv][f >> 6], 1ull << (f & 63));
+                    if (cnt == sup_e) atomicOr((unsigned long long*)&sE[wv][f >> 6], 1ull << (f & 63));
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) needed += __shfl_xor(needed, o, 64);
+            if (lane == 0) {
+                atomicAdd(&s_counts, (unsigned long long)needed);
+                atomicAdd(&s_words, (unsigned long long)(needed + 1) * (unsigned long long)sg.words);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            k = m.k + 1;
+            act = sg_node_decide(S, sg, sF[wv], sE[wv], sX[wv], k, e);
+        }
+        if (lane == 0) s_act[wv] = act;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int ne = 0, nr = 0;
+            for (int w = 0; w < SG_EXP_WAVES; w++) {
+                ne += s_act[w] == SG_ACT_EMIT;
+                nr += s_act[w] == SG_ACT_RECORD;
+            }
+            if (ne) s_ring = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], (unsigned long long)ne);
+            if (nr) {
+                s_nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], (unsigned long long)nr);
+                s_top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-(int64_t)nr));
+                s_key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], (unsigned long long)nr);
+            }
+        }
+        __syncthreads();
+        if (act != SG_ACT_NONE && lane == 0) {
+            int rank = 0;
+            for (int w = 0; w < wv; w++) rank += s_act[w] == act;
+            const int32_t slot = S.batch[g];
+            const SgNodeMeta m = S.meta[slot];
+            const SgSegment sg = segs[m.seg];
+            if (act == SG_ACT_EMIT) {
+                sg_emit_at(S, s_ring + rank, sX[wv], k, sup_e, m.seg, S.key[slot]);
+            } else {
+                const int64_t nbi = s_nb + rank;
+                const int32_t nslot = sg_take_slot(S, nbi, s_top - rank);
+                if (nslot >= 0) {
+                    float* fo = S.feat_on ? S.feat + nbi * SG_FEAT : nullptr;
+                    sg_make_record(S, sg, sup_all + seg_item_base[m.seg], pair, nslot, sX[wv], k, sup_e, e, m.seg,
+                                   sF[wv], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot]);
+                    S.newborn[nbi] = nslot;
+                }
+            }
+        }
+        __syncthreads();
     }
     __syncthreads();
-    if (!live) return;
-    const SgSegment sg = segs[m.seg];
-    const int32_t c0 = S.coff[b], nc = S.coff[b + 1] - c0;
-    for (int j = lane; j < nc; j += 64) {
-        const int32_t cnt = S.ccount[c0 + j];
-        if (cnt >= sg.thr) {
-            const int e = S.clist[c0 + j];
-            atomicOr((unsigned long long*)&sF[wv][e >> 6], 1ull << (e & 63));
-            if (cnt == m.sup) atomicOr((unsigned long long*)&sE[wv][e >> 6], 1ull << (e & 63));
-        }
+    if (threadIdx.x == 0 && s_counts) {
+        atomicAdd((unsigned long long*)&S.ctl[CTL_COUNTS], s_counts);
+        atomicAdd((unsigned long long*)&S.ctl[CTL_WORDS], s_words);
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    const int64_t key = S.key[slot];
-    sg_expand_common(S, sg, ok, sup_all + seg_item_base[m.seg], pair, sF[wv], sE[wv], sX[wv], m.k, m.sup,
-                     m.last, m.seg, S.clist + c0, S.ccount + c0, nc, key);
 }
 
 // ------------------------------------------------------------------ push newborns into buckets
 __global__ void k_push(SgSearchDev S, const SgSegment* __restrict__ segs, int32_t order_mode,
                        const float* __restrict__ scores)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = S.ctl[CTL_NEWBORN];
-    if (i >= n) return;
-    const int32_t slot = S.newborn[i];
-    if (slot < 0 || slot >= S.cap) return;
-    const SgNodeMeta m = S.meta[slot];
-    int b;
-    if (order_mode == 1) {
-        const float sc = scores[i];
-        S.meta[slot].score = sc;
-        b = sg_bucket_score(sc);
-    } else if (order_mode == 2) {
-        b = min(m.k, SG_NBUCKETS - 1);
-    } else {
-        b = sg_bucket_support(m.sup, segs[m.seg].thr);
-    }
-    const int32_t pos = (int32_t)atomicAdd((unsigned long long*)&S.ctl[CTL_BUCKET_TOP0 + b], 1ull);
-    S.bucket[(int64_t)b * S.cap + pos] = slot;
-    if (S.trace_cap) {
-        const int64_t t = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_TRACE_BIRTH], 1ull);
-        if (t < S.trace_cap) {
-            S.trace_key[2 * t] = S.key[slot];
-            S.trace_key[2 * t + 1] = S.pkey[slot];
-            for (int f = 0; f < SG_FEAT; f++) S.trace_feat[t * SG_FEAT + f] = S.feat[i * SG_FEAT + f];
+    const int64_t n = min(S.ctl[CTL_NEWBORN], S.newborn_cap);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t slot = S.newborn[i];
+        if (slot < 0 || slot >= S.cap) continue;
+        const SgNodeMeta m = S.meta[slot];
+        int b;
+        if (order_mode == 1) {
+            const float sc = scores[i];
+            S.meta[slot].score = sc;
+            b = sg_bucket_score(sc);
+        } else if (order_mode == 2) {
+            b = min(m.k, SG_NBUCKETS - 1);
+        } else {
+            b = sg_bucket_support(m.sup, segs[m.seg].thr);
+        }
+        const int32_t pos = (int32_t)atomicAdd((unsigned long long*)&S.ctl[CTL_BUCKET_TOP0 + b], 1ull);
+        S.bucket[(int64_t)b * S.cap + pos] = slot;
+        if (S.trace_cap) {
+            const int64_t t = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_TRACE_BIRTH], 1ull);
+            if (t < S.trace_cap) {
+                S.trace_key[2 * t] = S.key[slot];
+                S.trace_key[2 * t + 1] = S.pkey[slot];
+                for (int f = 0; f < SG_FEAT; f++) S.trace_feat[t * SG_FEAT + f] = S.feat[i * SG_FEAT + f];
+            }
         }
     }
 }
@@ -663,6 +853,20 @@ __global__ void k_frontier_sum(SgSearchDev S)
 }
 
 // ------------------------------------------------------------------ launchers
+// Resident workgroups per CU of the tile kernel (sizes its persistent grid).
+int sg_count_group_blocks_per_cu(int32_t chunk_words)
+{
+    int n = 0;
+    hipError_t e = chunk_words == 256
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<256>, SG_TILE_THREADS, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<512>, SG_TILE_THREADS, 0);
+    if (e != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        n = 1;
+    }
+    return n;
+}
+
 hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s)
 {
     const int64_t n = S.cap > CTL_SLOTS ? S.cap : CTL_SLOTS;
@@ -677,9 +881,7 @@ static hipError_t sg_push_newborn(const SgSearchDev& S, const SgSegment* segs, i
         hipError_t e = sg_launch_score(scorer, S.feat, (int32_t)S.newborn_cap, &S.ctl[CTL_NEWBORN], S.scores, s);
         if (e != hipSuccess) return e;
     }
-    const int64_t n = S.newborn_cap;
-    hipLaunchKernelGGL(k_push, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, segs, order_mode,
-                       (const float*)S.scores);
+    hipLaunchKernelGGL(k_push, dim3(1024), dim3(256), 0, s, S, segs, order_mode, (const float*)S.scores);
     return hipGetLastError();
 }
 
@@ -690,7 +892,6 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 {
     hipError_t e = hipSuccess;
     if (n_roots > 0) {
-        // newborn counter reset, then roots
         hipLaunchKernelGGL(k_zero_newborn, dim3(1), dim3(1), 0, s, S);
         hipLaunchKernelGGL(k_seed, dim3((unsigned)((n_roots + 3) / 4)), dim3(256), 0, s, S, segs, pair, ok,
                            roots, n_roots, sup_all, seg_item_base);
@@ -706,27 +907,19 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,
                            const SgScorer* scorer, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s)
 {
+    (void)uniform;
     hipError_t e;
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S, segs);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
+    hipLaunchKernelGGL(k_zero_cells, dim3(2048), dim3(256), 0, s, S);
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-    switch (S.chunk_words) {
-    case 256:
-        hipLaunchKernelGGL(k_count_search<256>, dim3(S.count_grid), dim3(256), 0, s, S, rows, segs, uniform);
-        break;
-    case 512:
-        hipLaunchKernelGGL(k_count_search<512>, dim3(S.count_grid), dim3(256), 0, s, S, rows, segs, uniform);
-        break;
-    case 2048:
-        hipLaunchKernelGGL(k_count_search<2048>, dim3(S.count_grid), dim3(256), 0, s, S, rows, segs, uniform);
-        break;
-    default:
-        hipLaunchKernelGGL(k_count_search<1024>, dim3(S.count_grid), dim3(256), 0, s, S, rows, segs, uniform);
-        break;
-    }
+    if (S.chunk_words == 256)
+        hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
+    else
+        hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
     if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_expand, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs, pair, ok,
-                       sup_all, seg_item_base);
+    hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,
+                       seg_item_base);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = sg_push_newborn(S, segs, order_mode, scorer, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_release, dim3((unsigned)((S.bmax + 255) / 256)), dim3(256), 0, s, S);

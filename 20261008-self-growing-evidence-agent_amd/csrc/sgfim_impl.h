@@ -34,15 +34,19 @@ struct SgSegment {
     int32_t pad_;
 };
 
-// One frontier node. Items are dense ids of the node's segment.
+// One frontier record: an itemset X whose frequent one-item extensions U = F(X) are
+// known (the candidate mask) and whose children X u {e}, e in U below `last`, are still
+// to be counted. Items are dense ids of the record's segment.
 struct SgNodeMeta {
     int32_t k;          // |X|
     int32_t sup;        // support of X
-    int32_t last;       // dense id of the last item added (the enumeration tail bound)
+    int32_t last;       // the enumeration tail bound: children take items below it
     int32_t seg;        // segment id
     int32_t parent_sup; // support of the parent (0 for roots)
-    int32_t parent_nf;  // frequent extensions of the parent
-    int32_t ncand;      // candidates inherited from the parent
+    int32_t parent_nf;  // |F(parent)|
+    int32_t ncand;      // |U|
+    int32_t ntail;      // children: |U below last|
+    int32_t nunits;     // counting units of the record's tile (child blocks x chunk ranges)
     float score;        // priority score
 };
 
@@ -95,8 +99,12 @@ struct SgSearchDev {
     int32_t* batch;      // [bmax + 1]
     int32_t* coff;       // [bmax + 1] candidate offsets
     int64_t* unit_off;   // [bmax + 1] work-unit offsets
-    int32_t* clist;      // [bmax * d_max] candidate ids
-    int32_t* ccount;     // [bmax * d_max] counts
+    int32_t* clist;      // [bmax * d_max] U lists of the batch (ascending ids)
+    int32_t* ccount;     // [bmax * d_max] child supports |T(X) & T(e)|
+    int32_t* choff;      // [bmax + 1] child offsets
+    int64_t* celloff;    // [bmax + 1] count-matrix offsets
+    int32_t* cells;      // [cell_cap] child x candidate counts
+    int64_t cell_cap;
     // children born in the current round
     int32_t* newborn;    // [newborn_cap]
     float* feat;         // [newborn_cap][SG_FEAT]
@@ -130,24 +138,27 @@ enum SgCtl {
     CTL_UNITS,            // counting work units of the current batch
     CTL_UNIT_NEXT,        // persistent-kernel work counter
     CTL_RING_POS,         // MFIs written so far (ring position = pos % ring_cap)
-    CTL_NODES,            // nodes processed (cumulative)
-    CTL_COUNTS,           // support counts computed by AND+popcount (cumulative)
-    CTL_WORDS,            // algorithmic words: (candidates + 1) * W per node (cumulative)
+    CTL_NODES,            // nodes (children of records, plus roots) processed (cumulative)
+    CTL_COUNTS,           // needed support counts computed by AND+popcount (cumulative)
+    CTL_WORDS,            // algorithmic words: (needed counts + 1) * W per node (cumulative)
     CTL_ERR,              // error flags: bit 0 capacity, bit 1 depth
     CTL_ROUND,            // rounds done
-    CTL_NEWBORN,          // children born this round
-    CTL_FRONTIER,         // nodes waiting in buckets
-    CTL_KEY,              // next node key
+    CTL_NEWBORN,          // records born this round
+    CTL_FRONTIER,         // records waiting in buckets
+    CTL_KEY,              // next record key
     CTL_TRACE_BIRTH,      // trace records written (births)
     CTL_TRACE_MFI,        // trace records written (MFIs)
     CTL_T0,               // earliest wall_clock64() at which a counting workgroup started (this round)
     CTL_BUCKET_TOP0 = 16, // SG_NBUCKETS slots
     CTL_T1 = 16 + SG_NBUCKETS, // latest wall_clock64() at which a counting workgroup finished
-    CTL_SLOTS = 17 + SG_NBUCKETS
+    CTL_CHILDREN,         // children to expand in the current batch
+    CTL_CELLS,            // count-matrix cells of the current batch
+    CTL_SLOTS = 19 + SG_NBUCKETS
 };
 
 struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)
 
+int sg_count_group_blocks_per_cu(int32_t chunk_words);
 hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s);
 hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int32_t* pair,
                           const uint64_t* ok, const int32_t* roots, int32_t n_roots,

@@ -22,6 +22,11 @@ sys.path.insert(0, os.path.join(ROOT, "20261008-self-growing-evidence-agent_amd"
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_count_search dispatch for this command, from the two rocprofv3 --pmc
+# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
+# what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
+# profiles/k_count_search_workload2_pmc.md holds the commands and the table.
+PMC_COUNT_KERNEL = {"fetch_kib": 63594.2, "write_kib": 23659.7, "fetch_scale": 2.0, "write_scale": 1.0}
 SUPPORTS = (0.10, 0.05, 0.02)
 
 
@@ -220,7 +225,11 @@ def main():
         "config": {"workload": f"workload 2: {T.n_txn} txns, 64 sensors x 8 bins = {T.n_items} items, "
                                f"min supports 10%/5%/2%", "mfis_per_support": mfis},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": 1024.0 * (PMC_COUNT_KERNEL["fetch_scale"] * PMC_COUNT_KERNEL["fetch_kib"] +
+                                          PMC_COUNT_KERNEL["write_scale"] * PMC_COUNT_KERNEL["write_kib"]),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command, "
+                                       "calibrated by the copy probe (profiles/k_count_search_workload2_pmc.md)",
                      "kernel": "k_count_search", "avg_launch_ms": avg_ms,
                      "alg_bytes_per_launch": bytes_per_launch, "timing_check": event_check,
                      "copy_probe_gbs": copy_gbs},

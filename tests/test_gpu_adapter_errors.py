@@ -92,6 +92,16 @@ def test_error_paths():
         a.load_csr(T.offsets, T.items)
         ref = a.mine(gen.min_count(0.02, T.n_txn)).as_dict()
     assert len(ref) > 0
+    # an explicit MFI ring bounds the children per round: smaller rounds, same result;
+    # a ring that cannot hold the roots plus one record's children is refused
+    with Context(device=0, ring_cap=1024) as a:
+        a.load_csr(T.offsets, T.items)
+        assert a.mine(gen.min_count(0.02, T.n_txn)).as_dict() == ref
+    with Context(device=0, ring_cap=8) as a:
+        a.load_csr(T.offsets, T.items)
+        with pytest.raises(L.SgfimError) as e:
+            a.mine(gen.min_count(0.02, T.n_txn))
+        assert e.value.code == L.E_CAPACITY
 
 
 def test_depth_overflow_and_deadline_and_cancel():
