@@ -46,7 +46,9 @@ struct SearchBufs {
     hipGraph_t graph[2] = {nullptr, nullptr};       // one captured round per frontier order
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;        // around the counting kernel (direct launches)
-    int64_t* h_ctl = nullptr;                       // pinned copy of the control block
+    int64_t* h_ctl = nullptr;                       // pinned: processed copy of the control block, then
+                                                    // two snapshots of rounds in flight
+    hipEvent_t snap_ev[2] = {nullptr, nullptr};     // a round and its snapshot have completed
     uint16_t* h_ring_items = nullptr;               // MFI ring, pinned host memory mapped into the
     int32_t* h_ring_meta = nullptr;                 // device: kernels write MFIs straight to the host
     int32_t* d_roots = nullptr;
@@ -100,7 +102,8 @@ struct sgfim_ctx {
     int32_t* d_sup = nullptr;
     int32_t* d_pair = nullptr;
     bool pair_valid = false;
-    std::vector<int32_t> h_pair;        // host copy of the pair matrix (d x d), filled lazily
+    int32_t* h_pair_pin = nullptr;      // pinned staging buffer for pair-matrix read-back (dpad x dpad)
+    int64_t h_pair_pin_n = 0;
     uint64_t* d_ok = nullptr;
     SgSegment* d_seg = nullptr;
     int64_t* d_seg_item_base = nullptr;
@@ -193,6 +196,8 @@ void bufs_free(sgfim_ctx* c)
     if (B->h_ring_meta) (void)hipHostFree(B->h_ring_meta);
     if (B->ev0) (void)hipEventDestroy(B->ev0);
     if (B->ev1) (void)hipEventDestroy(B->ev1);
+    for (int i = 0; i < 2; i++)
+        if (B->snap_ev[i]) (void)hipEventDestroy(B->snap_ev[i]);
     dev_free_all(c, B->bufs);
     delete B;
     c->SB = nullptr;
@@ -210,7 +215,6 @@ void data_free(sgfim_ctx* c)
     c->d_seg = nullptr;
     c->d_seg_item_base = nullptr;
     c->pair_valid = false;
-    c->h_pair.clear();
     c->win_thr.clear();
     c->loaded = false;
 }
@@ -249,21 +253,6 @@ int ensure_pair(sgfim_ctx* c)
         HIPCHK(c, sg_launch_pair_mfma_segs(c->d_rows, c->d_seg, c->nseg, c->dpad, maxw, c->d_pair, c->stream));
     }
     c->pair_valid = true;
-    c->h_pair.clear();
-    return SGFIM_OK;
-}
-
-int ensure_h_pair(sgfim_ctx* c)
-{
-    if (!c->h_pair.empty()) return SGFIM_OK;
-    int rc = ensure_pair(c);
-    if (rc) return rc;
-    std::vector<int32_t> full((size_t)c->dpad * c->dpad);
-    HIPCHK(c, hipMemcpyAsync(full.data(), c->d_pair, full.size() * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->h_pair.resize((size_t)c->d * c->d);
-    for (int i = 0; i < c->d; i++)
-        memcpy(&c->h_pair[(size_t)i * c->d], &full[(size_t)i * c->dpad], (size_t)c->d * 4);
     return SGFIM_OK;
 }
 
@@ -316,6 +305,14 @@ int read_ctl(sgfim_ctx* c, SearchState* S)
     return SGFIM_OK;
 }
 
+// Records waiting in the priority buckets of a control-block snapshot.
+int64_t frontier_size(const int64_t* h)
+{
+    int64_t f = 0;
+    for (int b = 0; b < SG_NBUCKETS; b++) f += h[CTL_BUCKET_TOP0 + b];
+    return f;
+}
+
 void fill_stats(sgfim_ctx* c, SearchState* S)
 {
     sgfim_round_stats& st = S->stats;
@@ -325,7 +322,7 @@ void fill_stats(sgfim_ctx* c, SearchState* S)
         st.nodes = h[CTL_NODES];
         st.counts = h[CTL_COUNTS];
         st.mfis = h[CTL_RING_POS];
-        st.frontier = h[CTL_FRONTIER];
+        st.frontier = frontier_size(h);
         st.words = h[CTL_WORDS];
     }
     st.round = S->rounds;
@@ -384,7 +381,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         const SgSearchDev& D = B->D;
         const bool fits = D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
                           D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.trace_cap == trace_cap &&
-                          B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + D.newborn_cap;
+                          B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + 2 * D.newborn_cap;
         if (fits) return SGFIM_OK;
         bufs_free(c);
     }
@@ -429,12 +426,12 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
             return rc;
         }
     }
-    if (hipHostMalloc((void**)&B->h_ctl, CTL_SLOTS * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&B->h_ctl, 3 * CTL_SLOTS * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         bufs_free(c);
         return fail(c, SGFIM_E_NOMEM, "pinned control block allocation failed");
     }
-    memset(B->h_ctl, 0, CTL_SLOTS * sizeof(int64_t));
+    memset(B->h_ctl, 0, 3 * CTL_SLOTS * sizeof(int64_t));
     if (hipHostMalloc((void**)&B->h_ring_items, (size_t)ring_cap * SG_MAX_DEPTH * 2, hipHostMallocMapped) != hipSuccess ||
         hipHostMalloc((void**)&B->h_ring_meta, (size_t)ring_cap * 16, hipHostMallocMapped) != hipSuccess) {
         (void)hipGetLastError();
@@ -445,6 +442,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     HIPCHK(c, hipHostGetDevicePointer((void**)&D.ring_meta, B->h_ring_meta, 0));
     HIPCHK(c, hipEventCreate(&B->ev0));
     HIPCHK(c, hipEventCreate(&B->ev1));
+    for (int i = 0; i < 2; i++) HIPCHK(c, hipEventCreateWithFlags(&B->snap_ev[i], hipEventDisableTiming));
     return SGFIM_OK;
 }
 
@@ -503,6 +501,7 @@ void sgfim_destroy(sgfim_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     data_free(c);
     dev_free_all(c, c->scorer_bufs);
+    if (c->h_pair_pin) (void)hipHostFree(c->h_pair_pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -823,8 +822,22 @@ static int pair_out(sgfim_ctx* c, int32_t* out, bool valu)
         return SGFIM_OK;
     }
     c->pair_valid = false;  // recompute: this entry point measures the MFMA path
-    if ((rc = ensure_h_pair(c))) return rc;
-    std::copy(c->h_pair.begin(), c->h_pair.end(), out);
+    if ((rc = ensure_pair(c))) return rc;
+    const int64_t n = (int64_t)c->dpad * c->dpad;
+    if (c->h_pair_pin_n < n) {
+        if (c->h_pair_pin) (void)hipHostFree(c->h_pair_pin);
+        c->h_pair_pin = nullptr;
+        c->h_pair_pin_n = 0;
+        if (hipHostMalloc((void**)&c->h_pair_pin, (size_t)n * 4, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            c->h_pair_pin = nullptr;
+            return fail(c, SGFIM_E_NOMEM, "pinned pair-matrix buffer allocation failed");
+        }
+        c->h_pair_pin_n = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->h_pair_pin, c->d_pair, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->d; i++) memcpy(out + (size_t)i * c->d, c->h_pair_pin + (size_t)i * c->dpad, (size_t)c->d * 4);
     return SGFIM_OK;
 }
 
@@ -1007,16 +1020,17 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
                                         (long long)cap, (long long)seed_records);
     // newborn_cap bounds the children of one round (each makes at most one record and emits
     // at most one MFI), so the MFI ring holds the seed's MFIs plus one round's
-    int64_t newborn_cap = std::max<int64_t>(seed_records, std::min<int64_t>(cap, (int64_t)1 << 20));
+    int64_t newborn_cap = std::max<int64_t>(seed_records, std::min<int64_t>(cap, (int64_t)1 << 19));
     // count cells of one round; at least one record's full tile (d x d) so every round advances
     const int64_t cell_cap = std::max<int64_t>((int64_t)1 << 26, (int64_t)d * d);
-    int64_t ring_cap = n_roots + newborn_cap;
+    // two rounds may be in flight before the host drains the ring
+    int64_t ring_cap = n_roots + 2 * newborn_cap;
     if (c->opts.ring_cap > 0) {
-        const int64_t need = n_roots + std::max<int64_t>(n_roots, d);
+        const int64_t need = n_roots + 2 * std::max<int64_t>(n_roots, d);
         if (c->opts.ring_cap < need)
             return fail(c, SGFIM_E_CAPACITY, "ring_cap must hold at least %lld MFIs", (long long)need);
         ring_cap = c->opts.ring_cap;
-        newborn_cap = std::min(newborn_cap, ring_cap - n_roots);
+        newborn_cap = std::min(newborn_cap, (ring_cap - n_roots) / 2);
     }
     int64_t maxw = 0;
     for (const SgSegment& g : c->segs) maxw = std::max(maxw, g.words);
@@ -1066,30 +1080,77 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
     if (!S || !S->active) return fail(c, SGFIM_E_STATE, "no anytime search in progress");
     (void)hipSetDevice(c->device);
     int rc = S->status;
-    for (int32_t i = 0; rc == SGFIM_OK && !S->done && (max_rounds <= 0 || i < max_rounds); i++) {
-        if (S->cancel) { rc = SGFIM_E_CANCELLED; break; }
-        if (S->args.deadline_s > 0 && now_s(S) > S->args.deadline_s) {
-            rc = fail(c, SGFIM_E_TIMEOUT, "deadline of %.3f s passed", S->args.deadline_s);
-            break;
+    if (rc == SGFIM_OK && !S->done && S->cancel) {
+        rc = S->status = fail(c, SGFIM_E_CANCELLED, "search cancelled");
+        S->done = true;
+    }
+    if (rc != SGFIM_OK || S->done) {
+        fill_stats(c, S);
+        if (st) *st = S->stats;
+        return rc;
+    }
+    SearchBufs* B = S->B;
+    // Rounds are pipelined two deep with captured graphs: round r+1 is queued before the
+    // host reads round r's control snapshot and drains its MFIs, so the host work and the
+    // launch latency hide behind the device's work. A round queued after the frontier
+    // emptied does nothing. Direct launches (event timing) run one round at a time.
+    const int depth = S->use_graph ? 2 : 1;
+    int64_t launched = 0;
+    int inflight = 0, head = 0;   // snapshots [head, head + inflight) mod 2 are pending
+    bool stop = false;
+    while (true) {
+        while (!stop && inflight < depth && (max_rounds <= 0 || launched < max_rounds)) {
+            if (S->cancel) { rc = fail(c, SGFIM_E_CANCELLED, "search cancelled"); stop = true; break; }
+            if (S->args.deadline_s > 0 && now_s(S) > S->args.deadline_s) {
+                rc = fail(c, SGFIM_E_TIMEOUT, "deadline of %.3f s passed", S->args.deadline_s);
+                stop = true;
+                break;
+            }
+            const int slot = (head + inflight) & 1;
+            int64_t* snap = B->h_ctl + (1 + slot) * CTL_SLOTS;
+            if ((rc = launch_round(c, S))) { stop = true; break; }
+            hipError_t e = hipMemcpyAsync(snap, B->D.ctl, CTL_SLOTS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(B->snap_ev[slot], c->stream);
+            if (e != hipSuccess) { rc = fail(c, SGFIM_E_DEVICE, "round snapshot: %s", hipGetErrorString(e)); stop = true; break; }
+            launched++;
+            inflight++;
         }
-        if ((rc = launch_round(c, S))) break;
-        if ((rc = read_ctl(c, S))) break;
-        S->rounds++;
-        if (S->B->h_ctl[CTL_UNITS] > 0) {
+        if (inflight == 0) break;
+        // process the oldest round in flight
+        const int slot = head;
+        head ^= 1;
+        inflight--;
+        hipError_t e = hipEventSynchronize(B->snap_ev[slot]);
+        if (e != hipSuccess) {
+            if (rc == SGFIM_OK) rc = fail(c, SGFIM_E_DEVICE, "round: %s", hipGetErrorString(e));
+            stop = true;
+            continue;
+        }
+        memcpy(B->h_ctl, B->h_ctl + (1 + slot) * CTL_SLOTS, CTL_SLOTS * sizeof(int64_t));
+        if (B->h_ctl[CTL_BATCH_N] > 0) S->rounds++;   // a round queued after the frontier emptied is not counted
+        if (B->h_ctl[CTL_UNITS] > 0 && B->h_ctl[CTL_BATCH_N] > 0) {
             // device-side duration of the counting kernel: first workgroup start to last
             // workgroup end, from the constant-rate wall clock
-            const int64_t t0 = S->B->h_ctl[CTL_T0], t1 = S->B->h_ctl[CTL_T1];
+            const int64_t t0 = B->h_ctl[CTL_T0], t1 = B->h_ctl[CTL_T1];
             if (t1 > t0 && c->wall_khz > 0) S->count_ms += (double)(t1 - t0) / (double)c->wall_khz;
             S->count_launches++;
             if (!S->use_graph) {
                 float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, S->B->ev0, S->B->ev1) == hipSuccess) S->event_ms += ms;
+                if (hipEventElapsedTime(&ms, B->ev0, B->ev1) == hipSuccess) S->event_ms += ms;
                 else (void)hipGetLastError();
             }
         }
-        if ((rc = check_err(c, S))) break;
-        if ((rc = drain_ring(c, S, S->B->h_ctl[CTL_RING_POS], now_s(S)))) break;
-        if (S->B->h_ctl[CTL_FRONTIER] == 0) S->done = true;
+        if (rc != SGFIM_OK) continue;   // an earlier failure: only wait for the rounds in flight
+        int r2;
+        if ((r2 = check_err(c, S)) || (r2 = drain_ring(c, S, B->h_ctl[CTL_RING_POS], now_s(S)))) {
+            rc = r2;
+            stop = true;
+            continue;
+        }
+        if (frontier_size(B->h_ctl) == 0) {
+            S->done = true;
+            stop = true;
+        }
     }
     if (rc != SGFIM_OK) {
         S->status = rc;

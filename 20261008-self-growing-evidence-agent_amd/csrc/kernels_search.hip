@@ -22,9 +22,9 @@
 // stay 0 (no superset of an infrequent pair is frequent).
 // Parent-equivalence pruning (exact): see sg_expand_node.
 //
-// One round = k_select -> k_candlist -> k_zero_cells -> k_count_group ->
-// k_expand_children -> [score] -> k_push -> k_release; every launch has a fixed shape
-// and reads its sizes from the control block, so the round is captured as one hipGraph.
+// One round = k_select -> k_candlist -> k_count_group -> k_expand_children -> [score]
+// -> k_commit; every launch has a fixed shape and reads its sizes from the control
+// block, so the round is captured as one hipGraph.
 #include "count_core.h"
 
 #define SEL_THREADS 1024
@@ -369,6 +369,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         s_total = taken;
         s_cut = taken;
         s_budget = min(S.ctl[CTL_FREE_TOP], S.newborn_cap);
+        S.ctl[CTL_ROUND] += 1;   // MFIs of this round carry its number; the seed's carry 0
         S.ctl[CTL_NEWBORN] = 0;
         S.ctl[CTL_UNIT_NEXT] = 0;
         S.ctl[CTL_T0] = 0x7fffffffffffffffll;
@@ -461,8 +462,12 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
 // ------------------------------------------------------------------ U lists
 // One wave per record: expand U into an ascending id list; the children are its
 // first ntail entries (U is ascending and children are the items below `last`).
+// The grid also clears the batch's count cells.
 __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
+    const int64_t ncell = S.ctl[CTL_CELLS];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (int64_t)gridDim.x * blockDim.x)
+        S.cells[i] = 0;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x * 4 + wv;
     if (b >= (int)S.ctl[CTL_BATCH_N]) return;
@@ -484,13 +489,6 @@ __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment
         S.clist[pos++] = lane * 64 + bit;
     }
     for (int j = lane; j < m.ntail; j += 64) S.ccount[S.choff[b] + j] = 0;
-}
-
-__global__ __launch_bounds__(256) void k_zero_cells(SgSearchDev S)
-{
-    const int64_t n = S.ctl[CTL_CELLS];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        S.cells[i] = 0;
 }
 
 // ------------------------------------------------------------------ counting tiles
@@ -794,12 +792,17 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
     }
 }
 
-// ------------------------------------------------------------------ push newborns into buckets
-__global__ void k_push(SgSearchDev S, const SgSegment* __restrict__ segs, int32_t order_mode,
-                       const float* __restrict__ scores)
+// ------------------------------------------------------------------ commit: push newborns, release the batch
+// Newborn records go onto their priority bucket; the batch's record slots go back to
+// the free list (their children have all been expanded). The two index spaces are
+// independent, so one grid-stride launch does both. The host sums the bucket tops.
+__global__ void k_commit(SgSearchDev S, const SgSegment* __restrict__ segs, int32_t order_mode,
+                         const float* __restrict__ scores, int32_t release)
 {
     const int64_t n = min(S.ctl[CTL_NEWBORN], S.newborn_cap);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t i = t0; i < n; i += stride) {
         const int32_t slot = S.newborn[i];
         if (slot < 0 || slot >= S.cap) continue;
         const SgNodeMeta m = S.meta[slot];
@@ -824,31 +827,12 @@ __global__ void k_push(SgSearchDev S, const SgSegment* __restrict__ segs, int32_
             }
         }
     }
-}
-
-// ------------------------------------------------------------------ release batch slots
-__global__ void k_release(SgSearchDev S)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nb = (int)S.ctl[CTL_BATCH_N];
-    if (i < nb) {
-        const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], 1ull);
-        if (pos >= 0 && pos < S.cap) S.free_idx[pos] = S.batch[i];   // out of range only after an overflow
-    }
-    if (i == 0) {
-        S.ctl[CTL_ROUND] += 1;
-        int64_t front = 0;
-        for (int b = 0; b < SG_NBUCKETS; b++) front += S.ctl[CTL_BUCKET_TOP0 + b];
-        S.ctl[CTL_FRONTIER] = front;
-    }
-}
-
-__global__ void k_frontier_sum(SgSearchDev S)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int64_t front = 0;
-        for (int b = 0; b < SG_NBUCKETS; b++) front += S.ctl[CTL_BUCKET_TOP0 + b];
-        S.ctl[CTL_FRONTIER] = front;
+    if (release) {
+        const int64_t nb = S.ctl[CTL_BATCH_N];
+        for (int64_t i = t0; i < nb; i += stride) {
+            const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], 1ull);
+            if (pos >= 0 && pos < S.cap) S.free_idx[pos] = S.batch[i];   // out of range only after an overflow
+        }
     }
 }
 
@@ -874,14 +858,14 @@ hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s)
     return hipGetLastError();
 }
 
-static hipError_t sg_push_newborn(const SgSearchDev& S, const SgSegment* segs, int32_t order_mode,
-                                  const SgScorer* scorer, hipStream_t s)
+static hipError_t sg_commit(const SgSearchDev& S, const SgSegment* segs, int32_t order_mode, const SgScorer* scorer,
+                            int32_t release, hipStream_t s)
 {
     if (order_mode == 1) {
         hipError_t e = sg_launch_score(scorer, S.feat, (int32_t)S.newborn_cap, &S.ctl[CTL_NEWBORN], S.scores, s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_push, dim3(1024), dim3(256), 0, s, S, segs, order_mode, (const float*)S.scores);
+    hipLaunchKernelGGL(k_commit, dim3(1024), dim3(256), 0, s, S, segs, order_mode, (const float*)S.scores, release);
     return hipGetLastError();
 }
 
@@ -890,16 +874,13 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
                           const int32_t* sup_all, const int64_t* seg_item_base, int32_t order_mode,
                           const SgScorer* scorer, hipStream_t s)
 {
-    hipError_t e = hipSuccess;
-    if (n_roots > 0) {
-        hipLaunchKernelGGL(k_zero_newborn, dim3(1), dim3(1), 0, s, S);
-        hipLaunchKernelGGL(k_seed, dim3((unsigned)((n_roots + 3) / 4)), dim3(256), 0, s, S, segs, pair, ok,
-                           roots, n_roots, sup_all, seg_item_base);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = sg_push_newborn(S, segs, order_mode, scorer, s)) != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_frontier_sum, dim3(1), dim3(64), 0, s, S);
-    return hipGetLastError();
+    if (n_roots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_newborn, dim3(1), dim3(1), 0, s, S);
+    hipLaunchKernelGGL(k_seed, dim3((unsigned)((n_roots + 3) / 4)), dim3(256), 0, s, S, segs, pair, ok, roots, n_roots,
+                       sup_all, seg_item_base);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sg_commit(S, segs, order_mode, scorer, 0, s);
 }
 
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
@@ -911,7 +892,6 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     hipError_t e;
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
-    hipLaunchKernelGGL(k_zero_cells, dim3(2048), dim3(256), 0, s, S);
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
     if (S.chunk_words == 256)
         hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
@@ -921,7 +901,6 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,
                        seg_item_base);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = sg_push_newborn(S, segs, order_mode, scorer, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_release, dim3((unsigned)((S.bmax + 255) / 256)), dim3(256), 0, s, S);
+    if ((e = sg_commit(S, segs, order_mode, scorer, 1, s)) != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -144,7 +144,7 @@ enum SgCtl {
     CTL_ERR,              // error flags: bit 0 capacity, bit 1 depth
     CTL_ROUND,            // rounds done
     CTL_NEWBORN,          // records born this round
-    CTL_FRONTIER,         // records waiting in buckets
+    CTL_SPARE,            // unused
     CTL_KEY,              // next record key
     CTL_TRACE_BIRTH,      // trace records written (births)
     CTL_TRACE_MFI,        // trace records written (MFIs)
