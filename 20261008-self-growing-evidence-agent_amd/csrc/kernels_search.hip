@@ -102,9 +102,12 @@ __device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
     return w < lw ? ~0ull : (w == lw ? ((1ull << lb) - 1ull) : 0ull);
 }
 
+#define SG_RC 4       // candidates held in registers per wave step
+
 // Counting units of a record: blocks of SG_CB children x ranges of SG_CR row chunks.
-__device__ __forceinline__ int32_t sg_tile_units(int32_t ntail, int64_t words, int32_t chunk)
+__device__ __forceinline__ int32_t sg_tile_units(int32_t ntail, int32_t nu, int64_t words, int32_t chunk)
 {
+    (void)nu;
     const int64_t nchunk = (words + chunk - 1) / chunk;
     return (int32_t)(((ntail + SG_CB - 1) / SG_CB) * ((nchunk + SG_CR - 1) / SG_CR));
 }
@@ -135,7 +138,7 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     m.parent_nf = nf_parent;
     m.ncand = nu;
     m.ntail = nt;
-    m.nunits = sg_tile_units(nt, sg.words, S.chunk_words);
+    m.nunits = sg_tile_units(nt, nu, sg.words, S.chunk_words);
     m.score = 0.0f;
     S.meta[slot] = m;
     S.key[slot] = key;
@@ -513,37 +516,88 @@ __device__ __forceinline__ uint32_t sg_wave_sum4(uint32_t v0, uint32_t v1, uint3
 }
 
 #define SG_TILE_THREADS 512
-#define SG_RC 4   // candidates held in registers per wave step
+#define SG_CW 256     // row words per chunk
+
+// Popcounts of x & c over the PER words a lane holds (all PER, or the first qn).
+template <int PER, bool FULL>
+__device__ __forceinline__ uint32_t sg_andpop(const uint64_t (&x)[PER], const uint64_t (&c)[PER], int qn)
+{
+    uint32_t a = 0u;
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+        if (FULL || q < qn) a += (uint32_t)__popcll(x[q] & c[q]);
+    return a;
+}
+
+// One wave step of a chunk: SG_RC listed candidates against the children that need
+// them. Candidate r's row chunk is loaded once into registers; for every child j that
+// needs one of them, the child prefix is read from LDS once and ANDed + popcounted with
+// each candidate that needs it; the four per-lane sums reduce together (sg_wave_sum4)
+// and lane 16r adds candidate r's total to its cell.
+template <int PER, bool FULL>
+__device__ __forceinline__ void sg_tile_step(const uint64_t* __restrict__ seg_rows, int64_t stride, int64_t w0,
+                                             int wn, int qn, uint64_t nm, const int32_t (&fr)[SG_RC], int il_mine,
+                                             const uint64_t (*cp)[SG_CW], int ncb, int32_t* cells, int j0, int nu)
+{
+    const int lane = threadIdx.x & 63;
+    const int rl = lane >> 4;
+    uint64_t x[SG_RC][PER];
+#pragma unroll
+    for (int rr = 0; rr < SG_RC; rr++) {
+        const bool any = ((nm >> rr) & 0x1111111111111111ull) != 0ull;   // some child needs candidate rr
+        const uint64_t* row = seg_rows + (int64_t)(fr[rr] < 0 ? 0 : fr[rr]) * stride + w0;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int w = lane + 64 * q;
+            x[rr][q] = (any && (FULL || w < wn)) ? row[w] : 0ull;
+        }
+    }
+    // keep the per-child masks inside the step: hoisted, their branch conditions would
+    // not fit the scalar registers
+    asm volatile("" : "+s"(nm));
+    for (int j = 0; j < ncb; j++) {
+        const uint32_t mj = (uint32_t)(nm >> (4 * j)) & 0xfu;   // candidates child j needs
+        if (mj == 0u) continue;
+        uint64_t c[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) c[q] = (FULL || q < qn) ? cp[j][lane + 64 * q] : 0ull;
+        const uint32_t a0 = (mj & 1u) ? sg_andpop<PER, FULL>(x[0], c, qn) : 0u;
+        const uint32_t a1 = (mj & 2u) ? sg_andpop<PER, FULL>(x[1], c, qn) : 0u;
+        const uint32_t a2 = (mj & 4u) ? sg_andpop<PER, FULL>(x[2], c, qn) : 0u;
+        const uint32_t a3 = (mj & 8u) ? sg_andpop<PER, FULL>(x[3], c, qn) : 0u;
+        const uint32_t t = sg_wave_sum4(a0, a1, a2, a3);
+        if ((lane & 15) == 0 && ((mj >> rl) & 1u) && t) atomicAdd(&cells[(int64_t)(j0 + j) * nu + il_mine], (int32_t)t);
+    }
+}
 
 // Unit (record g, block of up to SG_CB children, range of up to SG_CR row chunks).
 // Set-up, once per unit: the children's pair-ok rows give each candidate f of U the
 // mask of children e that need it (pair (e, f) frequent, f != e); candidates no child
 // needs are dropped, the others listed in LDS. Then per chunk: the prefix of X and the
-// prefixes of the children go to LDS; each wave takes SG_RC listed candidates at a
-// time, loads their row chunks once into registers (coalesced loads) and, for every
-// child that needs one of them, reads the child prefix from LDS once and ANDs +
-// popcounts it with each candidate that needs it. The four per-lane partial sums of a
-// child reduce together (sg_wave_sum4); one integer atomic per needed (child,
-// candidate, chunk). Skipped cells stay 0, which the expansion reads as infrequent --
-// exact, since an infrequent pair has no frequent superset. The child prefixes'
-// popcounts give the children's supports.
-template <int CW>
-__global__ __launch_bounds__(SG_TILE_THREADS) void k_count_group(SgSearchDev S, const uint64_t* __restrict__ rows,
+// prefixes of the children go to LDS, and each wave takes SG_RC listed candidates at a
+// time (sg_tile_step); lane (j, r) = (lane >> 2, lane & 3) of a step holds whether
+// child j needs candidate r, so the ballot keeps child j's four bits at 4j. One integer
+// atomic per needed (child, candidate, chunk). Skipped cells stay 0, which the
+// expansion reads as infrequent -- exact, since an infrequent pair has no frequent
+// superset. The child prefixes' popcounts give the children's supports.
+__global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_count_group(SgSearchDev S, const uint64_t* __restrict__ rows,
                                                                  const SgSegment* __restrict__ segs,
                                                                  const uint64_t* __restrict__ ok)
 {
+    constexpr int CW = SG_CW;
+    constexpr int PER = CW / 64;
+    constexpr int NW = SG_TILE_THREADS / 64;
+    static_assert(SG_RC * SG_CB == 64, "one lane per (child, candidate) pair of a wave step");
     __shared__ uint64_t pre[CW];
-    __shared__ uint64_t cpbuf[SG_CB * CW];   // child prefixes; the children's ok rows during set-up
+    __shared__ uint64_t cp[SG_CB][CW];       // child prefixes; the children's ok rows during set-up
     __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
     __shared__ int64_t s_unit;
     __shared__ int32_t s_items[SG_MAX_DEPTH];
     __shared__ int32_t s_child[SG_CB];
     __shared__ int32_t s_nc;
     static_assert(SG_CB * 64 <= SG_CB * CW, "ok rows fit the prefix buffer");
-    static_assert(SG_RC * SG_CB == 64, "one lane per (candidate, child) pair of a wave step");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    constexpr int NW = SG_TILE_THREADS / 64;
-    constexpr int PER = CW / 64;
+    uint64_t* okr = &cp[0][0];
     const int64_t n_units = S.ctl[CTL_UNITS];
     const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
     if (threadIdx.x == 0 && n_units > 0)
@@ -565,9 +619,9 @@ __global__ __launch_bounds__(SG_TILE_THREADS) void k_count_group(SgSearchDev S, 
         const int32_t slot = S.batch[g];
         const SgNodeMeta m = S.meta[slot];
         const SgSegment sg = segs[m.seg];
-        const int nblk = (m.ntail + SG_CB - 1) / SG_CB;
+        const int ncbk = (m.ntail + SG_CB - 1) / SG_CB;
         const int r = (int)(u - S.unit_off[g]);
-        const int cr = r / nblk, blk = r % nblk;
+        const int cr = r / ncbk, blk = r % ncbk;
         const int64_t nchunk = (sg.words + CW - 1) / CW;
         const int c0 = cr * SG_CR;
         const int c1 = (int)min((int64_t)(c0 + SG_CR), nchunk);
@@ -582,21 +636,20 @@ __global__ __launch_bounds__(SG_TILE_THREADS) void k_count_group(SgSearchDev S, 
         const uint64_t* okb = ok + sg.ok_base;
         for (int t = threadIdx.x; t < ncb * sg.dw; t += blockDim.x) {
             const int j = t / sg.dw, w = t - j * sg.dw;
-            cpbuf[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
+            okr[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
         }
         __syncthreads();
         for (int i = threadIdx.x; i < nu; i += blockDim.x) {
             const int32_t f = S.clist[u0 + i];
             uint32_t mask = 0u;
             for (int j = 0; j < ncb; j++)
-                if (s_child[j] != f && ((cpbuf[j * 64 + (f >> 6)] >> (f & 63)) & 1ull)) mask |= 1u << j;
+                if (s_child[j] != f && ((okr[j * 64 + (f >> 6)] >> (f & 63)) & 1ull)) mask |= 1u << j;
             if (mask) s_cand[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
         }
         __syncthreads();
         const int nc = s_nc;
         const uint64_t* seg_rows = rows + sg.row_base;
         int32_t* cells = S.cells + S.celloff[g];
-        const int rl = lane >> 4, jl = lane & 15;   // this lane's (candidate, child) of a step
         for (int c = c0; c < c1; c++) {
             const int64_t w0 = (int64_t)c * CW;
             const int wn = (int)min((int64_t)CW, sg.words - w0);
@@ -611,61 +664,36 @@ __global__ __launch_bounds__(SG_TILE_THREADS) void k_count_group(SgSearchDev S, 
                 pre[w] = v;
             }
             __syncthreads();
-            // child prefixes and child supports
+            // child prefixes (zero past the last word) and child supports
             for (int j = wave; j < ncb; j += NW) {
                 uint32_t acc = 0;
                 const uint64_t* row = seg_rows + (int64_t)s_child[j] * sg.stride + w0;
 #pragma unroll
                 for (int q = 0; q < PER; q++) {
-                    if (q < qn) {
-                        const int w = lane + 64 * q;
-                        const uint64_t v = w < wn ? (pre[w] & row[w]) : 0ull;
-                        cpbuf[j * CW + w] = v;
-                        acc += (uint32_t)__popcll(v);
-                    }
+                    const int w = lane + 64 * q;
+                    const uint64_t v = w < wn ? (pre[w] & row[w]) : 0ull;
+                    cp[j][w] = v;
+                    acc += (uint32_t)__popcll(v);
                 }
-                const uint32_t s = sg_wave_sum(acc);
-                if (lane == 0 && s) atomicAdd(&S.ccount[S.choff[g] + j0 + j], (int32_t)s);
+                const uint32_t sv = sg_wave_sum(acc);
+                if (lane == 0 && sv) atomicAdd(&S.ccount[S.choff[g] + j0 + j], (int32_t)sv);
             }
             __syncthreads();
             for (int i0 = wave * SG_RC; i0 < nc; i0 += NW * SG_RC) {
-                const uint32_t ent = i0 + rl < nc ? s_cand[i0 + rl] : 0u;
-                const int il = (int)(ent & 0xffffu);
-                const int32_t fl = ent ? S.clist[u0 + il] : -1;
-                const uint64_t nm = __ballot((ent >> (16 + jl)) & 1u);   // bit 16*r + j
-                uint64_t x[SG_RC][PER];
+                // lane (j, r): does child j need candidate r of this step?
+                const int rq = lane & 3, jq = lane >> 2;
+                const uint32_t ent = i0 + rq < nc ? s_cand[i0 + rq] : 0u;
+                const int32_t fl = ent ? S.clist[u0 + (int)(ent & 0xffffu)] : -1;
+                const uint64_t nm = __ballot((ent >> (16 + jq)) & 1u);   // bit 4*j + r
+                int32_t fr[SG_RC];
 #pragma unroll
-                for (int rr = 0; rr < SG_RC; rr++) {
-                    const bool any = ((nm >> (16 * rr)) & 0xffffull) != 0ull;
-                    const int32_t f = __builtin_amdgcn_readlane(fl, 16 * rr);
-                    const uint64_t* row = seg_rows + (int64_t)(f < 0 ? 0 : f) * sg.stride + w0;
-#pragma unroll
-                    for (int q = 0; q < PER; q++) {
-                        const int w = lane + 64 * q;
-                        x[rr][q] = (any && q < qn && w < wn) ? row[w] : 0ull;
-                    }
-                }
-                for (int j = 0; j < ncb; j++) {
-                    const uint32_t mj = (uint32_t)((nm >> j) & 1ull) | (uint32_t)((nm >> (15 + j)) & 2ull) |
-                                        (uint32_t)((nm >> (30 + j)) & 4ull) | (uint32_t)((nm >> (45 + j)) & 8ull);
-                    if (mj == 0u) continue;
-                    uint64_t cj[PER];
-#pragma unroll
-                    for (int q = 0; q < PER; q++) cj[q] = q < qn ? cpbuf[j * CW + lane + 64 * q] : 0ull;
-                    uint32_t a[SG_RC];
-#pragma unroll
-                    for (int rr = 0; rr < SG_RC; rr++) {
-                        a[rr] = 0u;
-                        if ((mj >> rr) & 1u) {
-#pragma unroll
-                            for (int q = 0; q < PER; q++)
-                                if (q < qn) a[rr] += (uint32_t)__popcll(x[rr][q] & cj[q]);
-                        }
-                    }
-                    const uint32_t t = sg_wave_sum4(a[0], a[1], a[2], a[3]);
-                    if (jl == 0 && ((mj >> rl) & 1u) && t)
-                        atomicAdd(&cells[(int64_t)(j0 + j) * nu + il], (int32_t)t);
-                }
+                for (int rr = 0; rr < SG_RC; rr++) fr[rr] = __builtin_amdgcn_readlane(fl, rr);
+                const uint32_t ent_mine = i0 + (lane >> 4) < nc ? s_cand[i0 + (lane >> 4)] : 0u;
+                const int il_mine = (int)(ent_mine & 0xffffu);
+                if (qn == PER)
+                    sg_tile_step<PER, true>(seg_rows, sg.stride, w0, wn, qn, nm, fr, il_mine, cp, ncb, cells, j0, nu);
+                else
+                    sg_tile_step<PER, false>(seg_rows, sg.stride, w0, wn, qn, nm, fr, il_mine, cp, ncb, cells, j0, nu);
             }
             __syncthreads();
         }
@@ -840,11 +868,9 @@ __global__ void k_commit(SgSearchDev S, const SgSegment* __restrict__ segs, int3
 // Resident workgroups per CU of the tile kernel (sizes its persistent grid).
 int sg_count_group_blocks_per_cu(int32_t chunk_words)
 {
+    (void)chunk_words;
     int n = 0;
-    hipError_t e = chunk_words == 256
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<256>, SG_TILE_THREADS, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<512>, SG_TILE_THREADS, 0);
-    if (e != hipSuccess || n < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group, SG_TILE_THREADS, 0) != hipSuccess || n < 1) {
         (void)hipGetLastError();
         n = 1;
     }
@@ -893,10 +919,7 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-    if (S.chunk_words == 256)
-        hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
-    else
-        hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
+    hipLaunchKernelGGL(k_count_group, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
     if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,
                        seg_item_base);
