@@ -102,6 +102,8 @@ struct sgfim_ctx {
     int32_t* d_sup = nullptr;
     int32_t* d_pair = nullptr;
     bool pair_valid = false;
+    hipEvent_t pair_ev[2] = {nullptr, nullptr};   // around the latest pair-matrix computation
+    bool pair_timed = false;
     int32_t* h_pair_pin = nullptr;      // pinned staging buffer for pair-matrix read-back (dpad x dpad)
     int64_t h_pair_pin_n = 0;
     uint64_t* d_ok = nullptr;
@@ -244,6 +246,7 @@ int ensure_pair(sgfim_ctx* c)
         int rc = dev_alloc_t(c, c->data_bufs, &c->d_pair, (size_t)c->dpad * c->dpad * c->nseg);
         if (rc) return rc;
     }
+    const bool timed = c->pair_ev[0] && hipEventRecord(c->pair_ev[0], c->stream) == hipSuccess;
     if (c->nseg == 1) {
         HIPCHK(c, sg_launch_pair_mfma(c->d_rows, c->dpad, c->Wp, c->W, c->d_pair, c->stream));
     } else {
@@ -252,6 +255,7 @@ int ensure_pair(sgfim_ctx* c)
         HIPCHK(c, hipMemsetAsync(c->d_pair, 0, (size_t)c->dpad * c->dpad * c->nseg * 4, c->stream));
         HIPCHK(c, sg_launch_pair_mfma_segs(c->d_rows, c->d_seg, c->nseg, c->dpad, maxw, c->d_pair, c->stream));
     }
+    c->pair_timed = timed && hipEventRecord(c->pair_ev[1], c->stream) == hipSuccess;
     c->pair_valid = true;
     return SGFIM_OK;
 }
@@ -490,6 +494,11 @@ int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out)
         c->wall_khz = 0;
     }
     if (prop.multiProcessorCount > 0) c->n_cu = prop.multiProcessorCount;
+    for (int i = 0; i < 2; i++)
+        if (hipEventCreate(&c->pair_ev[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            c->pair_ev[i] = nullptr;
+        }
     *out = c;
     return SGFIM_OK;
 }
@@ -502,6 +511,8 @@ void sgfim_destroy(sgfim_ctx* c)
     data_free(c);
     dev_free_all(c, c->scorer_bufs);
     if (c->h_pair_pin) (void)hipHostFree(c->h_pair_pin);
+    for (int i = 0; i < 2; i++)
+        if (c->pair_ev[i]) (void)hipEventDestroy(c->pair_ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -842,6 +853,19 @@ static int pair_out(sgfim_ctx* c, int32_t* out, bool valu)
 }
 
 int sgfim_pair_matrix(sgfim_ctx* c, int32_t* out) { return pair_out(c, out, false); }
+
+int sgfim_pair_time(sgfim_ctx* c, double* ms)
+{
+    if (!c || !ms) return SGFIM_E_INVAL;
+    *ms = 0.0;
+    if (!c->pair_timed) return SGFIM_OK;
+    float f = 0.0f;
+    hipError_t e = hipEventSynchronize(c->pair_ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, c->pair_ev[0], c->pair_ev[1]);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "pair timing: %s", hipGetErrorString(e));
+    *ms = (double)f;
+    return SGFIM_OK;
+}
 int sgfim_pair_matrix_valu(sgfim_ctx* c, int32_t* out) { return pair_out(c, out, true); }
 
 int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
@@ -1013,7 +1037,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     // search buffers (reused across runs when they fit)
     const int32_t bmax = std::min(SG_BMAX, c->opts.batch_nodes > 0 ? c->opts.batch_nodes : SG_BMAX);
     int64_t cap = c->opts.frontier_cap > 0 ? c->opts.frontier_cap
-                                           : std::max<int64_t>({(int64_t)1 << 16, 4 * seed_records + 4 * bmax + 4 * (int64_t)d,
+                                           : std::max<int64_t>({(int64_t)1 << 20, 4 * seed_records + 4 * bmax + 4 * (int64_t)d,
                                                                 c->cap_hint});
     if (cap > ((int64_t)1 << 31) - 1) cap = ((int64_t)1 << 31) - 1;
     if (seed_records > cap) return fail(c, SGFIM_E_CAPACITY, "frontier_cap %lld < %lld root records",

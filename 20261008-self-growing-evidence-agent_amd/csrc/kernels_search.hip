@@ -104,12 +104,11 @@ __device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
 
 #define SG_RC 4       // candidates held in registers per wave step
 
-// Counting units of a record: blocks of SG_CB children x ranges of SG_CR row chunks.
-__device__ __forceinline__ int32_t sg_tile_units(int32_t ntail, int32_t nu, int64_t words, int32_t chunk)
+// Row-chunk ranges of SG_CR chunks of a window of `words` words.
+__device__ __forceinline__ int32_t sg_ranges(int64_t words, int32_t chunk)
 {
-    (void)nu;
     const int64_t nchunk = (words + chunk - 1) / chunk;
-    return (int32_t)(((ntail + SG_CB - 1) / SG_CB) * ((nchunk + SG_CR - 1) / SG_CR));
+    return (int32_t)((nchunk + SG_CR - 1) / SG_CR);
 }
 
 // Write record X (items px[0..k), support, tail bound `last`, extensions U in LDS) to slot.
@@ -138,7 +137,8 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     m.parent_nf = nf_parent;
     m.ncand = nu;
     m.ntail = nt;
-    m.nunits = sg_tile_units(nt, nu, sg.words, S.chunk_words);
+    m.nblk = (nt + SG_CB - 1) / SG_CB;
+    m.nrange = sg_ranges(sg.words, S.chunk_words);
     m.score = 0.0f;
     S.meta[slot] = m;
     S.key[slot] = key;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
     __shared__ int32_t take[SG_NBUCKETS], tops[SG_NBUCKETS], ends[SG_NBUCKETS];
     __shared__ int64_t wsum[SEL_THREADS / 64];
     __shared__ int64_t s_budget, tot[4];
-    __shared__ int32_t s_total, s_cut;
+    __shared__ int32_t s_total, s_cut, s_maxr;
     const int tid = threadIdx.x;
     if (tid < SG_NBUCKETS) tops[tid] = (int32_t)S.ctl[CTL_BUCKET_TOP0 + tid];
     __syncthreads();
@@ -371,6 +371,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         }
         s_total = taken;
         s_cut = taken;
+        s_maxr = 0;
         s_budget = min(S.ctl[CTL_FREE_TOP], S.newborn_cap);
         S.ctl[CTL_ROUND] += 1;   // MFIs of this round carry its number; the seed's carry 0
         S.ctl[CTL_NEWBORN] = 0;
@@ -407,11 +408,11 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             su += m.ncand;
             sc += m.ntail;
             scell += (int64_t)m.ncand * m.ntail;
-            sunit += m.nunits;
+            sunit += m.nblk;
             S.batch[i] = nid[q];
             S.coff[i] = m.ncand;
             S.choff[i] = m.ntail;
-            S.unit_off[i] = m.nunits;
+            S.unit_off[i] = m.nblk;
         }
     }
     int64_t eu = sg_block_excl_scan(su, wsum, &tot[0]);
@@ -439,6 +440,16 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         }
     }
     __syncthreads();
+    {
+        // longest window of the records that made the cut: units = blocks x its ranges
+        const int cut = s_cut;
+        int32_t mr = 0;
+#pragma unroll
+        for (int q = 0; q < PER; q++)
+            if (tid * PER + q < cut) mr = max(mr, S.meta[nid[q]].nrange);
+        if (mr) atomicMax(&s_maxr, mr);
+    }
+    __syncthreads();
     if (tid == 0) {
         const int cut = s_cut;
         if (cut == total) {   // every taken record fits: the end offsets are the totals
@@ -457,7 +468,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         S.ctl[CTL_BATCH_N] = cut;
         S.ctl[CTL_CHILDREN] = nch;
         S.ctl[CTL_CELLS] = S.celloff[cut];
-        S.ctl[CTL_UNITS] = S.unit_off[cut];
+        S.ctl[CTL_BLOCKS] = S.unit_off[cut];
+        S.ctl[CTL_UNITS] = S.unit_off[cut] * (int64_t)s_maxr;
         S.ctl[CTL_NODES] += nch;
     }
 }
@@ -599,6 +611,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint64_t* okr = &cp[0][0];
     const int64_t n_units = S.ctl[CTL_UNITS];
+    const int64_t n_blocks = S.ctl[CTL_BLOCKS];
     const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
     if (threadIdx.x == 0 && n_units > 0)
         atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
@@ -610,18 +623,24 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         __syncthreads();
         const int64_t u = s_unit;
         if (u >= n_units) break;
+        // range-major order: all child blocks of the batch work on row-chunk range cr
+        // together, so the rows of one range stay in L2 while every record uses them
+        const int cr = (int)(u / n_blocks);
+        const int64_t bq = u - (int64_t)cr * n_blocks;
         int lo = 0, hi = nb - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (S.unit_off[mid] <= u) lo = mid; else hi = mid - 1;
+            if (S.unit_off[mid] <= bq) lo = mid; else hi = mid - 1;
         }
         const int g = lo;
         const int32_t slot = S.batch[g];
         const SgNodeMeta m = S.meta[slot];
+        if (cr >= m.nrange) {   // past this record's (shorter) window
+            __syncthreads();
+            continue;
+        }
         const SgSegment sg = segs[m.seg];
-        const int ncbk = (m.ntail + SG_CB - 1) / SG_CB;
-        const int r = (int)(u - S.unit_off[g]);
-        const int cr = r / ncbk, blk = r % ncbk;
+        const int blk = (int)(bq - S.unit_off[g]);
         const int64_t nchunk = (sg.words + CW - 1) / CW;
         const int c0 = cr * SG_CR;
         const int c1 = (int)min((int64_t)(c0 + SG_CR), nchunk);

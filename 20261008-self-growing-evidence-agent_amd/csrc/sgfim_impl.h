@@ -46,7 +46,8 @@ struct SgNodeMeta {
     int32_t parent_nf;  // |F(parent)|
     int32_t ncand;      // |U|
     int32_t ntail;      // children: |U below last|
-    int32_t nunits;     // counting units of the record's tile (child blocks x chunk ranges)
+    int32_t nblk;       // blocks of children in the record's counting tiles
+    int32_t nrange;     // row-chunk ranges of the record's window
     float score;        // priority score
 };
 
@@ -98,7 +99,7 @@ struct SgSearchDev {
     // batch
     int32_t* batch;      // [bmax + 1]
     int32_t* coff;       // [bmax + 1] candidate offsets
-    int64_t* unit_off;   // [bmax + 1] work-unit offsets
+    int64_t* unit_off;   // [bmax + 1] child-block offsets of the batch's records
     int32_t* clist;      // [bmax * d_max] U lists of the batch (ascending ids)
     int32_t* ccount;     // [bmax * d_max] child supports |T(X) & T(e)|
     int32_t* choff;      // [bmax + 1] child offsets
@@ -144,7 +145,7 @@ enum SgCtl {
     CTL_ERR,              // error flags: bit 0 capacity, bit 1 depth
     CTL_ROUND,            // rounds done
     CTL_NEWBORN,          // records born this round
-    CTL_SPARE,            // unused
+    CTL_BLOCKS,           // child blocks of the current batch (units = blocks x chunk ranges)
     CTL_KEY,              // next record key
     CTL_TRACE_BIRTH,      // trace records written (births)
     CTL_TRACE_MFI,        // trace records written (MFIs)

@@ -71,6 +71,7 @@ _SIGS = {
     "sgfim_item_supports": (C.c_int, [P, P]),
     "sgfim_pair_matrix": (C.c_int, [P, P]),
     "sgfim_pair_matrix_valu": (C.c_int, [P, P]),
+    "sgfim_pair_time": (C.c_int, [P, C.POINTER(C.c_double)]),
     "sgfim_count_batch": (C.c_int, [P, P, P, C.c_int32, P, P, P]),
     "sgfim_mine": (C.c_int, [P, C.POINTER(MineArgs), PP]),
     "sgfim_anytime_begin": (C.c_int, [P, C.POINTER(MineArgs)]),

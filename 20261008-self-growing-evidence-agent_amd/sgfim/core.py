@@ -238,6 +238,12 @@ class Context:
         self._check(self._L.sgfim_mine(self._h, C.byref(a), C.byref(h)), "mine")
         return _take_result(h, self.last_stats(), convert)
 
+    def pair_time(self) -> float:
+        """Device milliseconds of the most recent pair-matrix computation (0 if none)."""
+        ms = C.c_double()
+        self._check(self._L.sgfim_pair_time(self._h, C.byref(ms)), "pair_time")
+        return ms.value
+
     def copy_probe(self, nbytes: int = 1 << 31, iters: int = 5) -> float:
         """On-device copy bandwidth in GB/s (bytes read + written)."""
         g = C.c_double()

@@ -22,10 +22,11 @@ sys.path.insert(0, os.path.join(ROOT, "20261008-self-growing-evidence-agent_amd"
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_count_search dispatch for this command, from the two rocprofv3 --pmc
+INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
+# HBM bytes per k_count_group dispatch for this command, from the two rocprofv3 --pmc
 # passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
 # what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
-# profiles/k_count_search_workload2_pmc.md holds the commands and the table.
+# profiles/k_count_group_workload2_pmc.md holds the commands and the table.
 PMC_COUNT_KERNEL = {"fetch_kib": 63594.2, "write_kib": 23659.7, "fetch_scale": 2.0, "write_scale": 1.0}
 SUPPORTS = (0.10, 0.05, 0.02)
 
@@ -102,7 +103,7 @@ def workload5_run(Context, budget_s=8.0, seed=0):
     """10M txns x 2,048 Zipf items (2.56 GB of bitmaps, 10x the MALL): a budgeted anytime run
     down to the 0.5% floor; reports the counting kernel against the HBM roofline."""
     from sgfim import gen
-    ctx = Context(device=0)
+    ctx = Context(device=0, frontier_cap=1 << 23)   # 8M frontier records (~5 GB at 2,048 items)
     t0 = time.perf_counter()
     ctx.load_zipf(seed=seed, **gen.WORKLOAD5)
     log(f"workload 5 generated on the GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words})")
@@ -166,8 +167,11 @@ def main():
     log(f"loaded + transposed on GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words} words)")
     thrs = [gen.min_count(s, T.n_txn) for s in SUPPORTS]
 
+    pair_ms = []
+
     def step():
         ctx.pair_matrix()                      # int8 MFMA pair matrix, recomputed every step
+        pair_ms.append(ctx.pair_time())
         counts = ctx.n_items * ctx.n_items
         words = 0
         cms = 0.0
@@ -186,6 +190,7 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
+    pair_ms.clear()
     tot_counts = tot_words = tot_launch = 0
     tot_cms = 0.0
     t0 = time.perf_counter()
@@ -206,6 +211,9 @@ def main():
                    "launches": rx["count_launches"]}
     copy_gbs = ctx.copy_probe(1 << 31, 5)   # on-box HBM copy rate, 2 GiB buffers (8x the MALL)
     value = tot_counts / dt
+    pair_avg = sum(pair_ms) / len(pair_ms) if pair_ms else 0.0
+    pair_ops = 2.0 * ctx.n_items * ctx.n_items * T.n_txn
+    pair_tops = pair_ops / (pair_avg * 1e-3) / 1e12 if pair_avg > 0 else None
     avg_ms = tot_cms / max(tot_launch, 1)
     bytes_per_launch = 8.0 * tot_words / max(tot_launch, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -229,10 +237,14 @@ def main():
                      "traffic": 1024.0 * (PMC_COUNT_KERNEL["fetch_scale"] * PMC_COUNT_KERNEL["fetch_kib"] +
                                           PMC_COUNT_KERNEL["write_scale"] * PMC_COUNT_KERNEL["write_kib"]),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command, "
-                                       "calibrated by the copy probe (profiles/k_count_search_workload2_pmc.md)",
-                     "kernel": "k_count_search", "avg_launch_ms": avg_ms,
+                                       "calibrated by the copy probe (profiles/k_count_group_workload2_pmc.md)",
+                     "kernel": "k_count_group", "avg_launch_ms": avg_ms,
                      "alg_bytes_per_launch": bytes_per_launch, "timing_check": event_check,
                      "copy_probe_gbs": copy_gbs},
+        # the pair matrix (part of every step): 2 d^2 N int8 ops (the full square) per launch
+        "pair_roofline": {"bound": "mfma", "kernel": "k_pair_tile", "avg_launch_ms": pair_avg,
+                          "achieved": pair_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                          "frac": pair_tops / INT8_PEAK_TOPS if pair_tops else None},
     }
     ctx.close()
     if not args.no_extra:

@@ -79,12 +79,16 @@ typedef struct sgfim_mine_args {
 /* Round statistics reported by the anytime driver. */
 typedef struct sgfim_round_stats {
     int64_t round;           /* rounds completed                                           */
-    int64_t nodes;           /* search nodes processed so far                              */
+    int64_t nodes;           /* search nodes processed so far (roots, then every child
+                                X u {e} of a frontier record that a round counted)         */
     int64_t counts;          /* support counts (itemset, item) computed by AND+popcount    */
     int64_t mfis;            /* MFIs confirmed so far                                      */
-    int64_t frontier;        /* nodes waiting in the frontier                              */
+    int64_t frontier;        /* records (itemset X with its frequent extensions U) waiting
+                                in the frontier; a round counts all children of a record   */
     int64_t words;           /* algorithmic 64-bit words of the counting kernel so far:
-                                (candidates + 1) * W per processed node                    */
+                                (needed counts + 1) * W per child, W = the child's window
+                                words; a needed count is a candidate f of U \ {e} whose
+                                pair (e, f) is frequent                                    */
     int64_t pair_counts;     /* counts taken from the pair matrix (roots x items)          */
     int64_t count_launches;  /* launches of the counting kernel                            */
     double count_ms;         /* summed duration of the counting kernel, from the device's
@@ -156,6 +160,11 @@ int sgfim_item_supports(sgfim_ctx* ctx, int64_t* out);
  * first BMMA wave over the single-item roots (kernel_bmma.cu:772-872,1224-1236). */
 int sgfim_pair_matrix(sgfim_ctx* ctx, int32_t* out);
 
+/* Device duration in ms of the most recent int8-MFMA pair-matrix computation of the
+ * context (HIP events around its kernels; 0 if none yet). No reference counterpart:
+ * the measurement of the pair path. */
+int sgfim_pair_time(sgfim_ctx* ctx, double* ms);
+
 /* Same matrix by the VALU AND+popcount kernel (cross-check path). */
 int sgfim_pair_matrix_valu(sgfim_ctx* ctx, int32_t* out);
 
@@ -215,8 +224,9 @@ int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
 int sgfim_set_scorer(sgfim_ctx* ctx, const float* w1, const float* b1, const float* w2, float b2);
 
 /* Search trace for training the proposal network. With tracing on, the next
- * runs record every node born (its key, its parent's key and its 24 features)
- * and the key of every confirmed MFI, up to cap records each. sgfim_trace_get
+ * runs record every frontier record born (its key, its parent record's key and its
+ * 24 features) and, for every confirmed MFI, the key of the record whose round
+ * confirmed it (-1 for MFIs found by the seed), up to cap records each. sgfim_trace_get
  * returns the counts, and copies the records when the output arrays are given
  * (keys: 2 per birth; feat: 24 per birth; cap = capacity of the outputs). */
 int sgfim_set_trace(sgfim_ctx* ctx, int32_t enable, int64_t cap);
