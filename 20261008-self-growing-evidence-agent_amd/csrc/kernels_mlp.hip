@@ -151,8 +151,10 @@ hipError_t sg_launch_score(const SgScorer* sc, const float* feat, int32_t n_max,
                            const int64_t* n_dev, float* out, hipStream_t s)
 {
     if (!sc || !sc->w1) return hipErrorInvalidValue;
+    // the row count is read on the device (a round's newborns); a modest grid loops over
+    // the tiles, so a round with few newborns does not pay for thousands of idle groups
     int64_t blocks = ((int64_t)n_max + 127) / 128;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > 256) blocks = 256;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_mlp_score, dim3((unsigned)blocks), dim3(256), 0, s, sc->w1, sc->b1, sc->w2, sc->b2,
                        feat, n_max, n_dev, out, 0);

@@ -111,23 +111,55 @@ __device__ __forceinline__ int32_t sg_ranges(int64_t words, int32_t chunk)
     return (int32_t)((nchunk + SG_CR - 1) / SG_CR);
 }
 
-// Write record X (items px[0..k), support, tail bound `last`, extensions U in LDS) to slot.
+__device__ __forceinline__ int32_t sg_wave_imin(int32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int32_t sg_wave_imax(int32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int32_t sg_wave_isum(int32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float sg_wave_fsum(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Write record X (items px[0..k), support, tail bound `last`, extensions U in LDS) to
+// slot. Called by all 64 lanes of a wave (dw <= 64 words of U, k <= 32 items: one per
+// lane); lane 0 writes the metadata and the features.
 __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const int32_t* __restrict__ item_sup,
                                const int32_t* __restrict__ pair, int32_t slot, const uint16_t* px, int k,
                                int32_t sup, int32_t last, int seg, const uint64_t* U, int32_t sup_parent,
                                int32_t nf_parent, int32_t ntail_parent, int parent_is_root, float* feat_out,
                                int64_t key, int64_t parent_key)
 {
-    uint64_t* cm = S.cand + (int64_t)slot * S.dw_max;
-    int nu = 0, nt = 0;
-    for (int w = 0; w < sg.dw; w++) {
-        const uint64_t v = U[w];
-        cm[w] = v;
-        nu += __popcll(v);
-        nt += __popcll(v & sg_low_mask(w, last));
+    const int lane = threadIdx.x & 63;
+    const uint64_t v = lane < sg.dw ? U[lane] : 0ull;
+    if (lane < sg.dw) S.cand[(int64_t)slot * S.dw_max + lane] = v;
+    const int nu = sg_wave_isum(__popcll(v));
+    const int nt = sg_wave_isum(__popcll(v & sg_low_mask(lane, last)));
+    if (lane < k) S.items[(int64_t)slot * SG_MAX_DEPTH + lane] = px[lane];
+    // features: minimum / maximum / log-sum of the item supports, smallest pair support with e
+    int32_t mn = 0, mx = 0, pmin = 0;
+    float sl = 0.0f;
+    if (feat_out) {
+        const int e = last;
+        const int32_t x = lane < k ? (int32_t)px[lane] : -1;
+        const int32_t sx = x >= 0 ? item_sup[x] : 0;
+        mn = sg_wave_imin(x >= 0 ? sx : 0x7fffffff);
+        mx = sg_wave_imax(x >= 0 ? sx : 0);
+        sl = sg_wave_fsum(x >= 0 ? __log2f((float)sx) : 0.0f);
+        pmin = sg_wave_imin((x >= 0 && x != e) ? pair[sg.pair_base + (int64_t)e * sg.dpad + x] : 0x7fffffff);
     }
-    uint16_t* ci = S.items + (int64_t)slot * SG_MAX_DEPTH;
-    for (int i = 0; i < k; i++) ci[i] = px[i];
+    if (lane != 0) return;
     SgNodeMeta m;
     m.k = k;
     m.sup = sup;
@@ -145,17 +177,8 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     S.pkey[slot] = parent_key;
     if (feat_out) {
         const int e = last;
-        int32_t mn = item_sup[e], mx = item_sup[e], pmin = 0x7fffffff;
-        float sl = 0.0f;
-        const int32_t* pe = pair + sg.pair_base + (int64_t)e * sg.dpad;
-        for (int i = 0; i < k; i++) {
-            const int32_t x = px[i];
-            const int32_t sx = item_sup[x];
-            mn = min(mn, sx);
-            mx = max(mx, sx);
-            sl += __log2f((float)sx);
-            if (x != e) pmin = min(pmin, pe[x]);
-        }
+        mn = min(mn, item_sup[e]);
+        mx = max(mx, item_sup[e]);
         SgFeatIn f;
         f.sup_y = sup; f.sup_x = sup_parent > 0 ? sup_parent : sup; f.k_y = k; f.nf_x = nf_parent; f.ncand_y = nu;
         f.tail_y = nt; f.thr = sg.thr; f.n = sg.n; f.d = sg.d; f.e = e; f.sup_e = item_sup[e];
@@ -299,20 +322,28 @@ __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __
     const int32_t* isup = sup_all + seg_item_base[seg];
     int k = 1;
     const int act = sg_node_decide(S, sg, sF[wv], sE[wv], sX[wv], k, r);
-    if (lane != 0) return;
-    atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], 1ull);
+    if (lane == 0) atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], 1ull);
     if (act == SG_ACT_EMIT) {
-        const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], 1ull);
-        sg_emit_at(S, pos, sX[wv], k, isup[r], seg, -1);
+        if (lane == 0) {
+            const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], 1ull);
+            sg_emit_at(S, pos, sX[wv], k, isup[r], seg, -1);
+        }
     } else if (act == SG_ACT_RECORD) {
-        const int64_t nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], 1ull);
-        const int64_t top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-1ll));
-        const int32_t slot = sg_take_slot(S, nb, top);
+        int64_t nb = 0, key = 0;
+        int32_t slot = -1;
+        if (lane == 0) {
+            nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], 1ull);
+            const int64_t top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-1ll));
+            slot = sg_take_slot(S, nb, top);
+            if (slot >= 0) key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], 1ull);
+        }
+        nb = __shfl(nb, 0, 64);
+        key = __shfl(key, 0, 64);
+        slot = __shfl(slot, 0, 64);
         if (slot >= 0) {
-            const int64_t key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], 1ull);
             float* fo = S.feat_on ? S.feat + nb * SG_FEAT : nullptr;
             sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, key, -1);
-            S.newborn[nb] = slot;
+            if (lane == 0) S.newborn[nb] = slot;
         }
     }
 }
@@ -811,22 +842,24 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             }
         }
         __syncthreads();
-        if (act != SG_ACT_NONE && lane == 0) {
+        if (act != SG_ACT_NONE) {   // wave-uniform
             int rank = 0;
             for (int w = 0; w < wv; w++) rank += s_act[w] == act;
             const int32_t slot = S.batch[g];
             const SgNodeMeta m = S.meta[slot];
             const SgSegment sg = segs[m.seg];
             if (act == SG_ACT_EMIT) {
-                sg_emit_at(S, s_ring + rank, sX[wv], k, sup_e, m.seg, S.key[slot]);
+                if (lane == 0) sg_emit_at(S, s_ring + rank, sX[wv], k, sup_e, m.seg, S.key[slot]);
             } else {
                 const int64_t nbi = s_nb + rank;
-                const int32_t nslot = sg_take_slot(S, nbi, s_top - rank);
+                int32_t nslot = -1;
+                if (lane == 0) nslot = sg_take_slot(S, nbi, s_top - rank);
+                nslot = __shfl(nslot, 0, 64);
                 if (nslot >= 0) {
                     float* fo = S.feat_on ? S.feat + nbi * SG_FEAT : nullptr;
                     sg_make_record(S, sg, sup_all + seg_item_base[m.seg], pair, nslot, sX[wv], k, sup_e, e, m.seg,
                                    sF[wv], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot]);
-                    S.newborn[nbi] = nslot;
+                    if (lane == 0) S.newborn[nbi] = nslot;
                 }
             }
         }

@@ -169,11 +169,12 @@ class Context:
         mc = np.ascontiguousarray(min_counts, dtype=np.int64)
         self._check(self._L.sgfim_set_window_thresholds(self._h, _p(mc), int(mc.shape[0])), "set_window_thresholds")
 
-    def mine_windows(self, min_counts, flags: int = 0) -> MineResult:
-        """Exact MFIs of every loaded window in one search (result.windows tags each set)."""
+    def mine_windows(self, min_counts, flags: int = 0, convert: bool = True) -> MineResult:
+        """Exact MFIs of every loaded window in one search (result.windows tags each set;
+        convert=False keeps them as flat arrays, result.arrays)."""
         self.set_window_thresholds(min_counts)
         try:
-            return self.mine(1, flags=flags)
+            return self.mine(1, flags=flags, convert=convert)
         finally:
             self._L.sgfim_set_window_thresholds(self._h, None, 0)
 
@@ -267,10 +268,11 @@ class Context:
     def anytime_cancel(self):
         self._check(self._L.sgfim_anytime_cancel(self._h), "anytime_cancel")
 
-    def anytime_take(self) -> MineResult:
+    def anytime_take(self, convert: bool = True) -> MineResult:
+        """MFIs confirmed since the last take (convert=False: flat arrays, result.arrays)."""
         h = C.c_void_p()
         self._check(self._L.sgfim_anytime_take(self._h, C.byref(h)), "anytime_take")
-        return _take_result(h, self.last_stats())
+        return _take_result(h, self.last_stats(), convert)
 
     # ------------------------------------------------------------------ proposal network
     def set_scorer(self, w1: np.ndarray, b1: np.ndarray, w2: np.ndarray, b2: float):

@@ -101,11 +101,15 @@ class Trainer:
 
 
 # ---------------------------------------------------------------------------- traces
-def trace_labels(keys: np.ndarray, mfi_keys: np.ndarray) -> np.ndarray:
-    """Label of every born node: 1 iff the node is a confirmed MFI or an ancestor of one.
+def trace_labels(keys: np.ndarray, mfi_keys: np.ndarray, mode: str = "ancestor") -> np.ndarray:
+    """Label of every born frontier record: 1 iff the round that counted its children
+    confirmed an MFI (mfi_keys holds that record's key per MFI), or it is an ancestor
+    of such a record.
 
-    keys[i] = (key, parent key) of birth i; keys are unique per run and a child is
-    always born after its parent, so one reverse pass propagates labels upward."""
+    keys[i] = (key, parent key) of birth i; keys are unique per run and a record is
+    always born after its parent, so one reverse pass propagates labels upward.
+    mode="direct" keeps only the first part: 1 iff the record's own round confirmed an
+    MFI (what a coverage-first order wants to reach early)."""
     n = keys.shape[0]
     pos = {int(k): i for i, k in enumerate(keys[:, 0])}
     lab = np.zeros(n, dtype=np.float32)
@@ -113,6 +117,8 @@ def trace_labels(keys: np.ndarray, mfi_keys: np.ndarray) -> np.ndarray:
         i = pos.get(int(k))
         if i is not None:
             lab[i] = 1.0
+    if mode == "direct":
+        return lab
     for i in range(n - 1, -1, -1):
         if lab[i]:
             j = pos.get(int(keys[i, 1]))
@@ -121,7 +127,7 @@ def trace_labels(keys: np.ndarray, mfi_keys: np.ndarray) -> np.ndarray:
     return lab
 
 
-def collect_trace(ctx: Context, min_count: int, cap: int = 1 << 22):
+def collect_trace(ctx: Context, min_count: int, cap: int = 1 << 22, mode: str = "ancestor"):
     """Run one traced exact search; return (features, labels, result)."""
     ctx.set_trace(True, cap)
     try:
@@ -129,7 +135,19 @@ def collect_trace(ctx: Context, min_count: int, cap: int = 1 << 22):
         keys, feat, mk = ctx.trace()
     finally:
         ctx.set_trace(False)
-    return feat, trace_labels(keys, mk), res
+    if mode == "yield":
+        return feat, yield_labels(keys, mk, feat), res
+    return feat, trace_labels(keys, mk, mode), res
+
+
+def yield_labels(keys: np.ndarray, mfi_keys: np.ndarray, feat: np.ndarray) -> np.ndarray:
+    """Soft label of every born record: the fraction of its children that its round
+    confirms as MFIs, min(1, MFIs / children). Feature 6 is log2(1 + children)."""
+    uk, cnt = np.unique(mfi_keys, return_counts=True)
+    per = dict(zip(uk.tolist(), cnt.tolist()))
+    direct = np.asarray([per.get(int(k), 0) for k in keys[:, 0]], dtype=np.float64)
+    children = np.maximum(1.0, np.rint(np.exp2(feat[:, 6].astype(np.float64)) - 1.0))
+    return np.minimum(1.0, direct / children).astype(np.float32)
 
 
 def save_weights(params, path: str = WEIGHTS_PATH, meta: dict | None = None):

@@ -3,12 +3,15 @@
   python -m sgfim.train_proposal --train 200 --eval 20 --out weights.npz --report report.json
 
 1. For every training seed, a workload-4 stream (500,000 txns, 256 items) is mined
-   exactly at the 2% floor with tracing on; every node born during the search gives
-   one row of 24 features, labelled 1 when it is a confirmed MFI or an ancestor of one.
+   exactly at the 2% floor with tracing on; every frontier record born during the
+   search gives one row of 24 features and a label: the fraction of the record's
+   children that its round confirms as MFIs (--label yield, the default; a soft BCE
+   target), or 1 when that round confirms any MFI (--label direct), or also when the
+   record is an ancestor of such a record (--label ancestor).
 2. The rows (a per-stream sample) train the 24-256-1 network on the GPU with the
    library's own BF16-MFMA forward/backward kernels (minibatch Adam).
 3. On held-out seeds the anytime search runs twice, support-ordered (plain) and
-   score-ordered (learned), in rounds of --batch-nodes nodes; the report holds both
+   score-ordered (learned), in rounds of --batch-nodes records; the report holds both
    coverage curves (fraction of the final MFI set confirmed vs wall time and vs
    support counts), time-to-95%, and whether the final sets are identical.
 """
@@ -45,18 +48,27 @@ def streams(seeds, n_txn, workers):
 
 
 def anytime_curve(ctx: Context, min_count: int, learned_order: bool):
-    """Run one anytime search round by round: (per-round records, final MFI dict)."""
+    """Run one anytime search round by round: (per-round records, final MFI set).
+
+    MFIs are taken as flat arrays each round (no per-itemset Python objects inside the
+    timed loop); the final set is built once at the end for the identity check."""
     ctx.anytime_begin(min_count, ORDER_LEARNED if learned_order else 0)
     rec = []
-    found = {}
+    parts = []
+    n = 0
     while True:
         st = ctx.anytime_step(1)
-        r = ctx.anytime_take()
-        for x, s in zip(r.itemsets, r.supports):
-            found[frozenset(x)] = s
-        rec.append((st["elapsed_s"], st["counts"], len(found)))
+        r = ctx.anytime_take(convert=False)
+        if r.count:
+            parts.append(r.arrays)
+            n += r.count
+        rec.append((st["elapsed_s"], st["counts"], n))
         if st["done"]:
             break
+    found = {}
+    for it, off, sup in parts:
+        for i in range(len(sup)):
+            found[frozenset(it[off[i]:off[i + 1]].tolist())] = int(sup[i])
     return rec, found
 
 
@@ -77,19 +89,31 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--batch-nodes", type=int, default=512)
+    ap.add_argument("--batch-nodes", type=int, default=64)
+    ap.add_argument("--label", choices=("yield", "direct", "ancestor"), default="yield",
+                    help="yield: fraction of the record's children its round confirms as MFIs; "
+                         "direct: the record's own round confirms an MFI; ancestor: also its ancestors")
     ap.add_argument("--workers", type=int, default=min(16, len(os.sched_getaffinity(0))))
     ap.add_argument("--out", default=learned.WEIGHTS_PATH)
+    ap.add_argument("--eval-only", action="store_true",
+                    help="skip tracing and training; evaluate the weights at --out")
     ap.add_argument("--report", default="")
     args = ap.parse_args(argv)
     rng = np.random.default_rng(12345)
     ctx = Context(device=0, batch_nodes=args.batch_nodes)
+    if args.eval_only:
+        ctx.set_scorer(*learned.load_weights(args.out))
+        report = {"weights": args.out, "batch_nodes": args.batch_nodes, "floor": args.floor, "n_txn": args.n_txn,
+                  "eval": []}
+        evaluate(ctx, args, report)
+        ctx.close()
+        return
     X, Y = [], []
     t0 = time.time()
     for s, (off, items) in streams(range(args.train), args.n_txn, args.workers):
         ctx.load_csr(off, items)
         thr = gen.min_count(args.floor, ctx.n_txn)
-        feat, lab, res = learned.collect_trace(ctx, thr)
+        feat, lab, res = learned.collect_trace(ctx, thr, mode=args.label)
         if feat.shape[0] > args.per_stream:
             idx = rng.choice(feat.shape[0], args.per_stream, replace=False)
             feat, lab = feat[idx], lab[idx]
@@ -111,8 +135,16 @@ def main(argv=None):
     learned.save_weights(params, args.out, {"rows": int(X.shape[0]), "epochs": args.epochs, "lr": args.lr,
                                             "train_seeds": args.train, "n_txn": args.n_txn, "floor": args.floor})
     ctx.set_scorer(*params)
-    report = {"train_rows": int(X.shape[0]), "losses": [float(x) for x in losses], "train_accuracy": acc,
+    report = {"train_rows": int(X.shape[0]), "label": args.label, "positive": float(Y.mean()),
+              "losses": [float(x) for x in losses], "train_accuracy": acc,
               "batch_nodes": args.batch_nodes, "floor": args.floor, "n_txn": args.n_txn, "eval": []}
+    evaluate(ctx, args, report)
+    ctx.close()
+    tr.close()
+
+
+def evaluate(ctx: Context, args, report: dict):
+    """Plain vs learned anytime runs on the held-out seeds; fills and writes the report."""
     seeds = range(1_000_000, 1_000_000 + args.eval)
     warm = True
     for s, (off, items) in streams(seeds, args.n_txn, args.workers):
@@ -147,8 +179,6 @@ def main(argv=None):
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f)
-    ctx.close()
-    tr.close()
 
 
 if __name__ == "__main__":

@@ -27,7 +27,7 @@ INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
 # passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
 # what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
 # profiles/k_count_group_workload2_pmc.md holds the commands and the table.
-PMC_COUNT_KERNEL = {"fetch_kib": 63594.2, "write_kib": 23659.7, "fetch_scale": 2.0, "write_scale": 1.0}
+PMC_COUNT_KERNEL = {"fetch_kib": 52606.3, "write_kib": 36040.9, "fetch_scale": 2.0, "write_scale": 1.0}
 SUPPORTS = (0.10, 0.05, 0.02)
 
 
@@ -86,9 +86,9 @@ def workload3_run(Context, seed=0, workers=16):
     ctx = Context(device=0)
     ctx.load_windows(win_off, off, items)
     thr = [gen.min_count(0.03, int(n)) for n in lens]
-    ctx.mine_windows(thr)                                  # warm
+    ctx.mine_windows(thr, convert=False)                   # warm
     t0 = time.perf_counter()
-    r = ctx.mine_windows(thr)
+    r = ctx.mine_windows(thr, convert=False)
     dt = time.perf_counter() - t0
     st = r.stats
     ctx.close()
@@ -149,6 +149,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the workload 3 and workload 5 measurements")
     ap.add_argument("--w5-budget", type=float, default=8.0)
+    ap.add_argument("--extra-only", choices=("workload3", "workload5"), default=None,
+                    help="run one extra workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -157,6 +159,11 @@ def main():
 
     from sgfim import gen
     from sgfim.core import Context
+
+    if args.extra_only:
+        res = workload5_run(Context, args.w5_budget) if args.extra_only == "workload5" else workload3_run(Context)
+        print(json.dumps(res))
+        return
 
     t0 = time.perf_counter()
     T = gen.workload2(args.seed, args.n_txn)

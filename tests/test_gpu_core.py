@@ -92,6 +92,24 @@ def test_pair_matrix_mfma_vs_valu_vs_oracle(ctx, seed, n):
     assert (P == R).all()
 
 
+def test_pair_matrix_three_tiles_and_timing(ctx):
+    say("case: pair matrix 700 items (3 x 3 tiles of 256, partial last tile), 20011 txns")
+    rng = np.random.default_rng(11)
+    n, d = 20011, 700
+    dense = rng.random((n, d)) < 0.05
+    rows_l = [np.nonzero(r)[0] + 1 for r in dense]
+    off = np.zeros(n + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(r) for r in rows_l])
+    items = np.concatenate(rows_l).astype(np.int32)
+    ctx.load_csr(off, items)
+    ids = ctx.item_ids()
+    P = ctx.pair_matrix()
+    assert ctx.pair_time() > 0.0
+    D = dense[:, ids - 1].astype(np.float64)   # exact: counts < 2^53
+    assert (P == (D.T @ D).astype(np.int64)).all()
+    assert (P == P.T).all()
+
+
 def test_pair_matrix_sparse_random(ctx):
     say("case: pair matrix sparse random 300 items")
     rng = np.random.default_rng(5)

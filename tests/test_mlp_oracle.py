@@ -59,3 +59,21 @@ def test_adam_matches_textbook_first_step():
     p1, m, v = M.adam_step(p, g, np.zeros(2), np.zeros(2), 1, 0.1)
     # first bias-corrected step moves each parameter by lr * sign(g)
     assert np.allclose(p1, p - 0.1 * np.sign(g), atol=1e-6)
+
+
+def test_trace_labels_direct_and_ancestor():
+    from sgfim.learned import trace_labels
+    # births (key, parent): 0 root record, 1 and 2 children of 0, 3 child of 1
+    keys = np.asarray([[0, -1], [1, 0], [2, 0], [3, 1]], dtype=np.int64)
+    mfi_keys = np.asarray([3, -1, 3], dtype=np.int64)   # record 3 confirmed two MFIs; -1 = the seed
+    assert trace_labels(keys, mfi_keys, "direct").tolist() == [0, 0, 0, 1]
+    assert trace_labels(keys, mfi_keys, "ancestor").tolist() == [1, 1, 0, 1]
+
+
+def test_yield_labels():
+    from sgfim.learned import yield_labels
+    keys = np.asarray([[0, -1], [1, 0], [2, 0]], dtype=np.int64)
+    mfi_keys = np.asarray([0, 0, 2, -1], dtype=np.int64)
+    feat = np.zeros((3, 24), dtype=np.float32)
+    feat[:, 6] = np.log2(1.0 + np.asarray([4.0, 3.0, 1.0]))   # children 4, 3, 1
+    assert np.allclose(yield_labels(keys, mfi_keys, feat), [0.5, 0.0, 1.0])
