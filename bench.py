@@ -128,15 +128,31 @@ def workload5_run(Context, budget_s=8.0, seed=0):
             "alg_gbs": alg, "alg_frac_of_hbm_peak": alg / HBM_PEAK_GBS if alg else None}
 
 
-def cpu_extra(budget_s=20.0):
-    """Single-thread restatements of the reference's Python miner on workload 1 (10k txns)."""
+def cpu_extra(T2=None, budget_s=15.0):
+    """Single-thread restatements of the reference's Python miners: the set-based miner
+    (skab_spike.py:84-135) and the NumPy packed-bitmap miner (exp_e6_cpuvec.py:43-107) on
+    workload 1 (10k txns, 2%), and the NumPy miner on workload 2 at 10% under a budget."""
+    from oracle import cpuvec as V
     from oracle import mfi_oracle as O
     from sgfim import gen
     T = gen.workload1(0)
     rows = T.rows()
     t0 = time.perf_counter()
     out, _ = O.mine_mfi(rows, min_count=gen.min_count(0.02, T.n_txn), node_cap=0)
-    return {"pure_python_mine_mfi_workload1_2pct_s": time.perf_counter() - t0, "mfis": len(out), "cores": 1}
+    res = {"pure_python_mine_mfi_workload1_2pct_s": time.perf_counter() - t0, "mfis": len(out), "cores": 1}
+    bm, n = V.bitmap_from_rows(rows, T.n_items)
+    mf, dt = V.mine_mfi_cpuvec(bm, n, 0.02)
+    res["numpy_cpuvec_workload1_2pct_s"] = dt
+    res["numpy_cpuvec_workload1_mfis"] = len(mf)
+    if T2 is not None:
+        bm2, n2 = V.bitmap_from_rows(T2.rows(), T2.n_items)
+        try:
+            mf2, dt2 = V.mine_mfi_cpuvec(bm2, n2, 0.10, budget_s=budget_s)
+            res["numpy_cpuvec_workload2_10pct_s"] = dt2
+            res["numpy_cpuvec_workload2_10pct_mfis"] = len(mf2)
+        except TimeoutError:
+            res["numpy_cpuvec_workload2_10pct_s"] = f">{budget_s} (budget)"
+    return res
 
 
 def main():
@@ -260,7 +276,7 @@ def main():
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(T)
-        out.setdefault("extra", {})["cpu_python"] = cpu_extra()
+        out.setdefault("extra", {})["cpu_python"] = cpu_extra(T)
     print(json.dumps(out), flush=True)
 
 
