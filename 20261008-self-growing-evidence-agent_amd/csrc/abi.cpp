@@ -75,6 +75,10 @@ struct SearchState {
     double event_ms = 0.0;           // HIP-event duration (direct launches only)
     int64_t count_launches = 0;
     SearchBufs* B = nullptr;         // the context's buffers used by this run
+    // host sources of the run's asynchronous uploads, alive until the next run
+    std::vector<SgSegment> vsegs;
+    std::vector<int64_t> vibase;
+    std::vector<int32_t> roots;
 };
 
 struct sgfim_ctx {
@@ -106,9 +110,16 @@ struct sgfim_ctx {
     bool pair_timed = false;
     int32_t* h_pair_pin = nullptr;      // pinned staging buffer for pair-matrix read-back (dpad x dpad)
     int64_t h_pair_pin_n = 0;
-    uint64_t* d_ok = nullptr;
-    SgSegment* d_seg = nullptr;
+    uint64_t* d_ok = nullptr;           // pair-ok masks, one block per search segment (vseg_cap blocks)
+    SgSegment* d_seg = nullptr;         // data segments (windows) + one for count_batch
     int64_t* d_seg_item_base = nullptr;
+    // Search segments: one per window, or one per threshold of a single set (a batched
+    // multi-threshold search); stable device arrays so captured rounds stay valid.
+    SgSegment* d_vseg = nullptr;
+    int64_t* d_vseg_ibase = nullptr;
+    int32_t vseg_cap = 0;
+    std::vector<int64_t> multi_thr;     // thresholds of the next run on a single set (empty: one)
+    std::vector<int32_t> multi_topk;    // per-threshold root counts for SGFIM_ROOT_TOPK (empty: args)
     std::vector<DevBuf> data_bufs;
     // proposal network
     bool scorer_set = false;
@@ -216,6 +227,11 @@ void data_free(sgfim_ctx* c)
     c->d_ok = nullptr;
     c->d_seg = nullptr;
     c->d_seg_item_base = nullptr;
+    c->d_vseg = nullptr;
+    c->d_vseg_ibase = nullptr;
+    c->vseg_cap = 0;
+    c->multi_thr.clear();
+    c->multi_topk.clear();
     c->pair_valid = false;
     c->win_thr.clear();
     c->loaded = false;
@@ -258,6 +274,15 @@ int ensure_pair(sgfim_ctx* c)
     c->pair_timed = timed && hipEventRecord(c->pair_ev[1], c->stream) == hipSuccess;
     c->pair_valid = true;
     return SGFIM_OK;
+}
+
+// Search-segment tables for up to max(windows, 256) segments (allocated with the data).
+int alloc_vsegs(sgfim_ctx* c)
+{
+    c->vseg_cap = std::max<int32_t>(c->nseg, 256);
+    int rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_vseg, (size_t)c->vseg_cap))) return rc;
+    return dev_alloc_t(c, c->data_bufs, &c->d_vseg_ibase, (size_t)c->vseg_cap);
 }
 
 double now_s(const SearchState* S)
@@ -357,15 +382,15 @@ int launch_round(sgfim_ctx* c, SearchState* S)
     SearchBufs* B = S->B;
     (void)hipGetLastError();
     if (!S->use_graph) {
-        HIPCHK(c, sg_search_round(B->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup, c->d_seg_item_base,
+        HIPCHK(c, sg_search_round(B->D, c->d_rows, c->d_vseg, c->d_pair, c->d_ok, c->d_sup, c->d_vseg_ibase,
                                   uniform, S->order_mode, &c->scorer, B->ev0, B->ev1, c->stream));
         return SGFIM_OK;
     }
     const int g = S->order_mode;
     if (!B->exec[g]) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = sg_search_round(B->D, c->d_rows, c->d_seg, c->d_pair, c->d_ok, c->d_sup,
-                                       c->d_seg_item_base, uniform, S->order_mode, &c->scorer, nullptr, nullptr,
+        hipError_t e = sg_search_round(B->D, c->d_rows, c->d_vseg, c->d_pair, c->d_ok, c->d_sup,
+                                       c->d_vseg_ibase, uniform, S->order_mode, &c->scorer, nullptr, nullptr,
                                        c->stream);
         hipError_t e2 = hipStreamEndCapture(c->stream, &B->graph[g]);
         if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round capture failed: %s", hipGetErrorString(e));
@@ -624,6 +649,7 @@ int sgfim_load_windows(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, 
     HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), c->segs.size() * sizeof(SgSegment), hipMemcpyHostToDevice,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = alloc_vsegs(c))) return rc;
     c->loaded = true;
     return SGFIM_OK;
 }
@@ -761,6 +787,7 @@ int sgfim_load_zipf(sgfim_ctx* c, int64_t n_txn, int32_t n_items, double zipf_s,
     HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, &zero, 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), sizeof(SgSegment), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = alloc_vsegs(c))) return rc;
     c->loaded = true;
     return SGFIM_OK;
 }
@@ -983,35 +1010,47 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     S->t0 = std::chrono::steady_clock::now();
     const int32_t d = c->d;
     const int32_t nseg = c->nseg;
+    const int32_t dw = (d + 63) / 64;
     int rc;
-    // per-window thresholds
-    std::vector<int64_t> thr(nseg, a->min_count);
-    if (!c->win_thr.empty()) {
+    // search segments: one per window (its own threshold), or one per threshold of a
+    // single set (one batched search over several thresholds; roots and results per
+    // threshold, rows and pair matrix shared)
+    const bool multi = !c->multi_thr.empty();
+    if (multi && nseg != 1)
+        return fail(c, SGFIM_E_INVAL, "a threshold list needs a single loaded set (windows take per-window thresholds)");
+    const int32_t nv = multi ? (int32_t)c->multi_thr.size() : nseg;
+    if (nv > c->vseg_cap) return fail(c, SGFIM_E_INVAL, "at most %d thresholds per search", c->vseg_cap);
+    std::vector<int64_t> thr(nv, a->min_count);
+    if (multi) thr = c->multi_thr;
+    else if (!c->win_thr.empty()) {
         if ((int32_t)c->win_thr.size() != nseg) return fail(c, SGFIM_E_INVAL, "window threshold count mismatch");
         thr = c->win_thr;
     }
     for (int64_t t : thr)
         if (t < 1 || t > (int64_t)INT32_MAX) return fail(c, SGFIM_E_INVAL, "min_count out of range");
-    // roots (seg, dense id): frequent items passing the root rule, per window
+    auto data_seg = [&](int32_t v) { return multi ? 0 : v; };
+    // roots (search segment, dense id): frequent items passing the root rule
     std::vector<int32_t> rp;
-    for (int32_t w = 0; w < nseg; w++) {
+    for (int32_t v = 0; v < nv; v++) {
+        const int32_t w = data_seg(v);
         const int64_t* sup = c->h_sup_seg.data() + (size_t)w * d;
         const int64_t n = c->segs[w].n;
         if (a->flags & SGFIM_ROOT_TOPK) {
+            const int32_t topk = (multi && !c->multi_topk.empty()) ? c->multi_topk[v] : a->root_topk;
             std::vector<int32_t> order(d);
             for (int32_t i = 0; i < d; i++) order[i] = i;
             std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return sup[x] > sup[y]; });
             std::vector<int32_t> r;
-            for (int32_t i = 0; i < std::min<int32_t>(a->root_topk, d); i++)
-                if (sup[order[i]] >= thr[w]) r.push_back(order[i]);
+            for (int32_t i = 0; i < std::min<int32_t>(topk, d); i++)
+                if (sup[order[i]] >= thr[v]) r.push_back(order[i]);
             std::sort(r.begin(), r.end());
-            for (int32_t x : r) { rp.push_back(w); rp.push_back(x); }
+            for (int32_t x : r) { rp.push_back(v); rp.push_back(x); }
             continue;
         }
         for (int32_t x = 0; x < d; x++) {
-            if (sup[x] < thr[w]) continue;
+            if (sup[x] < thr[v]) continue;
             if ((a->flags & SGFIM_ROOT_STRICT) && !(((double)sup[x] / (double)n) > a->rel_thr)) continue;
-            rp.push_back(w);
+            rp.push_back(v);
             rp.push_back(x);
         }
     }
@@ -1025,13 +1064,21 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     }
     if ((rc = ensure_pair(c))) return rc;
     S->pair_counts = n_roots * d;
-    // segment thresholds + pair_ok masks
-    const int32_t dw = (d + 63) / 64;
-    for (int32_t w = 0; w < nseg; w++) c->segs[w].thr = (int32_t)thr[w];
-    HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), (size_t)nseg * sizeof(SgSegment), hipMemcpyHostToDevice,
-                             c->stream));
-    if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * nseg))) return rc;
-    HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_seg, nseg, d, c->d_ok, c->stream));
+    // search-segment table (thresholds, pair-ok blocks) + pair_ok masks
+    std::vector<SgSegment>& vs = S->vsegs;
+    std::vector<int64_t>& vib = S->vibase;
+    vs.resize(nv);
+    vib.resize(nv);
+    for (int32_t v = 0; v < nv; v++) {
+        vs[v] = c->segs[data_seg(v)];
+        vs[v].thr = (int32_t)thr[v];
+        vs[v].ok_base = (int64_t)v * d * dw;
+        vib[v] = (int64_t)data_seg(v) * c->dpad;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_vseg, vs.data(), (size_t)nv * sizeof(SgSegment), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_vseg_ibase, vib.data(), (size_t)nv * 8, hipMemcpyHostToDevice, c->stream));
+    if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * c->vseg_cap))) return rc;
+    HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_vseg, nv, d, c->d_ok, c->stream));
     // the seed turns every root into at most one frontier record
     const int64_t seed_records = n_roots;
     // search buffers (reused across runs when they fit)
@@ -1057,7 +1104,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         newborn_cap = std::min(newborn_cap, (ring_cap - n_roots) / 2);
     }
     int64_t maxw = 0;
-    for (const SgSegment& g : c->segs) maxw = std::max(maxw, g.words);
+    for (const SgSegment& g : vs) maxw = std::max(maxw, g.words);
     if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, search_chunk(c, maxw),
                           c->trace ? c->trace_cap : 0, n_roots, cell_cap)))
         return rc;
@@ -1075,8 +1122,9 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     }
     HIPCHK(c, sg_search_init(D, c->stream));
     S->n_roots = (int32_t)n_roots;
-    HIPCHK(c, hipMemcpyAsync(S->B->d_roots, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, sg_search_seed(D, c->d_seg, c->d_pair, c->d_ok, S->B->d_roots, S->n_roots, c->d_sup, c->d_seg_item_base,
+    S->roots = std::move(rp);
+    HIPCHK(c, hipMemcpyAsync(S->B->d_roots, S->roots.data(), S->roots.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, sg_search_seed(D, c->d_vseg, c->d_pair, c->d_ok, S->B->d_roots, S->n_roots, c->d_sup, c->d_vseg_ibase,
                              S->order_mode, &c->scorer, c->stream));
     // no synchronisation here: the seed's MFIs, errors and frontier are read after the
     // first round (a round over an empty frontier does nothing)
@@ -1271,6 +1319,23 @@ int sgfim_set_window_thresholds(sgfim_ctx* c, const int64_t* min_counts, int32_t
     for (int32_t w = 0; w < n; w++)
         if (min_counts[w] < 1) return fail(c, SGFIM_E_INVAL, "min_count of window %d must be >= 1", w);
     c->win_thr.assign(min_counts, min_counts + n);
+    return SGFIM_OK;
+}
+
+int sgfim_set_thresholds(sgfim_ctx* c, const int64_t* min_counts, const int32_t* root_topk, int32_t n)
+{
+    if (!c || n < 0 || (n > 0 && !min_counts)) return SGFIM_E_INVAL;
+    c->multi_thr.clear();
+    c->multi_topk.clear();
+    if (n == 0) return SGFIM_OK;
+    if (!c->loaded || c->nseg != 1) return fail(c, SGFIM_E_STATE, "a threshold list needs a single loaded set");
+    if (n > c->vseg_cap) return fail(c, SGFIM_E_INVAL, "at most %d thresholds per search", c->vseg_cap);
+    for (int32_t i = 0; i < n; i++) {
+        if (min_counts[i] < 1) return fail(c, SGFIM_E_INVAL, "min_count %d must be >= 1", i);
+        if (root_topk && root_topk[i] < 0) return fail(c, SGFIM_E_INVAL, "root_topk %d must be >= 0", i);
+    }
+    c->multi_thr.assign(min_counts, min_counts + n);
+    if (root_topk) c->multi_topk.assign(root_topk, root_topk + n);
     return SGFIM_OK;
 }
 

@@ -64,6 +64,7 @@ def _take_result(h: C.c_void_p, stats: dict | None = None, convert: bool = True)
             it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
             s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
             res.arrays = (it, o, s)
+            res.windows = np.ctypeslib.as_array(C.cast(win, C.POINTER(C.c_int32)), shape=(nn,)).copy()
         elif nn:
             o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
             it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
@@ -168,6 +169,19 @@ class Context:
     def set_window_thresholds(self, min_counts):
         mc = np.ascontiguousarray(min_counts, dtype=np.int64)
         self._check(self._L.sgfim_set_window_thresholds(self._h, _p(mc), int(mc.shape[0])), "set_window_thresholds")
+
+    def mine_thresholds(self, min_counts, flags: int = 0, root_topk=None, convert: bool = True) -> MineResult:
+        """Exact MFIs of one loaded set at several thresholds in one batched search
+        (result.windows tags each set with its threshold's index). With ROOT_TOPK,
+        root_topk[i] gives threshold i's number of roots (AnyFIM stages)."""
+        mc = np.ascontiguousarray(np.asarray(min_counts, dtype=np.int64))
+        tk = None if root_topk is None else np.ascontiguousarray(np.asarray(root_topk, dtype=np.int32))
+        self._check(self._L.sgfim_set_thresholds(self._h, _p(mc), _p(tk) if tk is not None else None, len(mc)),
+                    "set_thresholds")
+        try:
+            return self.mine(int(mc.max()) if len(mc) else 1, flags=flags, convert=convert)
+        finally:
+            self._L.sgfim_set_thresholds(self._h, None, None, 0)
 
     def mine_windows(self, min_counts, flags: int = 0, convert: bool = True) -> MineResult:
         """Exact MFIs of every loaded window in one search (result.windows tags each set;

@@ -272,9 +272,14 @@ def anyfim_thresholds(supports: np.ndarray, upto_stage: int):
     return [int(x) for x in np.sort(supports)[::-1][:upto_stage]]
 
 
-def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False):
+def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
     """Run stages 1..upto_stage; returns {'ok', 'stages': {k: [(frozenset, freq)]}, 'meta', 'results_path'}
-    and writes <fimi>-<upto_stage>-stages=Results.txt."""
+    and writes <fimi>-<upto_stage>-stages=Results.txt.
+
+    batched=True mines every stage in one search (sgfim_set_thresholds) instead of
+    re-mining stage by stage as the engine does (kernel.cu:844-907); the stage sets are
+    the same, and runtimePerStage then holds the time at which each stage's last MFI was
+    confirmed in that shared run."""
     rp = f"{fimi_path}-{upto_stage}-stages=Results.txt"
     if os.path.exists(rp):
         os.remove(rp)
@@ -290,6 +295,27 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False):
     thrs = anyfim_thresholds(sup, upto_stage)
     stages, runtimes = {}, []
     t_start = time.perf_counter()
+    if batched:
+        try:
+            res = ctx.mine_thresholds(thrs, flags=0 if exact else L.ROOT_TOPK,
+                                      root_topk=list(range(1, upto_stage + 1)))
+        except L.SgfimError as e:
+            if e.code == L.E_TIMEOUT:
+                return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+            raise
+        total = time.perf_counter() - t_start
+        if total > timeout:
+            return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+        for k in range(1, upto_stage + 1):
+            stages[k] = []
+        done_at = [0.0] * upto_stage
+        for x, sp, w, t in zip(res.itemsets, res.supports, res.windows, res.times):
+            stages[w + 1].append((x, sp))
+            done_at[w] = max(done_at[w], t)
+        runtimes = done_at
+        write_anyfim_results(rp, fimi_path, stages, thrs, runtimes, ctx.n_txn, pre)
+        st, meta = parse_anyfim_results(rp)
+        return {"ok": True, "stages": st, "meta": meta, "results_path": rp}
     for k in range(1, upto_stage + 1):
         left = timeout - (time.perf_counter() - t_start)
         if left <= 0:

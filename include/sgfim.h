@@ -131,6 +131,14 @@ int sgfim_window_supports(sgfim_ctx* ctx, int32_t window, int64_t* out);
  * sgfim_mine_args.min_count then applies to every window). */
 int sgfim_set_window_thresholds(sgfim_ctx* ctx, const int64_t* min_counts, int32_t n);
 
+/* Several thresholds of one loaded set in one batched search: the next runs mine the
+ * exact MFI family at each min_counts[i] (with SGFIM_ROOT_TOPK, root_topk[i] roots for
+ * threshold i when root_topk is given) and tag every MFI with i (sgfim_result_windows).
+ * Replaces the AnyFIM stage loop, which re-mines every stage from scratch
+ * (kernel.cu:844-907): the stages share the bitmaps, the pair matrix and every round.
+ * n = 0 clears; at most 256 thresholds. */
+int sgfim_set_thresholds(sgfim_ctx* ctx, const int64_t* min_counts, const int32_t* root_topk, int32_t n);
+
 /* Synthetic dense-sparse set generated on the GPU straight into the bitmaps: item i
  * (1..n_items) is present independently with probability q_i = min(1, c i^-zipf_s),
  * c set so that sum q_i = mean_len; n_planted itemsets of size_lo..size_hi items are

@@ -49,6 +49,9 @@ def test_run_anyfim_matches_stage_families(fimi):
         assert dict(r["stages"][k]) == fam
         assert r["meta"]["frequencyThrePerStage"][k - 1] == thr
     assert not EA.run_anyfim(path, 41)["ok"]        # more stages than items: the engine refuses
+    b = EA.run_anyfim(path, 12, batched=True)        # all stages in one search: same stage sets
+    assert b["ok"] and b["stages"] == r["stages"]
+    assert b["meta"]["frequencyThrePerStage"] == r["meta"]["frequencyThrePerStage"]
 
 
 def test_error_paths():

@@ -175,6 +175,27 @@ def test_engine_root_rules(ctx):
         assert got == fam
 
 
+def test_mine_thresholds_batched_equals_separate_runs(ctx):
+    say("case: batched thresholds vs separate runs, workload1 seed 6 n=8000")
+    T = gen.workload1(6, 8000)
+    ctx.load_csr(T.offsets, T.items)
+    thrs = [gen.min_count(s, T.n_txn) for s in (0.1, 0.05, 0.02, 0.2)]
+    res = ctx.mine_thresholds(thrs)
+    got = [dict() for _ in thrs]
+    for x, s, w in zip(res.itemsets, res.supports, res.windows):
+        got[w][frozenset(x)] = s
+    for i, thr in enumerate(thrs):
+        assert got[i] == ctx.mine(thr).as_dict() == _oracle_mfis(T, thr), i
+    # AnyFIM stages: threshold k = k-th item count, top-k roots
+    txns = T.rows()
+    fams = [O.anyfim_stage_family(txns, k) for k in range(1, 13)]
+    res = ctx.mine_thresholds([t for _, t in fams], flags=ROOT_TOPK, root_topk=list(range(1, 13)))
+    got = [dict() for _ in fams]
+    for x, s, w in zip(res.itemsets, res.supports, res.windows):
+        got[w][frozenset(x)] = s
+    assert got == [f for f, _ in fams]
+
+
 def test_empty_and_degenerate_inputs(ctx):
     say("case: degenerate inputs")
     ctx.load_csr(np.zeros(1, dtype=np.int64), np.zeros(0, dtype=np.int32))
