@@ -355,3 +355,37 @@ def support_confidence_lift(itemset, txns, labels):
     conf = pos / npos if npos else 0.0
     lift = (pos / npos) / (cnt / n) if cnt and npos else 0.0
     return sup, conf, lift
+
+
+def support_confidence_lift_batch(itemsets, txns, labels):
+    """support_confidence_lift for many patterns at once on the GPU: the label becomes one
+    more item (present in the label-1 transactions) and every pattern takes two AND+popcount
+    counts, |T(X)| (prefix X minus its last item, candidate that item) and |T(X) & T(label)|
+    (prefix X, candidate the label item), in one count_batch call. Same formula and float
+    results as support_confidence_lift."""
+    n = len(txns)
+    npos = sum(1 for lab in labels if lab == 1)
+    label_id = 1 + max((max(t) for t in txns if len(t)), default=0)
+    rows = [tuple(sorted(set(t))) + ((label_id,) if lab == 1 else ()) for t, lab in zip(txns, labels)]
+    from .core import csr_from_rows
+    off, items = csr_from_rows(rows)
+    ctx = _ctx()
+    ctx.load_csr(off, items)
+    prefixes, cands = [], []
+    for x in itemsets:
+        xs = sorted(set(x))
+        prefixes.append(xs[:-1] if xs else [])
+        cands.append(xs[-1:] if xs else [])
+        prefixes.append(xs)
+        cands.append([label_id])
+    got = ctx.count_batch(prefixes, cands) if itemsets else []
+    out = []
+    for i, x in enumerate(itemsets):
+        cnt = int(got[2 * i][0]) if len(x) else n
+        pos = int(got[2 * i + 1][0])
+        sup = cnt / n if n else 0.0
+        conf = pos / npos if npos else 0.0
+        lift = (pos / npos) / (cnt / n) if cnt and npos else 0.0
+        out.append((sup, conf, lift))
+    return out
+

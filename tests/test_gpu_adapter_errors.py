@@ -138,3 +138,40 @@ def test_depth_overflow_and_deadline_and_cancel():
         full = ctx.mine(thr).as_dict()
         for x, s in zip(got.itemsets, got.supports):
             assert full[frozenset(x)] == s
+
+
+def test_scoring_batch_matches_cpu(fimi):
+    print("case: support/confidence/lift on the GPU", flush=True)
+    _, rows = fimi
+    rng = np.random.default_rng(4)
+    labels = (rng.random(len(rows)) < 0.3).astype(int).tolist()
+    items = sorted({x for r in rows for x in r})
+    pats = [frozenset(rng.choice(items, size=k, replace=False).tolist()) for k in (1, 2, 2, 3, 4, 5) for _ in range(5)]
+    pats += [frozenset(), frozenset({items[0], 999})]            # empty pattern, absent id
+    got = EA.support_confidence_lift_batch(pats, rows, labels)
+    assert got == [EA.support_confidence_lift(p, rows, labels) for p in pats]
+
+
+def test_engine_argv_shims(fimi):
+    import subprocess
+    import sys
+    print("case: engine argv shims", flush=True)
+    path, rows = fimi
+    bindir = os.path.join(os.path.dirname(EA.__file__), "bin")
+    rp = f"{path}-{0.05:.6f}=Results.txt"
+    if os.path.exists(rp):
+        os.remove(rp)
+    p = subprocess.run([sys.executable, os.path.join(bindir, "CoParaCG_bmma.exe"), path, "0.05"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert "computation_time(s):" in p.stdout and "total_time(s):" in p.stdout
+    assert EA.parse_tensorfim_results(rp) == O.tensorfim_family(rows, 0.05)
+    p = subprocess.run([sys.executable, os.path.join(bindir, "AnytimeMining.exe"), path, "5"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    st, meta = EA.parse_anyfim_results(f"{path}-5-stages=Results.txt")
+    for k in range(1, 6):
+        assert dict(st[k]) == O.anyfim_stage_family(rows, k)[0]
+    p = subprocess.run([sys.executable, os.path.join(bindir, "AnytimeMining.exe"), path, "41"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 1                                      # more stages than items
