@@ -545,6 +545,80 @@ void sgfim_destroy(sgfim_ctx* c)
 // Load windows: window w holds transactions [win_off[w], win_off[w+1]) of the CSR.
 // All windows share one item universe (the distinct ids of the whole CSR); each has
 // its own bitmaps (dpad rows x Wp_w words), supports, pair matrix and threshold.
+// The bitmaps of a loaded set from a CSR in device memory: dense ids from idmap (original
+// id -> dense, -1 = absent; c->ids already filled), one segment per window, the GPU
+// transpose per window, supports, segment tables. d_off indexes d_items globally.
+static int build_from_device_csr(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, int64_t n_txn,
+                                 const int64_t* d_off, const int32_t* d_items, int64_t nnz, int32_t idmax,
+                                 const std::vector<int32_t>& idmap)
+{
+    c->n_txn = n_txn;
+    c->d = (int32_t)c->ids.size();
+    if (c->d > 65535) return fail(c, SGFIM_E_INVAL, "at most 65535 distinct items are supported (got %d)", c->d);
+    c->dpad = std::max(128, (c->d + 127) / 128 * 128);
+    c->nseg = n_windows;
+    c->segs.assign(n_windows, SgSegment{});
+    int64_t base = 0;
+    for (int32_t w = 0; w < n_windows; w++) {
+        SgSegment& g = c->segs[w];
+        const int64_t n = win_off[w + 1] - win_off[w];
+        g.n = (int32_t)n;
+        g.words = (n + 63) / 64;
+        g.stride = std::max<int64_t>(64, (g.words + 63) / 64 * 64);
+        g.row_base = base;
+        base += (int64_t)c->dpad * g.stride;
+        g.d = c->d;
+        g.dpad = c->dpad;
+        g.dw = (c->d + 63) / 64;
+        g.pair_base = (int64_t)w * c->dpad * c->dpad;
+        g.ok_base = (int64_t)w * c->d * g.dw;
+        g.thr = 1;
+    }
+    c->W = c->segs[0].words;
+    c->Wp = c->segs[0].stride;
+    c->rows_words = base;
+    int rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_rows, (size_t)base))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_sup, (size_t)c->dpad * n_windows))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_rows, 0, (size_t)base * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_sup, 0, (size_t)c->dpad * n_windows * 4, c->stream));
+    if (nnz > 0) {
+        std::vector<DevBuf> tmp;
+        int32_t* d_map = nullptr;
+        if ((rc = dev_alloc_t(c, tmp, &d_map, (size_t)idmax + 1))) return rc;
+        hipError_t e = hipMemcpyAsync(d_map, idmap.data(), ((size_t)idmax + 1) * 4, hipMemcpyHostToDevice, c->stream);
+        for (int32_t w = 0; w < n_windows && e == hipSuccess; w++) {
+            const SgSegment& g = c->segs[w];
+            if (g.n == 0) continue;
+            e = sg_launch_transpose(d_off + win_off[w], d_items, d_map, g.n, idmax, c->d_rows + g.row_base, c->d,
+                                    c->dpad, g.stride, c->d_sup + (int64_t)w * c->dpad, c->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dev_free_all(c, tmp);
+        if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "bitmap build failed: %s", hipGetErrorString(e));
+    }
+    std::vector<int32_t> sup32((size_t)c->dpad * n_windows);
+    HIPCHK(c, hipMemcpyAsync(sup32.data(), c->d_sup, sup32.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_sup_seg.assign((size_t)c->d * n_windows, 0);
+    for (int32_t w = 0; w < n_windows; w++)
+        for (int32_t i = 0; i < c->d; i++) c->h_sup_seg[(size_t)w * c->d + i] = sup32[(size_t)w * c->dpad + i];
+    c->h_sup.assign(c->h_sup_seg.begin(), c->h_sup_seg.begin() + c->d);
+    // segment tables: [0, nseg) search, [nseg] count_batch
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg, (size_t)n_windows + 1))) return rc;
+    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg_item_base, (size_t)n_windows))) return rc;
+    std::vector<int64_t> ib(n_windows);
+    for (int32_t w = 0; w < n_windows; w++) ib[w] = (int64_t)w * c->dpad;
+    HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, ib.data(), ib.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), c->segs.size() * sizeof(SgSegment), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = alloc_vsegs(c))) return rc;
+    c->loaded = true;
+    return SGFIM_OK;
+}
+
+
 int sgfim_load_windows(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, const int64_t* off, int64_t n_txn,
                        const int32_t* items, int64_t nnz)
 {
@@ -580,78 +654,159 @@ int sgfim_load_windows(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, 
             idmap[x] = (int32_t)c->ids.size();
             c->ids.push_back(x);
         }
-    c->n_txn = n_txn;
-    c->d = (int32_t)c->ids.size();
-    if (c->d > 65535) return fail(c, SGFIM_E_INVAL, "at most 65535 distinct items are supported (got %d)", c->d);
-    c->dpad = std::max(128, (c->d + 127) / 128 * 128);
-    c->nseg = n_windows;
-    c->segs.assign(n_windows, SgSegment{});
-    int64_t base = 0;
-    for (int32_t w = 0; w < n_windows; w++) {
-        SgSegment& g = c->segs[w];
-        const int64_t n = win_off[w + 1] - win_off[w];
-        g.n = (int32_t)n;
-        g.words = (n + 63) / 64;
-        g.stride = std::max<int64_t>(64, (g.words + 63) / 64 * 64);
-        g.row_base = base;
-        base += (int64_t)c->dpad * g.stride;
-        g.d = c->d;
-        g.dpad = c->dpad;
-        g.dw = (c->d + 63) / 64;
-        g.pair_base = (int64_t)w * c->dpad * c->dpad;
-        g.ok_base = (int64_t)w * c->d * g.dw;
-        g.thr = 1;
-    }
-    c->W = c->segs[0].words;
-    c->Wp = c->segs[0].stride;
-    c->rows_words = base;
+    if (nnz == 0) return build_from_device_csr(c, n_windows, win_off, n_txn, nullptr, nullptr, 0, idmax, idmap);
+    std::vector<DevBuf> tmp;
+    int64_t* d_off = nullptr;
+    int32_t* d_items = nullptr;
     int rc;
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_rows, (size_t)base))) return rc;
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_sup, (size_t)c->dpad * n_windows))) return rc;
-    HIPCHK(c, hipMemsetAsync(c->d_rows, 0, (size_t)base * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_sup, 0, (size_t)c->dpad * n_windows * 4, c->stream));
-    if (nnz > 0) {
-        std::vector<DevBuf> tmp;
-        int64_t* d_off = nullptr;
-        int32_t* d_items = nullptr;
-        int32_t* d_map = nullptr;
-        if ((rc = dev_alloc_t(c, tmp, &d_off, (size_t)n_txn + 1)) || (rc = dev_alloc_t(c, tmp, &d_items, (size_t)nnz)) ||
-            (rc = dev_alloc_t(c, tmp, &d_map, (size_t)idmax + 1))) {
-            dev_free_all(c, tmp);
-            return rc;
-        }
-        hipError_t e = hipMemcpyAsync(d_off, off, ((size_t)n_txn + 1) * 8, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_items, items, (size_t)nnz * 4, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_map, idmap.data(), ((size_t)idmax + 1) * 4, hipMemcpyHostToDevice, c->stream);
-        for (int32_t w = 0; w < n_windows && e == hipSuccess; w++) {
-            const SgSegment& g = c->segs[w];
-            if (g.n == 0) continue;
-            e = sg_launch_transpose(d_off + win_off[w], d_items, d_map, g.n, idmax, c->d_rows + g.row_base, c->d,
-                                    c->dpad, g.stride, c->d_sup + (int64_t)w * c->dpad, c->stream);
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if ((rc = dev_alloc_t(c, tmp, &d_off, (size_t)n_txn + 1)) || (rc = dev_alloc_t(c, tmp, &d_items, (size_t)nnz))) {
         dev_free_all(c, tmp);
-        if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "bitmap build failed: %s", hipGetErrorString(e));
+        return rc;
     }
-    std::vector<int32_t> sup32((size_t)c->dpad * n_windows);
-    HIPCHK(c, hipMemcpyAsync(sup32.data(), c->d_sup, sup32.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    hipError_t e = hipMemcpyAsync(d_off, off, ((size_t)n_txn + 1) * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_items, items, (size_t)nnz * 4, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+        dev_free_all(c, tmp);
+        return fail(c, SGFIM_E_DEVICE, "CSR upload failed: %s", hipGetErrorString(e));
+    }
+    rc = build_from_device_csr(c, n_windows, win_off, n_txn, d_off, d_items, nnz, idmax, idmap);
+    (void)hipStreamSynchronize(c->stream);
+    dev_free_all(c, tmp);
+    return rc;
+}
+
+// Quantile edges of one sorted column with NumPy's default (linear) method, then
+// unique, then the constant-column rule of fit_bins (skab_spike.py:37-50). `val`
+// holds the order statistics at the positions sensor_quantile_positions chose.
+static void sensor_edges(int64_t n, int32_t B, const double* val, std::vector<double>& e)
+{
+    e.clear();
+    const double step = 1.0 / (double)B;   // linspace(0, 1, B + 1): i * step, the last one exactly 1
+    for (int32_t i = 0; i <= B; i++) {
+        const double q = i == B ? 1.0 : (double)i * step;
+        const double v = (double)(n - 1) * q;
+        const double a = val[2 * i], b = val[2 * i + 1];
+        double gamma;
+        if (v >= (double)(n - 1)) gamma = v + 1.0;   // both neighbours are the last value
+        else gamma = v - std::floor(v);
+        const double diff = b - a;
+        double r = a + diff * gamma;
+        if (gamma >= 0.5) r = b - diff * (1.0 - gamma);
+        e.push_back(r);
+    }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    if (e.size() < 2) {
+        const double x = e[0];
+        e.assign({x - 0.5, x + 0.5});
+    }
+}
+
+int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
+                       int32_t window, int32_t stride)
+{
+    (void)hipGetLastError();
+    if (!c) return SGFIM_E_INVAL;
+    const int32_t S = n_sensors, B = n_bins;
+    if (!x || n_rows < 1 || S < 1 || B < 1 || window < 1 || window > 64 || stride < 1 || n_rows < window)
+        return fail(c, SGFIM_E_INVAL, "bad sensor matrix arguments (1 <= window <= 64, window <= rows)");
+    if ((int64_t)S * B > 65535) return fail(c, SGFIM_E_INVAL, "sensors x bins must be <= 65535");
+    const int64_t n_win = (n_rows - window) / stride + 1;
+    if (n_win >= ((int64_t)1 << 31)) return fail(c, SGFIM_E_INVAL, "too many windows");
+    for (int64_t i = 0; i < n_rows * S; i++)
+        if (!std::isfinite(x[i])) return fail(c, SGFIM_E_INVAL, "sensor value %lld is not finite", (long long)i);
+    (void)hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->h_sup_seg.assign((size_t)c->d * n_windows, 0);
-    for (int32_t w = 0; w < n_windows; w++)
-        for (int32_t i = 0; i < c->d; i++) c->h_sup_seg[(size_t)w * c->d + i] = sup32[(size_t)w * c->dpad + i];
-    c->h_sup.assign(c->h_sup_seg.begin(), c->h_sup_seg.begin() + c->d);
-    // segment tables: [0, nseg) search, [nseg] count_batch
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg, (size_t)n_windows + 1))) return rc;
-    if ((rc = dev_alloc_t(c, c->data_bufs, &c->d_seg_item_base, (size_t)n_windows))) return rc;
-    std::vector<int64_t> ib(n_windows);
-    for (int32_t w = 0; w < n_windows; w++) ib[w] = (int64_t)w * c->dpad;
-    HIPCHK(c, hipMemcpyAsync(c->d_seg_item_base, ib.data(), ib.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_seg, c->segs.data(), c->segs.size() * sizeof(SgSegment), hipMemcpyHostToDevice,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if ((rc = alloc_vsegs(c))) return rc;
-    c->loaded = true;
-    return SGFIM_OK;
+    data_free(c);
+    const int64_t nv = n_rows * S;
+    std::vector<DevBuf> tmp;
+    double *d_x = nullptr, *d_xc = nullptr, *d_xs = nullptr, *d_val = nullptr, *d_edges = nullptr;
+    int64_t *d_beg = nullptr, *d_end = nullptr, *d_pos = nullptr, *d_off = nullptr;
+    int32_t *d_ne = nullptr, *d_items = nullptr;
+    uint32_t* d_present = nullptr;
+    const int64_t m = (int64_t)S * (B + 1) * 2;
+    int rc;
+    if ((rc = dev_alloc_t(c, tmp, &d_x, (size_t)nv)) || (rc = dev_alloc_t(c, tmp, &d_xc, (size_t)nv)) ||
+        (rc = dev_alloc_t(c, tmp, &d_xs, (size_t)nv)) || (rc = dev_alloc_t(c, tmp, &d_beg, (size_t)S)) ||
+        (rc = dev_alloc_t(c, tmp, &d_end, (size_t)S)) || (rc = dev_alloc_t(c, tmp, &d_pos, (size_t)m)) ||
+        (rc = dev_alloc_t(c, tmp, &d_val, (size_t)m)) || (rc = dev_alloc_t(c, tmp, &d_edges, (size_t)S * (B + 1))) ||
+        (rc = dev_alloc_t(c, tmp, &d_ne, (size_t)S)) || (rc = dev_alloc_t(c, tmp, &d_items, (size_t)(n_win * S))) ||
+        (rc = dev_alloc_t(c, tmp, &d_present, (size_t)S * B)) || (rc = dev_alloc_t(c, tmp, &d_off, (size_t)n_win + 1))) {
+        dev_free_all(c, tmp);
+        return rc;
+    }
+    auto bail = [&](const char* what, hipError_t e) {
+        dev_free_all(c, tmp);
+        return fail(c, SGFIM_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    };
+    // columns, sorted per sensor
+    std::vector<int64_t> beg(S), end(S);
+    for (int32_t s = 0; s < S; s++) { beg[s] = (int64_t)s * n_rows; end[s] = beg[s] + n_rows; }
+    hipError_t e = hipMemcpyAsync(d_x, x, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_beg, beg.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_end, end.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = sg_sensor_columns(d_x, n_rows, S, d_xc, c->stream);
+    size_t temp_bytes = 0;
+    if (e == hipSuccess) e = sg_sensor_sort(nullptr, &temp_bytes, d_xc, d_xs, n_rows, S, d_beg, d_end, c->stream);
+    if (e != hipSuccess) return bail("sensor columns", e);
+    void* d_temp = nullptr;
+    if ((rc = dev_alloc(c, tmp, &d_temp, std::max<size_t>(temp_bytes, 16)))) {
+        dev_free_all(c, tmp);
+        return rc;
+    }
+    e = sg_sensor_sort(d_temp, &temp_bytes, d_xc, d_xs, n_rows, S, d_beg, d_end, c->stream);
+    // order statistics for the quantiles: floor(v) and floor(v) + 1 of v = (n - 1) q
+    std::vector<int64_t> pos((size_t)m);
+    {
+        const double step = 1.0 / (double)B;
+        for (int32_t s = 0; s < S; s++)
+            for (int32_t i = 0; i <= B; i++) {
+                const double q = i == B ? 1.0 : (double)i * step;
+                const double v = (double)(n_rows - 1) * q;
+                int64_t lo, hi;
+                if (v >= (double)(n_rows - 1)) lo = hi = n_rows - 1;
+                else { lo = (int64_t)std::floor(v); hi = lo + 1; }
+                pos[(size_t)(s * (B + 1) + i) * 2] = (int64_t)s * n_rows + lo;
+                pos[(size_t)(s * (B + 1) + i) * 2 + 1] = (int64_t)s * n_rows + hi;
+            }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pos, pos.data(), (size_t)m * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = sg_sensor_gather(d_xs, d_pos, m, d_val, c->stream);
+    std::vector<double> val((size_t)m);
+    if (e == hipSuccess) e = hipMemcpyAsync(val.data(), d_val, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail("sensor quantiles", e);
+    std::vector<double> edges((size_t)S * (B + 1), 0.0);
+    std::vector<int32_t> ne(S);
+    std::vector<double> es;
+    for (int32_t s = 0; s < S; s++) {
+        sensor_edges(n_rows, B, val.data() + (size_t)s * (B + 1) * 2, es);
+        ne[s] = (int32_t)es.size();
+        std::copy(es.begin(), es.end(), edges.begin() + (size_t)s * (B + 1));
+    }
+    // medians, bins, items, presence
+    e = hipMemcpyAsync(d_edges, edges.data(), edges.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ne, ne.data(), (size_t)S * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_present, 0, (size_t)S * B * 4, c->stream);
+    if (e == hipSuccess)
+        e = sg_sensor_bins(d_x, S, B, window, stride, n_win, d_edges, d_ne, d_items, d_present, d_off, c->stream);
+    std::vector<uint32_t> present((size_t)S * B);
+    if (e == hipSuccess) e = hipMemcpyAsync(present.data(), d_present, present.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail("sensor bins", e);
+    const int32_t idmax = S * B;
+    std::vector<int32_t> idmap((size_t)idmax + 1, -1);
+    c->ids.clear();
+    for (int32_t id = 1; id <= idmax; id++)
+        if (present[id - 1]) {
+            idmap[id] = (int32_t)c->ids.size();
+            c->ids.push_back(id);
+        }
+    const int64_t win_off[2] = {0, n_win};
+    rc = build_from_device_csr(c, 1, win_off, n_win, d_off, d_items, n_win * S, idmax, idmap);
+    (void)hipStreamSynchronize(c->stream);
+    dev_free_all(c, tmp);
+    return rc;
 }
 
 namespace {

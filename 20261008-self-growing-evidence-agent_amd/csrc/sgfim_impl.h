@@ -159,6 +159,13 @@ enum SgCtl {
 
 struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)
 
+hipError_t sg_sensor_columns(const double* x, int64_t n, int32_t S, double* xc, hipStream_t s);
+hipError_t sg_sensor_sort(void* temp, size_t* temp_bytes, const double* xc, double* xs, int64_t n, int32_t S,
+                          const int64_t* seg_begin, const int64_t* seg_end, hipStream_t s);
+hipError_t sg_sensor_gather(const double* xs, const int64_t* pos, int64_t m, double* out, hipStream_t s);
+hipError_t sg_sensor_bins(const double* x, int32_t S, int32_t B, int32_t window, int32_t stride, int64_t n_win,
+                          const double* edges, const int32_t* ne, int32_t* items, uint32_t* present, int64_t* off,
+                          hipStream_t s);
 int sg_count_group_blocks_per_cu(int32_t chunk_words);
 hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s);
 hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int32_t* pair,

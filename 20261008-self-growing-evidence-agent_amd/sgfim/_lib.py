@@ -62,6 +62,7 @@ _SIGS = {
     "sgfim_window_supports": (C.c_int, [P, C.c_int32, P]),
     "sgfim_set_window_thresholds": (C.c_int, [P, P, C.c_int32]),
     "sgfim_set_thresholds": (C.c_int, [P, P, P, C.c_int32]),
+    "sgfim_load_sensors": (C.c_int, [P, P, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "sgfim_result_windows": (C.c_int, [P, PP]),
     "sgfim_load_zipf": (C.c_int, [P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                   C.c_int32, C.c_double, C.c_double, C.c_uint64]),

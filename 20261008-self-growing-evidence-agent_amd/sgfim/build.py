@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libsgfim.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["abi.cpp", "kernels_bitmap.hip", "kernels_count.hip", "kernels_pair.hip",
-           "kernels_search.hip", "kernels_mlp.hip", "kernels_gen.hip", "mlp_abi.cpp", "windows.cpp"]
+           "kernels_search.hip", "kernels_mlp.hip", "kernels_gen.hip", "kernels_sensor.hip", "mlp_abi.cpp"]
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

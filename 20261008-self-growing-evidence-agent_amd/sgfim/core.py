@@ -170,6 +170,21 @@ class Context:
         mc = np.ascontiguousarray(min_counts, dtype=np.int64)
         self._check(self._L.sgfim_set_window_thresholds(self._h, _p(mc), int(mc.shape[0])), "set_window_thresholds")
 
+    def load_sensors(self, x: np.ndarray, n_bins: int, window: int = 1, stride: int = 1):
+        """Transactionise a sensor matrix [rows][sensors] on the GPU: quantile bins per
+        sensor over the whole column, one transaction per window of its medians' bins
+        (item of sensor s, bin b = s * n_bins + b + 1)."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        if x.ndim != 2:
+            raise ValueError("x must be [rows][sensors]")
+        self._check(self._L.sgfim_load_sensors(self._h, _p(x), x.shape[0], x.shape[1], int(n_bins), int(window),
+                                               int(stride)), "load_sensors")
+        nt = C.c_int64()
+        ni = C.c_int32()
+        w = C.c_int64()
+        self._L.sgfim_info(self._h, C.byref(nt), C.byref(ni), C.byref(w))
+        self.n_txn, self.n_items, self.words = nt.value, ni.value, w.value
+
     def mine_thresholds(self, min_counts, flags: int = 0, root_topk=None, convert: bool = True) -> MineResult:
         """Exact MFIs of one loaded set at several thresholds in one batched search
         (result.windows tags each set with its threshold's index). With ROOT_TOPK,

@@ -131,6 +131,17 @@ int sgfim_window_supports(sgfim_ctx* ctx, int32_t window, int64_t* out);
  * sgfim_mine_args.min_count then applies to every window). */
 int sgfim_set_window_thresholds(sgfim_ctx* ctx, const int64_t* min_counts, int32_t n);
 
+/* Sensor matrix -> transactions on the GPU (replaces the reference's transactionisation,
+ * skab_spike.py:37-80 fit_bins / bin_index / window_transactions, and the FIMI text
+ * round trip): x is row-major [n_rows][n_sensors] (finite values); per sensor the bin
+ * edges are unique(quantile(column, linspace(0, 1, n_bins + 1))) over the whole column
+ * (NumPy's linear interpolation; a constant column gets [v - 0.5, v + 0.5]); windows
+ * [s, s + window) for s = 0, stride, ... give one transaction each, holding for every
+ * sensor the item s * n_bins + b + 1 of the bin b of the window's median (the mean of the
+ * two middle values for an even window). window <= 64; window = stride = 1 bins rows. */
+int sgfim_load_sensors(sgfim_ctx* ctx, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
+                       int32_t window, int32_t stride);
+
 /* Several thresholds of one loaded set in one batched search: the next runs mine the
  * exact MFI family at each min_counts[i] (with SGFIM_ROOT_TOPK, root_topk[i] roots for
  * threshold i when root_topk is given) and tag every MFI with i (sgfim_result_windows).
