@@ -121,6 +121,7 @@ struct sgfim_ctx {
     std::vector<int64_t> multi_thr;     // thresholds of the next run on a single set (empty: one)
     std::vector<int32_t> multi_topk;    // per-threshold root counts for SGFIM_ROOT_TOPK (empty: args)
     std::vector<DevBuf> data_bufs;
+    std::vector<int32_t> sensor_items;  // [windows][sensors] items of the latest sensor load
     // proposal network
     bool scorer_set = false;
     SgScorer scorer{};
@@ -234,6 +235,7 @@ void data_free(sgfim_ctx* c)
     c->multi_topk.clear();
     c->pair_valid = false;
     c->win_thr.clear();
+    c->sensor_items.clear();
     c->loaded = false;
 }
 
@@ -702,6 +704,11 @@ static void sensor_edges(int64_t n, int32_t B, const double* val, std::vector<do
     }
 }
 
+namespace {
+int finish_sensor_load(sgfim_ctx* c, std::vector<DevBuf>& tmp, hipError_t e, int32_t S, int32_t B, int64_t n_win,
+                       const int32_t* d_items, const uint32_t* d_present, const int64_t* d_off);
+}
+
 int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
                        int32_t window, int32_t stride)
 {
@@ -790,10 +797,26 @@ int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_
     if (e == hipSuccess) e = hipMemsetAsync(d_present, 0, (size_t)S * B * 4, c->stream);
     if (e == hipSuccess)
         e = sg_sensor_bins(d_x, S, B, window, stride, n_win, d_edges, d_ne, d_items, d_present, d_off, c->stream);
+    return finish_sensor_load(c, tmp, e, S, B, n_win, d_items, d_present, d_off);
+}
+
+namespace {
+
+// The items [n_win][S] are on the device with their presence flags and the window offsets:
+// keep a host copy of the items, number the present ids densely and build the bitmaps.
+int finish_sensor_load(sgfim_ctx* c, std::vector<DevBuf>& tmp, hipError_t e, int32_t S, int32_t B, int64_t n_win,
+                       const int32_t* d_items, const uint32_t* d_present, const int64_t* d_off)
+{
     std::vector<uint32_t> present((size_t)S * B);
+    std::vector<int32_t> items((size_t)(n_win * S));
     if (e == hipSuccess) e = hipMemcpyAsync(present.data(), d_present, present.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpyAsync(items.data(), d_items, items.size() * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return bail("sensor bins", e);
+    if (e != hipSuccess) {
+        dev_free_all(c, tmp);
+        return fail(c, SGFIM_E_DEVICE, "sensor bins: %s", hipGetErrorString(e));
+    }
     const int32_t idmax = S * B;
     std::vector<int32_t> idmap((size_t)idmax + 1, -1);
     c->ids.clear();
@@ -803,10 +826,58 @@ int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_
             c->ids.push_back(id);
         }
     const int64_t win_off[2] = {0, n_win};
-    rc = build_from_device_csr(c, 1, win_off, n_win, d_off, d_items, n_win * S, idmax, idmap);
+    const int rc = build_from_device_csr(c, 1, win_off, n_win, d_off, d_items, n_win * S, idmax, idmap);
     (void)hipStreamSynchronize(c->stream);
     dev_free_all(c, tmp);
+    if (rc == SGFIM_OK) c->sensor_items.swap(items);
     return rc;
+}
+
+}  // namespace
+
+int sgfim_load_sensors_online(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
+                              int32_t window, int32_t stride, double* edges_out)
+{
+    (void)hipGetLastError();
+    if (!c) return SGFIM_E_INVAL;
+    const int32_t S = n_sensors, B = n_bins;
+    if (!x || n_rows < 1 || S < 1 || B < 2 || window < 1 || window > 64 || stride < 1 || n_rows < window)
+        return fail(c, SGFIM_E_INVAL, "bad sensor matrix arguments (n_bins >= 2, 1 <= window <= 64, window <= rows)");
+    if ((int64_t)S * B > 65535) return fail(c, SGFIM_E_INVAL, "sensors x bins must be <= 65535");
+    const int64_t n_win = (n_rows - window) / stride + 1;
+    if (n_win >= ((int64_t)1 << 31)) return fail(c, SGFIM_E_INVAL, "too many windows");
+    for (int64_t i = 0; i < n_rows * S; i++)
+        if (!std::isfinite(x[i])) return fail(c, SGFIM_E_INVAL, "sensor value %lld is not finite", (long long)i);
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    data_free(c);
+    const int64_t nv = n_rows * S, ne = n_win * S * (B + 1);
+    std::vector<DevBuf> tmp;
+    double *d_x = nullptr, *d_E = nullptr;
+    int64_t* d_off = nullptr;
+    int32_t* d_items = nullptr;
+    uint32_t* d_present = nullptr;
+    int rc;
+    if ((rc = dev_alloc_t(c, tmp, &d_x, (size_t)nv)) || (rc = dev_alloc_t(c, tmp, &d_E, (size_t)ne)) ||
+        (rc = dev_alloc_t(c, tmp, &d_items, (size_t)(n_win * S))) || (rc = dev_alloc_t(c, tmp, &d_present, (size_t)S * B)) ||
+        (rc = dev_alloc_t(c, tmp, &d_off, (size_t)n_win + 1))) {
+        dev_free_all(c, tmp);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_x, x, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_present, 0, (size_t)S * B * 4, c->stream);
+    if (e == hipSuccess) e = sg_online_bins(d_x, S, B, window, stride, n_win, d_E, d_items, d_present, d_off, c->stream);
+    if (e == hipSuccess && edges_out)
+        e = hipMemcpyAsync(edges_out, d_E, (size_t)ne * 8, hipMemcpyDeviceToHost, c->stream);
+    return finish_sensor_load(c, tmp, e, S, B, n_win, d_items, d_present, d_off);
+}
+
+int sgfim_sensor_items(sgfim_ctx* c, int32_t* items, int64_t cap, int64_t* n)
+{
+    if (!c || !n) return SGFIM_E_INVAL;
+    *n = (int64_t)c->sensor_items.size();
+    if (items) std::copy(c->sensor_items.begin(), c->sensor_items.begin() + std::min<int64_t>(cap, *n), items);
+    return SGFIM_OK;
 }
 
 namespace {

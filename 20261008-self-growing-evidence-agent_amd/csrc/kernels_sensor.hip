@@ -11,6 +11,8 @@
 //                              s + window <= n; one item per sensor: the bin of the
 //                              window's median (the mean of the two middle values for an
 //                              even window).
+// The online variant (k_online_p2, k_online_bins) bins each window median against
+// streaming P^2 estimates instead (code/pipeline/online_discretize.py:22-131).
 // The item of sensor s in bin b is s * B + b + 1. The column sort runs on the device
 // (hipCUB segmented radix sort); the host turns the gathered order statistics into the
 // edges with NumPy's interpolation arithmetic, so the bins are the reference's bit for bit.
@@ -38,6 +40,32 @@ __global__ void k_sensor_gather(const double* __restrict__ xs, const int64_t* __
     if (i < m) out[i] = xs[pos[i]];
 }
 
+// Median of the window col[0], col[S], ..., col[(window - 1) S] (insertion sort; the mean
+// of the two middle values for an even window, as NumPy and pandas compute it).
+__device__ inline double sg_window_median(const double* col, int32_t S, int32_t window)
+{
+    double v[SG_MAX_WINDOW];
+    for (int k = 0; k < window; k++) {
+        const double y = col[(int64_t)k * S];
+        int j = k;
+        while (j > 0 && v[j - 1] > y) {
+            v[j] = v[j - 1];
+            j--;
+        }
+        v[j] = y;
+    }
+    return (window & 1) ? v[window / 2] : (v[window / 2 - 1] + v[window / 2]) / 2.0;
+}
+
+// searchsorted(e[0..n_e), v, side="right") - 1, clamped to [0, n_e - 2].
+__device__ inline int32_t sg_bin_of(const double* e, int32_t n_e, double v)
+{
+    int32_t c = 0;
+    while (c < n_e && e[c] <= v) c++;
+    const int32_t b = c - 1;
+    return b < 0 ? 0 : (b > n_e - 2 ? n_e - 2 : b);
+}
+
 // One thread per (window t, sensor s): median of the window, its bin, the item id.
 // edges[s][0..ne[s]) ascending; present[s*B + b] = 1 for every bin that occurs.
 __global__ __launch_bounds__(256) void k_sensor_bins(const double* __restrict__ x, int32_t S, int32_t B,
@@ -50,25 +78,169 @@ __global__ __launch_bounds__(256) void k_sensor_bins(const double* __restrict__ 
     if (i >= n_win * S) return;
     const int64_t t = i / S;
     const int32_t s = (int32_t)(i - t * S);
-    double v[SG_MAX_WINDOW];
-    const double* col = x + (int64_t)t * stride * S + s;
-    for (int k = 0; k < window; k++) {   // insertion sort of the window
-        const double y = col[(int64_t)k * S];
-        int j = k;
-        while (j > 0 && v[j - 1] > y) {
-            v[j] = v[j - 1];
+    const double med = sg_window_median(x + (int64_t)t * stride * S + s, S, window);
+    const int32_t id = s * B + sg_bin_of(edges + (int64_t)s * (B + 1), ne[s], med) + 1;
+    items[i] = id;
+    present[id - 1] = 1u;
+}
+
+// ---- Online discretisation: P^2 quantile estimators (code/pipeline/online_discretize.py) ----
+//
+// One estimator for quantile q keeps five marker heights h and positions n. The first
+// five samples are kept sorted in h (stable insertion, as Python's sorted); at the fifth,
+// n = (0, 1 + 2q, 1 + 4q, 3 + 2q, 4). Each later sample x finds its cell k (a new minimum
+// or maximum replaces h[0] or h[4]), every position gains 1 and positions 1..k gain one
+// more, then markers i = 1, 2, 3 in turn move by qp = (d + 1)(h[i+1] - h[i-1]) / span,
+// d = n[i] - n[i-1], span = n[i+1] - n[i-1], when d >= 1, span > 1 and h[i] + qp lies
+// strictly between h[i-1] and h[i+1] (P2Quantile.add, :37-64). The estimate is h[2], or
+// the median of the fewer than five samples seen (0 before any) (value, :66-69).
+// Each step is rounded as the reference rounds it: no multiply-add contraction.
+struct SgP2 {
+    double h[5];
+    double n[5];
+    int c;   // samples seen, up to 5
+};
+
+__device__ inline void sg_p2_add(SgP2& p, double q, double x)
+{
+#pragma clang fp contract(off)
+    if (p.c < 5) {
+        double y = x;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            if (j < p.c) {
+                if (p.h[j] > y) {
+                    const double t = p.h[j];
+                    p.h[j] = y;
+                    y = t;
+                }
+            } else if (j == p.c) {
+                p.h[j] = y;
+            }
+        }
+        if (++p.c == 5) {
+            p.n[0] = 0.0;
+            p.n[1] = 1.0 + 2.0 * q;
+            p.n[2] = 1.0 + 4.0 * q;
+            p.n[3] = 3.0 + 2.0 * q;
+            p.n[4] = 4.0;
+        }
+        return;
+    }
+    int k;
+    if (x < p.h[0]) {
+        p.h[0] = x;
+        k = 0;
+    } else if (x < p.h[1]) {
+        k = 0;
+    } else if (x < p.h[2]) {
+        k = 1;
+    } else if (x < p.h[3]) {
+        k = 2;
+    } else if (x <= p.h[4]) {
+        k = 3;
+    } else {
+        p.h[4] = x;
+        k = 3;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) p.n[i] = p.n[i] + 1.0;
+#pragma unroll
+    for (int i = 1; i < 4; i++)
+        if (i <= k) p.n[i] = p.n[i] + 1.0;
+#pragma unroll
+    for (int i = 1; i <= 3; i++) {
+        const double d = p.n[i] - p.n[i - 1];
+        const double span = p.n[i + 1] - p.n[i - 1];
+        if (d >= 1.0 && span > 1.0) {
+            const double qp = (d + 1.0) * (p.h[i + 1] - p.h[i - 1]) / span;
+            const double m = p.h[i] + qp;
+            if (p.h[i - 1] < m && m < p.h[i + 1]) p.h[i] = m;
+        }
+    }
+}
+
+__device__ inline double sg_p2_value(const SgP2& p)
+{
+#pragma clang fp contract(off)
+    switch (p.c) {
+    case 0: return 0.0;
+    case 1: return p.h[0];
+    case 2: return (p.h[0] + p.h[1]) / 2.0;
+    case 3: return p.h[1];
+    case 4: return (p.h[1] + p.h[2]) / 2.0;
+    default: return p.h[2];
+    }
+}
+
+// One lane per (sensor s, estimator e), e = 0..B-2 at quantile (e + 1) / B: the lane runs
+// the sensor's stream window by window (every sample of [w stride, w stride + window) is
+// added, overlapping windows add a sample again, as run_file does, :118-123) and writes
+// its estimate after window w to E[w][s][1 + e]. Consecutive lanes take consecutive
+// sensors, so the loads of one row coalesce.
+__global__ __launch_bounds__(64) void k_online_p2(const double* __restrict__ x, int32_t S, int32_t B,
+                                                  int32_t window, int32_t stride, int64_t n_win,
+                                                  double* __restrict__ E)
+{
+#pragma clang fp contract(off)
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= S * (B - 1)) return;
+    const int32_t s = i % S, e = i / S;
+    const double q = (double)(e + 1) / (double)B;
+    SgP2 p;
+#pragma unroll
+    for (int j = 0; j < 5; j++) p.h[j] = p.n[j] = 0.0;
+    p.c = 0;
+    for (int64_t w = 0; w < n_win; w++) {
+        const double* col = x + w * stride * S + s;
+        for (int32_t t = 0; t < window; t++) sg_p2_add(p, q, col[(int64_t)t * S]);
+        E[(w * S + s) * (B + 1) + 1 + e] = sg_p2_value(p);
+    }
+}
+
+// One thread per (window w, sensor s): the edges of OnlineBinner.edges (:86-103) from the
+// B - 1 estimates in E[w][s][1..B-1] -- sorted, span = max(q_last - q_0, 1e-9), minimum
+// gap 0.05 span (B = 3: both inner edges re-centred on their midpoint when closer than
+// the gap; otherwise each edge pushed up to the previous one plus the gap), outer edges
+// q_0 - span and q_last + span -- written back in place, then the window median's bin
+// under them (bin_index, :106-108) and the item s * B + b + 1.
+__global__ __launch_bounds__(256) void k_online_bins(const double* __restrict__ x, int32_t S, int32_t B,
+                                                     int32_t window, int32_t stride, int64_t n_win,
+                                                     double* __restrict__ E, int32_t* __restrict__ items,
+                                                     uint32_t* __restrict__ present)
+{
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_win * S) return;
+    const int64_t t = i / S;
+    const int32_t s = (int32_t)(i - t * S);
+    double* ed = E + i * (B + 1);
+    for (int32_t k = 2; k < B; k++) {   // stable insertion sort of ed[1..B-1]
+        const double y = ed[k];
+        int32_t j = k;
+        while (j > 1 && ed[j - 1] > y) {
+            ed[j] = ed[j - 1];
             j--;
         }
-        v[j] = y;
+        ed[j] = y;
     }
-    const double med = (window & 1) ? v[window / 2] : (v[window / 2 - 1] + v[window / 2]) / 2.0;
-    const double* e = edges + (int64_t)s * (B + 1);
-    const int32_t n_e = ne[s];
-    int32_t c = 0;   // edges <= med
-    while (c < n_e && e[c] <= med) c++;
-    int32_t b = c - 1;
-    b = b < 0 ? 0 : (b > n_e - 2 ? n_e - 2 : b);
-    const int32_t id = s * B + b + 1;
+    double span = ed[B - 1] - ed[1];
+    if (span < 1e-9) span = 1e-9;
+    const double gap = 0.05 * span;
+    if (B == 3) {
+        if (ed[2] - ed[1] < gap) {
+            const double mid = (ed[1] + ed[2]) / 2.0;
+            ed[1] = mid - gap / 2.0;
+            ed[2] = mid + gap / 2.0;
+        }
+    } else {
+        for (int32_t k = 2; k < B; k++)
+            if (ed[k] < ed[k - 1] + gap) ed[k] = ed[k - 1] + gap;
+    }
+    ed[0] = ed[1] - span;
+    ed[B] = ed[B - 1] + span;
+    const double med = sg_window_median(x + t * stride * S + s, S, window);
+    const int32_t id = s * B + sg_bin_of(ed, B + 1, med) + 1;
     items[i] = id;
     present[id - 1] = 1u;
 }
@@ -108,6 +280,21 @@ hipError_t sg_sensor_bins(const double* x, int32_t S, int32_t B, int32_t window,
     if (m > 0)
         hipLaunchKernelGGL(k_sensor_bins, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, S, B, window, stride,
                            n_win, edges, ne, items, present);
+    hipLaunchKernelGGL(k_sensor_offsets, dim3((unsigned)((n_win + 1 + 255) / 256)), dim3(256), 0, s, n_win, S, off);
+    return hipGetLastError();
+}
+
+hipError_t sg_online_bins(const double* x, int32_t S, int32_t B, int32_t window, int32_t stride, int64_t n_win,
+                          double* E, int32_t* items, uint32_t* present, int64_t* off, hipStream_t s)
+{
+    const int32_t lanes = S * (B - 1);
+    if (n_win > 0) {
+        hipLaunchKernelGGL(k_online_p2, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0, s, x, S, B, window, stride,
+                           n_win, E);
+        const int64_t m = n_win * S;
+        hipLaunchKernelGGL(k_online_bins, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, S, B, window, stride,
+                           n_win, E, items, present);
+    }
     hipLaunchKernelGGL(k_sensor_offsets, dim3((unsigned)((n_win + 1 + 255) / 256)), dim3(256), 0, s, n_win, S, off);
     return hipGetLastError();
 }

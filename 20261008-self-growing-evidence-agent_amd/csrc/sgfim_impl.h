@@ -166,6 +166,9 @@ hipError_t sg_sensor_gather(const double* xs, const int64_t* pos, int64_t m, dou
 hipError_t sg_sensor_bins(const double* x, int32_t S, int32_t B, int32_t window, int32_t stride, int64_t n_win,
                           const double* edges, const int32_t* ne, int32_t* items, uint32_t* present, int64_t* off,
                           hipStream_t s);
+// Online (P^2) variant: E [n_win][S][B + 1] receives the edges used for each window.
+hipError_t sg_online_bins(const double* x, int32_t S, int32_t B, int32_t window, int32_t stride, int64_t n_win,
+                          double* E, int32_t* items, uint32_t* present, int64_t* off, hipStream_t s);
 int sg_count_group_blocks_per_cu(int32_t chunk_words);
 hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s);
 hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int32_t* pair,

@@ -179,6 +179,36 @@ class Context:
             raise ValueError("x must be [rows][sensors]")
         self._check(self._L.sgfim_load_sensors(self._h, _p(x), x.shape[0], x.shape[1], int(n_bins), int(window),
                                                int(stride)), "load_sensors")
+        self._refresh_info()
+
+    def load_sensors_online(self, x: np.ndarray, n_bins: int = 3, window: int = 30, stride: int = 15,
+                            return_edges: bool = False):
+        """Transactionise a sensor matrix [rows][sensors] on the GPU with streaming P^2
+        quantile bins (each window's median binned with the edges of that moment); the
+        items and bitmaps are as for load_sensors. With return_edges, returns the edges used,
+        float64 [windows][sensors][n_bins + 1]."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        if x.ndim != 2:
+            raise ValueError("x must be [rows][sensors]")
+        n, S = x.shape
+        edges = None
+        if return_edges and n >= window >= 1 and stride >= 1:
+            edges = np.empty(((n - window) // stride + 1, S, int(n_bins) + 1), dtype=np.float64)
+        self._check(self._L.sgfim_load_sensors_online(self._h, _p(x), n, S, int(n_bins), int(window), int(stride),
+                                                      _p(edges) if edges is not None else None),
+                    "load_sensors_online")
+        self._refresh_info()
+        return edges
+
+    def sensor_items(self) -> np.ndarray:
+        """Items [windows][sensors] of the latest sensor load (sensor s, bin b -> s * n_bins + b + 1)."""
+        n = C.c_int64()
+        self._check(self._L.sgfim_sensor_items(self._h, None, 0, C.byref(n)), "sensor_items")
+        out = np.empty(n.value, dtype=np.int32)
+        self._check(self._L.sgfim_sensor_items(self._h, _p(out), n.value, C.byref(n)), "sensor_items")
+        return out
+
+    def _refresh_info(self):
         nt = C.c_int64()
         ni = C.c_int32()
         w = C.c_int64()

@@ -142,6 +142,23 @@ int sgfim_set_window_thresholds(sgfim_ctx* ctx, const int64_t* min_counts, int32
 int sgfim_load_sensors(sgfim_ctx* ctx, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
                        int32_t window, int32_t stride);
 
+/* Online (streaming) variant for the self-growing pattern library: per sensor, n_bins - 1
+ * P^2 quantile estimators at i / n_bins follow the stream (replaces online_discretize.py:
+ * 22-108 P2Quantile / OnlineBinner / bin_index, driven as run_file :111-131 and the
+ * ablation scripts drive them): for each window [s, s + window), s = 0, stride, ..., every
+ * sample of the window is added to the sensor's estimators (overlapping windows add a
+ * sample again), then the window median is binned with the edges of that moment --
+ * sorted estimates, minimum gap 0.05 of their span, outer edges one span outside. The
+ * items and the loaded bitmaps are as for sgfim_load_sensors. When edges is not NULL it
+ * receives the edges used, [windows][n_sensors][n_bins + 1] doubles. n_bins >= 2,
+ * window <= 64. */
+int sgfim_load_sensors_online(sgfim_ctx* ctx, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
+                              int32_t window, int32_t stride, double* edges);
+
+/* Items [windows][n_sensors] of the latest sensor load (either variant): *n receives their
+ * number, and when items is not NULL the first min(cap, *n) are copied to it. */
+int sgfim_sensor_items(sgfim_ctx* ctx, int32_t* items, int64_t cap, int64_t* n);
+
 /* Several thresholds of one loaded set in one batched search: the next runs mine the
  * exact MFI family at each min_counts[i] (with SGFIM_ROOT_TOPK, root_topk[i] roots for
  * threshold i when root_topk is given) and tag every MFI with i (sgfim_result_windows).
