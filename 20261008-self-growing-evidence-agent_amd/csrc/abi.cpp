@@ -1426,12 +1426,12 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
         }
         memcpy(B->h_ctl, B->h_ctl + (1 + slot) * CTL_SLOTS, CTL_SLOTS * sizeof(int64_t));
         if (B->h_ctl[CTL_BATCH_N] > 0) S->rounds++;   // a round queued after the frontier emptied is not counted
+        S->count_launches++;   // every queued round launches the counting kernel once (an empty one returns at once)
         if (B->h_ctl[CTL_UNITS] > 0 && B->h_ctl[CTL_BATCH_N] > 0) {
             // device-side duration of the counting kernel: first workgroup start to last
             // workgroup end, from the constant-rate wall clock
             const int64_t t0 = B->h_ctl[CTL_T0], t1 = B->h_ctl[CTL_T1];
             if (t1 > t0 && c->wall_khz > 0) S->count_ms += (double)(t1 - t0) / (double)c->wall_khz;
-            S->count_launches++;
             if (!S->use_graph) {
                 float ms = 0.0f;
                 if (hipEventElapsedTime(&ms, B->ev0, B->ev1) == hipSuccess) S->event_ms += ms;

@@ -560,6 +560,7 @@ __device__ __forceinline__ uint32_t sg_wave_sum4(uint32_t v0, uint32_t v1, uint3
 
 #define SG_TILE_THREADS 512
 #define SG_CW 256     // row words per chunk
+#define SG_SORT_MAX 1024   // longest candidate list that a unit sorts
 
 // Popcounts of x & c over the PER words a lane holds (all PER, or the first qn).
 template <int PER, bool FULL>
@@ -573,18 +574,16 @@ __device__ __forceinline__ uint32_t sg_andpop(const uint64_t (&x)[PER], const ui
 }
 
 // One wave step of a chunk: SG_RC listed candidates against the children that need
-// them. Candidate r's row chunk is loaded once into registers; for every child j that
-// needs one of them, the child prefix is read from LDS once and ANDed + popcounted with
-// each candidate that needs it; the four per-lane sums reduce together (sg_wave_sum4)
-// and lane 16r adds candidate r's total to its cell.
-template <int PER, bool FULL>
-__device__ __forceinline__ void sg_tile_step(const uint64_t* __restrict__ seg_rows, int64_t stride, int64_t w0,
-                                             int wn, int qn, uint64_t nm, const int32_t (&fr)[SG_RC], int il_mine,
-                                             const uint64_t (*cp)[SG_CW], int ncb, int32_t* cells, int j0, int nu)
+// them. Candidate r's row chunk is loaded once into registers (sg_step_load); for every
+// child j that needs one of them, the child prefix is read from LDS once and ANDed +
+// popcounted with each candidate that needs it; the four per-lane sums reduce together
+// (sg_wave_sum4) and lane 16r adds candidate r's total to its cell (sg_step_count).
+template <int PER>
+__device__ __forceinline__ void sg_step_load(uint64_t (&x)[SG_RC][PER], const uint64_t* __restrict__ seg_rows,
+                                             int64_t stride, int64_t w0, int wn, uint64_t nm,
+                                             const int32_t (&fr)[SG_RC])
 {
     const int lane = threadIdx.x & 63;
-    const int rl = lane >> 4;
-    uint64_t x[SG_RC][PER];
 #pragma unroll
     for (int rr = 0; rr < SG_RC; rr++) {
         const bool any = ((nm >> rr) & 0x1111111111111111ull) != 0ull;   // some child needs candidate rr
@@ -592,9 +591,17 @@ __device__ __forceinline__ void sg_tile_step(const uint64_t* __restrict__ seg_ro
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int w = lane + 64 * q;
-            x[rr][q] = (any && (FULL || w < wn)) ? row[w] : 0ull;
+            x[rr][q] = (any && w < wn) ? row[w] : 0ull;
         }
     }
+}
+
+template <int PER, bool FULL>
+__device__ __forceinline__ void sg_step_count(const uint64_t (&x)[SG_RC][PER], int qn, uint64_t nm, int il_mine,
+                                              const uint64_t (*cp)[SG_CW], int ncb, int32_t* cells, int j0, int nu)
+{
+    const int lane = threadIdx.x & 63;
+    const int rl = lane >> 4;
     // keep the per-child masks inside the step: hoisted, their branch conditions would
     // not fit the scalar registers
     asm volatile("" : "+s"(nm));
@@ -611,6 +618,22 @@ __device__ __forceinline__ void sg_tile_step(const uint64_t* __restrict__ seg_ro
         const uint32_t t = sg_wave_sum4(a0, a1, a2, a3);
         if ((lane & 15) == 0 && ((mj >> rl) & 1u) && t) atomicAdd(&cells[(int64_t)(j0 + j) * nu + il_mine], (int32_t)t);
     }
+}
+
+// Step i0 of the unit's listed candidates: lane (j, r) = (lane >> 2, lane & 3) holds
+// whether child j needs candidate r, so the ballot keeps child j's four bits at 4j.
+__device__ __forceinline__ void sg_step_plan(const uint32_t* s_cand, const int32_t* clist, int nc, int i0,
+                                             uint64_t& nm, int32_t (&fr)[SG_RC], int& il_mine)
+{
+    const int lane = threadIdx.x & 63;
+    const int rq = lane & 3, jq = lane >> 2;
+    const uint32_t ent = i0 + rq < nc ? s_cand[i0 + rq] : 0u;
+    const int32_t fl = ent ? clist[(int)(ent & 0xffffu)] : -1;
+    nm = __ballot((ent >> (16 + jq)) & 1u);   // bit 4*j + r
+#pragma unroll
+    for (int rr = 0; rr < SG_RC; rr++) fr[rr] = __builtin_amdgcn_readlane(fl, rr);
+    const uint32_t ent_mine = i0 + (lane >> 4) < nc ? s_cand[i0 + (lane >> 4)] : 0u;
+    il_mine = (int)(ent_mine & 0xffffu);
 }
 
 // Unit (record g, block of up to SG_CB children, range of up to SG_CR row chunks).
@@ -631,7 +654,6 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     constexpr int PER = CW / 64;
     constexpr int NW = SG_TILE_THREADS / 64;
     static_assert(SG_RC * SG_CB == 64, "one lane per (child, candidate) pair of a wave step");
-    __shared__ uint64_t pre[CW];
     __shared__ uint64_t cp[SG_CB][CW];       // child prefixes; the children's ok rows during set-up
     __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
     __shared__ int64_t s_unit;
@@ -644,7 +666,8 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     const int64_t n_units = S.ctl[CTL_UNITS];
     const int64_t n_blocks = S.ctl[CTL_BLOCKS];
     const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
-    if (threadIdx.x == 0 && n_units > 0)
+    if (n_units == 0) return;   // a round queued after the frontier emptied: nothing to count
+    if (threadIdx.x == 0)
         atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
     for (;;) {
         if (threadIdx.x == 0) {
@@ -689,66 +712,110 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
             okr[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
         }
         __syncthreads();
+        // Candidate i needs counting for child j0 + j when the pair is frequent, except that
+        // a candidate that is itself a child (i < ntail) is counted only below the
+        // diagonal: |T(X u {e_a, e_b})| is one support, counted once in cell (a, b),
+        // a > b, and read from there for cell (b, a).
+        uint32_t* lst = reinterpret_cast<uint32_t*>(okr + SG_CB * 64);   // past the ok rows
         for (int i = threadIdx.x; i < nu; i += blockDim.x) {
             const int32_t f = S.clist[u0 + i];
+            const int jlo = i < m.ntail ? i + 1 - j0 : 0;   // children j with j0 + j > i
             uint32_t mask = 0u;
-            for (int j = 0; j < ncb; j++)
-                if (s_child[j] != f && ((okr[j * 64 + (f >> 6)] >> (f & 63)) & 1ull)) mask |= 1u << j;
-            if (mask) s_cand[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
+            for (int j = max(jlo, 0); j < ncb; j++)
+                if ((okr[j * 64 + (f >> 6)] >> (f & 63)) & 1ull) mask |= 1u << j;
+            if (mask) lst[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
         }
         __syncthreads();
         const int nc = s_nc;
+        // listed in descending order of (children mask, position): candidates that the
+        // same children need share wave steps, so fewer child steps hold unneeded lanes
+        // (a rank sort; long lists keep their order)
+        for (int a = threadIdx.x; a < nc; a += blockDim.x) {
+            const uint32_t v = lst[a];
+            int r = a;
+            if (nc <= SG_SORT_MAX) {
+                r = 0;
+                for (int b = 0; b < nc; b++) r += lst[b] > v;
+            }
+            s_cand[r] = v;
+        }
+        __syncthreads();
         const uint64_t* seg_rows = rows + sg.row_base;
         int32_t* cells = S.cells + S.celloff[g];
+        const int32_t* clist = S.clist + u0;
+        // the wave's first step takes the same candidates in every chunk of the unit
+        const int i0f = wave * SG_RC;
+        uint64_t nmf;
+        int32_t frf[SG_RC];
+        int ilf;
+        sg_step_plan(s_cand, clist, nc, i0f, nmf, frf, ilf);
         for (int c = c0; c < c1; c++) {
             const int64_t w0 = (int64_t)c * CW;
             const int wn = (int)min((int64_t)CW, sg.words - w0);
             const int qn = (wn + 63) >> 6;   // 64-word columns holding row words
-            // prefix of X over the chunk (zero past the last word)
-            for (int w = threadIdx.x; w < CW; w += blockDim.x) {
-                uint64_t v = 0ull;
-                if (w < wn) {
-                    v = ~0ull;
-                    for (int i = 0; i < m.k; i++) v &= seg_rows[(int64_t)s_items[i] * sg.stride + w0 + w];
-                }
-                pre[w] = v;
-            }
-            __syncthreads();
-            // child prefixes (zero past the last word) and child supports
-            for (int j = wave; j < ncb; j += NW) {
-                uint32_t acc = 0;
-                const uint64_t* row = seg_rows + (int64_t)s_child[j] * sg.stride + w0;
+            __syncthreads();   // every wave is done with the previous chunk's prefixes
+            // child prefixes T(X) & T(e) (zero past the last word) and the children's
+            // supports: each wave ANDs the rows of X at its lanes' words itself, so the
+            // rows of X and of its children arrive in one round trip
+            {
+                uint64_t pv[PER], rv[2][PER];
 #pragma unroll
-                for (int q = 0; q < PER; q++) {
-                    const int w = lane + 64 * q;
-                    const uint64_t v = w < wn ? (pre[w] & row[w]) : 0ull;
-                    cp[j][w] = v;
-                    acc += (uint32_t)__popcll(v);
+                for (int jj = 0; jj < 2; jj++) {
+                    const int j = wave + NW * jj;
+                    const uint64_t* row = seg_rows + (int64_t)(j < ncb ? s_child[j] : 0) * sg.stride + w0;
+#pragma unroll
+                    for (int q = 0; q < PER; q++) {
+                        const int w = lane + 64 * q;
+                        rv[jj][q] = (j < ncb && w < wn) ? row[w] : 0ull;
+                    }
                 }
-                const uint32_t sv = sg_wave_sum(acc);
-                if (lane == 0 && sv) atomicAdd(&S.ccount[S.choff[g] + j0 + j], (int32_t)sv);
+#pragma unroll
+                for (int q = 0; q < PER; q++) pv[q] = lane + 64 * q < wn ? ~0ull : 0ull;
+                for (int i = 0; i < m.k; i++) {
+                    const uint64_t* row = seg_rows + (int64_t)s_items[i] * sg.stride + w0;
+#pragma unroll
+                    for (int q = 0; q < PER; q++) {
+                        const int w = lane + 64 * q;
+                        if (w < wn) pv[q] &= row[w];
+                    }
+                }
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++) {
+                    const int j = wave + NW * jj;
+                    if (j < ncb) {
+                        uint32_t acc = 0;
+#pragma unroll
+                        for (int q = 0; q < PER; q++) {
+                            const uint64_t v = pv[q] & rv[jj][q];
+                            cp[j][lane + 64 * q] = v;
+                            acc += (uint32_t)__popcll(v);
+                        }
+                        const uint32_t sv = sg_wave_sum(acc);
+                        if (lane == 0 && sv) atomicAdd(&S.ccount[S.choff[g] + j0 + j], (int32_t)sv);
+                    }
+                }
             }
+            // the first step's candidates load while the other waves finish their prefixes
+            uint64_t xf[SG_RC][PER];
+            sg_step_load<PER>(xf, seg_rows, sg.stride, w0, wn, i0f < nc ? nmf : 0ull, frf);
             __syncthreads();
-            for (int i0 = wave * SG_RC; i0 < nc; i0 += NW * SG_RC) {
-                // lane (j, r): does child j need candidate r of this step?
-                const int rq = lane & 3, jq = lane >> 2;
-                const uint32_t ent = i0 + rq < nc ? s_cand[i0 + rq] : 0u;
-                const int32_t fl = ent ? S.clist[u0 + (int)(ent & 0xffffu)] : -1;
-                const uint64_t nm = __ballot((ent >> (16 + jq)) & 1u);   // bit 4*j + r
+            if (i0f < nc) {
+                if (qn == PER) sg_step_count<PER, true>(xf, qn, nmf, ilf, cp, ncb, cells, j0, nu);
+                else sg_step_count<PER, false>(xf, qn, nmf, ilf, cp, ncb, cells, j0, nu);
+            }
+            for (int i0 = i0f + NW * SG_RC; i0 < nc; i0 += NW * SG_RC) {
+                uint64_t nm;
                 int32_t fr[SG_RC];
-#pragma unroll
-                for (int rr = 0; rr < SG_RC; rr++) fr[rr] = __builtin_amdgcn_readlane(fl, rr);
-                const uint32_t ent_mine = i0 + (lane >> 4) < nc ? s_cand[i0 + (lane >> 4)] : 0u;
-                const int il_mine = (int)(ent_mine & 0xffffu);
-                if (qn == PER)
-                    sg_tile_step<PER, true>(seg_rows, sg.stride, w0, wn, qn, nm, fr, il_mine, cp, ncb, cells, j0, nu);
-                else
-                    sg_tile_step<PER, false>(seg_rows, sg.stride, w0, wn, qn, nm, fr, il_mine, cp, ncb, cells, j0, nu);
+                int il;
+                sg_step_plan(s_cand, clist, nc, i0, nm, fr, il);
+                uint64_t x[SG_RC][PER];
+                sg_step_load<PER>(x, seg_rows, sg.stride, w0, wn, nm, fr);
+                if (qn == PER) sg_step_count<PER, true>(x, qn, nm, il, cp, ncb, cells, j0, nu);
+                else sg_step_count<PER, false>(x, qn, nm, il, cp, ncb, cells, j0, nu);
             }
-            __syncthreads();
         }
     }
-    if (threadIdx.x == 0 && n_units > 0) {
+    if (threadIdx.x == 0) {
         __threadfence();
         atomicMax((unsigned long long*)&S.ctl[CTL_T1], (unsigned long long)wall_clock64());
     }
@@ -808,8 +875,10 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             for (int i = lane; i < nu; i += 64) {
                 if (i == j) continue;
                 const int32_t f = S.clist[u0 + i];
-                const int32_t cnt = crow[i];
-                if ((oke[f >> 6] >> (f & 63)) & 1ull) needed++;
+                // a later child's support with e sits in that child's row (counted once)
+                const bool mirror = i < m.ntail && i > j;
+                const int32_t cnt = mirror ? crow[(int64_t)(i - j) * nu + j] : crow[i];
+                if (!mirror && ((oke[f >> 6] >> (f & 63)) & 1ull)) needed++;
                 if (cnt >= sg.thr) {
                     atomicOr((unsigned long long*)&sF[wv][f >> 6], 1ull << (f & 63));
                     if (cnt == sup_e) atomicOr((unsigned long long*)&sE[wv][f >> 6], 1ull << (f & 63));
