@@ -89,6 +89,8 @@ struct sgfim_ctx {
     int64_t mem_used = 0;
     int wall_khz = 0;                   // wall_clock64() rate
     int n_cu = 256;                     // compute units of the device
+    std::string dev_name;               // gcnArchName of the device
+    int32_t tile_cw = 0, tile_cr = 0;   // counting-tile shape of the search (0: default)
     sgfim_opts opts{};
     // loaded data
     bool loaded = false;
@@ -249,13 +251,17 @@ int32_t pick_chunk(const sgfim_ctx* c, int64_t W)
     return 1024;
 }
 
-// Row-chunk words of one counting tile of the search (the tile kernel holds 17 chunks in LDS;
-// 256 words keep three 512-thread workgroups resident per CU).
-int32_t search_chunk(const sgfim_ctx* c, int64_t W)
+// Row-chunk words of one counting tile of the search (the tile kernel holds 16 child
+// prefixes of one chunk in LDS; 256 words keep three 512-thread workgroups resident per CU),
+// and chunks per counting unit.
+int32_t search_chunk(const sgfim_ctx* c)
 {
-    (void)W;
-    return c->opts.chunk_words == 512 ? 512 : 256;
+    if (c->tile_cw) return c->tile_cw;
+    const int32_t cw = c->opts.chunk_words;
+    return (cw == 128 || cw == 256 || cw == 512) ? cw : 256;
 }
+
+int32_t search_range(const sgfim_ctx* c) { return c->tile_cr ? c->tile_cr : 4; }
 
 int ensure_pair(sgfim_ctx* c)
 {
@@ -405,13 +411,13 @@ int launch_round(sgfim_ctx* c, SearchState* S)
 
 // (Re)use the context's search buffers for a run needing the given sizes.
 int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap, int32_t d, int32_t dw,
-                int32_t bmax, int32_t chunk, int64_t trace_cap, int64_t n_roots, int64_t cell_cap)
+                int32_t bmax, int32_t chunk, int32_t range, int64_t trace_cap, int64_t n_roots, int64_t cell_cap)
 {
     SearchBufs* B = c->SB;
     if (B) {
         const SgSearchDev& D = B->D;
         const bool fits = D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
-                          D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.trace_cap == trace_cap &&
+                          D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.chunk_range == range && D.trace_cap == trace_cap &&
                           B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + 2 * D.newborn_cap;
         if (fits) return SGFIM_OK;
         bufs_free(c);
@@ -426,6 +432,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     D.dw_max = dw;
     D.bmax = bmax;
     D.chunk_words = chunk;
+    D.chunk_range = range;
     D.count_grid = sg_count_group_blocks_per_cu(chunk) * c->n_cu;   // one persistent wave of workgroups
     D.trace_cap = trace_cap;
     D.cell_cap = cell_cap;
@@ -521,6 +528,7 @@ int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out)
         c->wall_khz = 0;
     }
     if (prop.multiProcessorCount > 0) c->n_cu = prop.multiProcessorCount;
+    c->dev_name = prop.gcnArchName;
     for (int i = 0; i < 2; i++)
         if (hipEventCreate(&c->pair_ev[i]) != hipSuccess) {
             (void)hipGetLastError();
@@ -870,6 +878,38 @@ int sgfim_load_sensors_online(sgfim_ctx* c, const double* x, int64_t n_rows, int
     if (e == hipSuccess && edges_out)
         e = hipMemcpyAsync(edges_out, d_E, (size_t)ne * 8, hipMemcpyDeviceToHost, c->stream);
     return finish_sensor_load(c, tmp, e, S, B, n_win, d_items, d_present, d_off);
+}
+
+int sgfim_set_tile(sgfim_ctx* c, int32_t chunk_words, int32_t chunk_range)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (!(chunk_words == 0 || chunk_words == 128 || chunk_words == 256 || chunk_words == 512) || chunk_range < 0 ||
+        chunk_range > 64)
+        return fail(c, SGFIM_E_INVAL, "tile: chunk_words must be 0, 128, 256 or 512 and chunk_range 0..64");
+    if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "tile: an anytime search is in progress");
+    c->tile_cw = chunk_words;
+    c->tile_cr = chunk_range;
+    return SGFIM_OK;
+}
+
+int sgfim_get_tile(const sgfim_ctx* c, int32_t* chunk_words, int32_t* chunk_range)
+{
+    if (!c || !chunk_words || !chunk_range) return SGFIM_E_INVAL;
+    *chunk_words = search_chunk(c);
+    *chunk_range = search_range(c);
+    return SGFIM_OK;
+}
+
+int sgfim_device_info(const sgfim_ctx* c, char* name, int32_t len, int32_t* n_cu)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (name && len > 0) {
+        const size_t n = std::min<size_t>((size_t)len - 1, c->dev_name.size());
+        memcpy(name, c->dev_name.data(), n);
+        name[n] = 0;
+    }
+    if (n_cu) *n_cu = c->n_cu;
+    return SGFIM_OK;
 }
 
 int sgfim_sensor_items(sgfim_ctx* c, int32_t* items, int64_t cap, int64_t* n)
@@ -1329,9 +1369,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         ring_cap = c->opts.ring_cap;
         newborn_cap = std::min(newborn_cap, (ring_cap - n_roots) / 2);
     }
-    int64_t maxw = 0;
-    for (const SgSegment& g : vs) maxw = std::max(maxw, g.words);
-    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, search_chunk(c, maxw),
+    if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, search_chunk(c), search_range(c),
                           c->trace ? c->trace_cap : 0, n_roots, cell_cap)))
         return rc;
     S->B = c->SB;

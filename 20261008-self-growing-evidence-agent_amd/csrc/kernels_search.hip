@@ -93,7 +93,6 @@ __device__ __forceinline__ void sg_features(const SgFeatIn& f, float* out)
 
 // ------------------------------------------------------------------ records
 #define SG_CB 16      // children per counting tile
-#define SG_CR 4       // row chunks per counting unit
 #define SG_DMAX 4096  // most items a search handles (the host refuses more)
 
 __device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
@@ -104,11 +103,11 @@ __device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
 
 #define SG_RC 4       // candidates held in registers per wave step
 
-// Row-chunk ranges of SG_CR chunks of a window of `words` words.
-__device__ __forceinline__ int32_t sg_ranges(int64_t words, int32_t chunk)
+// Row-chunk ranges (units of `range` chunks of `chunk` words) of a window of `words` words.
+__device__ __forceinline__ int32_t sg_ranges(int64_t words, int32_t chunk, int32_t range)
 {
     const int64_t nchunk = (words + chunk - 1) / chunk;
-    return (int32_t)((nchunk + SG_CR - 1) / SG_CR);
+    return (int32_t)((nchunk + range - 1) / range);
 }
 
 __device__ __forceinline__ int32_t sg_wave_imin(int32_t v)
@@ -170,7 +169,7 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     m.ncand = nu;
     m.ntail = nt;
     m.nblk = (nt + SG_CB - 1) / SG_CB;
-    m.nrange = sg_ranges(sg.words, S.chunk_words);
+    m.nrange = sg_ranges(sg.words, S.chunk_words, S.chunk_range);
     m.score = 0.0f;
     S.meta[slot] = m;
     S.key[slot] = key;
@@ -559,7 +558,6 @@ __device__ __forceinline__ uint32_t sg_wave_sum4(uint32_t v0, uint32_t v1, uint3
 }
 
 #define SG_TILE_THREADS 512
-#define SG_CW 256     // row words per chunk
 #define SG_SORT_MAX 1024   // longest candidate list that a unit sorts
 
 // Popcounts of x & c over the PER words a lane holds (all PER, or the first qn).
@@ -598,7 +596,7 @@ __device__ __forceinline__ void sg_step_load(uint64_t (&x)[SG_RC][PER], const ui
 
 template <int PER, bool FULL>
 __device__ __forceinline__ void sg_step_count(const uint64_t (&x)[SG_RC][PER], int qn, uint64_t nm, int il_mine,
-                                              const uint64_t (*cp)[SG_CW], int ncb, int32_t* cells, int j0, int nu)
+                                              const uint64_t (*cp)[PER * 64], int ncb, int32_t* cells, int j0, int nu)
 {
     const int lane = threadIdx.x & 63;
     const int rl = lane >> 4;
@@ -636,7 +634,7 @@ __device__ __forceinline__ void sg_step_plan(const uint32_t* s_cand, const int32
     il_mine = (int)(ent_mine & 0xffffu);
 }
 
-// Unit (record g, block of up to SG_CB children, range of up to SG_CR row chunks).
+// Unit (record g, block of up to SG_CB children, range of up to chunk_range row chunks).
 // Set-up, once per unit: the children's pair-ok rows give each candidate f of U the
 // mask of children e that need it (pair (e, f) frequent, f != e); candidates no child
 // needs are dropped, the others listed in LDS. Then per chunk: the prefix of X and the
@@ -646,11 +644,13 @@ __device__ __forceinline__ void sg_step_plan(const uint32_t* s_cand, const int32
 // atomic per needed (child, candidate, chunk). Skipped cells stay 0, which the
 // expansion reads as infrequent -- exact, since an infrequent pair has no frequent
 // superset. The child prefixes' popcounts give the children's supports.
-__global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_count_group(SgSearchDev S, const uint64_t* __restrict__ rows,
+// Registers are capped for 6 waves per SIMD (three workgroups per CU, as the LDS of a
+// 256-word chunk allows); 128-word chunks leave LDS for four workgroups, and ask for 8.
+template <int CW>
+__global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu(CW <= 128 ? 8 : 6, 8))) void k_count_group(SgSearchDev S, const uint64_t* __restrict__ rows,
                                                                  const SgSegment* __restrict__ segs,
                                                                  const uint64_t* __restrict__ ok)
 {
-    constexpr int CW = SG_CW;
     constexpr int PER = CW / 64;
     constexpr int NW = SG_TILE_THREADS / 64;
     static_assert(SG_RC * SG_CB == 64, "one lane per (child, candidate) pair of a wave step");
@@ -696,8 +696,8 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         const SgSegment sg = segs[m.seg];
         const int blk = (int)(bq - S.unit_off[g]);
         const int64_t nchunk = (sg.words + CW - 1) / CW;
-        const int c0 = cr * SG_CR;
-        const int c1 = (int)min((int64_t)(c0 + SG_CR), nchunk);
+        const int c0 = cr * S.chunk_range;
+        const int c1 = (int)min((int64_t)(c0 + S.chunk_range), nchunk);
         const int j0 = blk * SG_CB;
         const int ncb = min(SG_CB, m.ntail - j0);
         const int32_t u0 = S.coff[g];
@@ -716,7 +716,11 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         // a candidate that is itself a child (i < ntail) is counted only below the
         // diagonal: |T(X u {e_a, e_b})| is one support, counted once in cell (a, b),
         // a > b, and read from there for cell (b, a).
-        uint32_t* lst = reinterpret_cast<uint32_t*>(okr + SG_CB * 64);   // past the ok rows
+        // room for the unsorted list past the ok rows; a list that may not fit goes
+        // straight to s_cand and keeps its order
+        constexpr int LST_CAP = 2 * SG_CB * (CW - 64);
+        const bool sort = nu <= LST_CAP && nu <= SG_SORT_MAX;
+        uint32_t* lst = sort ? reinterpret_cast<uint32_t*>(okr + SG_CB * 64) : s_cand;
         for (int i = threadIdx.x; i < nu; i += blockDim.x) {
             const int32_t f = S.clist[u0 + i];
             const int jlo = i < m.ntail ? i + 1 - j0 : 0;   // children j with j0 + j > i
@@ -730,16 +734,15 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         // listed in descending order of (children mask, position): candidates that the
         // same children need share wave steps, so fewer child steps hold unneeded lanes
         // (a rank sort; long lists keep their order)
-        for (int a = threadIdx.x; a < nc; a += blockDim.x) {
-            const uint32_t v = lst[a];
-            int r = a;
-            if (nc <= SG_SORT_MAX) {
-                r = 0;
+        if (sort) {
+            for (int a = threadIdx.x; a < nc; a += blockDim.x) {
+                const uint32_t v = lst[a];
+                int r = 0;
                 for (int b = 0; b < nc; b++) r += lst[b] > v;
+                s_cand[r] = v;
             }
-            s_cand[r] = v;
+            __syncthreads();
         }
-        __syncthreads();
         const uint64_t* seg_rows = rows + sg.row_base;
         int32_t* cells = S.cells + S.celloff[g];
         const int32_t* clist = S.clist + u0;
@@ -989,9 +992,14 @@ __global__ void k_commit(SgSearchDev S, const SgSegment* __restrict__ segs, int3
 // Resident workgroups per CU of the tile kernel (sizes its persistent grid).
 int sg_count_group_blocks_per_cu(int32_t chunk_words)
 {
-    (void)chunk_words;
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group, SG_TILE_THREADS, 0) != hipSuccess || n < 1) {
+    hipError_t e;
+    switch (chunk_words) {
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<128>, SG_TILE_THREADS, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<512>, SG_TILE_THREADS, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<256>, SG_TILE_THREADS, 0); break;
+    }
+    if (e != hipSuccess || n < 1) {
         (void)hipGetLastError();
         n = 1;
     }
@@ -1040,7 +1048,11 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_count_group, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok);
+    switch (S.chunk_words) {
+    case 128: hipLaunchKernelGGL(k_count_group<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+    case 512: hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+    default: hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+    }
     if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,
                        seg_item_base);

@@ -127,7 +127,8 @@ struct SgSearchDev {
     int32_t dw_max;
     int32_t d_max;
     int32_t bmax;
-    int32_t chunk_words;
+    int32_t chunk_words;  // row words per counting chunk (128, 256 or 512)
+    int32_t chunk_range;  // chunks per counting unit
     int32_t count_grid;
     int32_t feat_on;     // compute the 24 features of newborn nodes (learned order or tracing)
 };

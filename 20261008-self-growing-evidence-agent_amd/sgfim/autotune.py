@@ -1,0 +1,184 @@
+"""Autotuner of the search's counting tile.
+
+The counting kernel stages the 16 child prefixes of one row chunk in LDS and streams the
+candidate rows through registers. Two knobs shape it and neither changes a result:
+  chunk_words  row words per chunk: words per lane = chunk_words / 64 (candidate words a
+               lane holds per step), LDS prefix size = 16 x 8 x chunk_words bytes (128 ->
+               16 KB and four workgroups per CU, 256 -> 32 KB and three, 512 -> 64 KB and one);
+  chunk_range  chunks per work unit: the unit's set-up (need masks, the sorted candidate
+               list -- the candidates a workgroup takes) is paid once per range, and a
+               range of every record is counted before the next (its rows stay in L2).
+tune() times a caller's search for every candidate shape on the loaded data, checks that
+every shape returns the same MFIs, keeps the fastest, and records it in a JSON cache keyed
+by device and input shape; apply() sets a cached shape. `python -m sgfim.autotune` tunes
+the benchmark workloads and writes tile_cache.json next to this file.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+CANDIDATES = ((128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (512, 1), (512, 2), (512, 4))
+DEFAULT = (256, 4)
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_cache.json")
+
+
+def _bucket(x: int) -> int:
+    """Next power of two (shapes within a factor of two share an entry)."""
+    return 1 << max(0, math.ceil(math.log2(max(1, int(x)))))
+
+
+def shape_key(device: str, n_cu: int, n_items: int, words: int, n_sets: int) -> str:
+    return f"{device}:{n_cu}cu|items:{_bucket(n_items)}|words:{_bucket(words)}|sets:{_bucket(n_sets)}"
+
+
+def context_key(ctx, n_sets: int = 1) -> str:
+    dev, ncu = ctx.device_info()
+    return shape_key(dev, ncu, ctx.n_items, ctx.words, n_sets)
+
+
+def load_cache(path: str = CACHE) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def save_entry(key: str, entry: dict, path: str = CACHE):
+    cache = load_cache(path)
+    cache[key] = entry
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+        f.write("\n")
+    os.replace(tmp, path)
+
+
+def lookup(key: str, path: str = CACHE):
+    """(chunk_words, chunk_range) cached for key, or None."""
+    e = load_cache(path).get(key)
+    return (int(e["chunk_words"]), int(e["chunk_range"])) if e else None
+
+
+def apply(ctx, n_sets: int = 1, path: str = CACHE):
+    """Set the cached shape for this context's device and data; returns it (or the default)."""
+    t = lookup(context_key(ctx, n_sets), path)
+    ctx.set_tile(*(t or (0, 0)))
+    return t or DEFAULT
+
+
+def choose(timings: dict):
+    """The fastest shape; ties go to the earlier candidate."""
+    best = None
+    for shape, ms in timings.items():
+        if best is None or ms < timings[best]:
+            best = shape
+    return best
+
+
+def _mix(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser on uint64 (wrapping arithmetic)."""
+    z = x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def signature(res) -> tuple:
+    """Order-free digest of a result: the number of MFIs and a wrapping sum over MFIs of a
+    hash of (set of items, support, set index)."""
+    if res.arrays:
+        items, off, sup = (np.asarray(a) for a in res.arrays)
+    else:
+        lens = [len(x) for x in res.itemsets]
+        items = np.asarray([i for x in res.itemsets for i in x], dtype=np.int64)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        sup = np.asarray(res.supports, dtype=np.int64)
+    n = len(sup)
+    if n == 0:
+        return (0, 0)
+    win = np.asarray(res.windows, dtype=np.int64) if len(res.windows) else np.zeros(n, dtype=np.int64)
+    with np.errstate(over="ignore"):
+        per_item = _mix(items.astype(np.int64))
+        csum = np.concatenate([[np.uint64(0)], np.cumsum(per_item, dtype=np.uint64)])
+        set_h = csum[off[1:]] - csum[off[:-1]]
+        h = _mix(set_h ^ _mix(sup.astype(np.uint64) * np.uint64(0x100000001B3) + win.astype(np.uint64)))
+        return (n, int(np.sum(h, dtype=np.uint64)))
+
+
+def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None, path: str | None = CACHE,
+         verify: bool = True, log=None) -> dict:
+    """Time run() (a search on ctx returning a MineResult) under every candidate shape and
+    keep the fastest (best of reps wall times after one warm-up run, which also captures
+    the round graphs). With verify, every shape must give the same MFIs. The chosen shape
+    is left set on ctx and, when key and path are given, stored in the cache."""
+    timings, ref = {}, None
+    for cw, cr in candidates:
+        ctx.set_tile(cw, cr)
+        r = run()
+        if verify:
+            sig = signature(r)
+            if ref is None:
+                ref = sig
+            elif sig != ref:
+                raise RuntimeError(f"tile {cw}x{cr} changed the result")
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+        timings[(cw, cr)] = 1e3 * min(ts)
+        if log:
+            log(f"tile chunk_words={cw} chunk_range={cr}: {timings[(cw, cr)]:.3f} ms")
+    best = choose(timings)
+    ctx.set_tile(*best)
+    entry = {"chunk_words": best[0], "chunk_range": best[1], "ms": round(timings[best], 4),
+             "default_ms": round(timings.get(DEFAULT, float("nan")), 4),
+             "all": {f"{cw}x{cr}": round(ms, 4) for (cw, cr), ms in timings.items()}}
+    if key and path:
+        save_entry(key, entry, path)
+    return entry
+
+
+def main(argv=None):
+    """Tune the benchmark's workload 2 (three supports in one search) and workload 3 (512
+    windows) on device 0 and write the cache."""
+    import argparse
+    from . import gen
+    from .core import Context
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workloads", default="2,3")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--cache", default=CACHE)
+    a = ap.parse_args(argv)
+    say = lambda m: print(m, flush=True)
+    for w in a.workloads.split(","):
+        ctx = Context(device=0)
+        if w == "2":
+            T = gen.workload2(0)
+            ctx.load_csr(T.offsets, T.items)
+            thrs = [gen.min_count(s, T.n_txn) for s in (0.10, 0.05, 0.02)]
+            run = lambda: ctx.mine_thresholds(thrs, convert=False)
+            n_sets = len(thrs)
+        elif w == "3":
+            win_off, off, items, lens = gen.workload3_csr(0)
+            ctx.load_windows(win_off, off, items)
+            thrs = [gen.min_count(0.03, int(n)) for n in lens]
+            run = lambda: ctx.mine_windows(thrs, convert=False)
+            n_sets = len(lens)
+        else:
+            raise SystemExit(f"unknown workload {w}")
+        key = context_key(ctx, n_sets)
+        say(f"workload {w}: {key}")
+        e = tune(ctx, run, reps=a.reps, key=key, path=a.cache, log=say)
+        say(f"workload {w}: chose {e['chunk_words']}x{e['chunk_range']} ({e['ms']} ms; default {e['default_ms']} ms)")
+        ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

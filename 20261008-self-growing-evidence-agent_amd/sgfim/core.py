@@ -304,6 +304,23 @@ class Context:
         self._check(self._L.sgfim_pair_time(self._h, C.byref(ms)), "pair_time")
         return ms.value
 
+    def set_tile(self, chunk_words: int = 0, chunk_range: int = 0):
+        """Counting-tile shape of later searches (0 = default); results do not depend on it."""
+        self._check(self._L.sgfim_set_tile(self._h, int(chunk_words), int(chunk_range)), "set_tile")
+
+    def tile(self) -> tuple:
+        """(chunk_words, chunk_range) the next search uses."""
+        cw, cr = C.c_int32(), C.c_int32()
+        self._check(self._L.sgfim_get_tile(self._h, C.byref(cw), C.byref(cr)), "get_tile")
+        return cw.value, cr.value
+
+    def device_info(self) -> tuple:
+        """(gcnArchName, compute units) of the context's GPU."""
+        buf = C.create_string_buffer(64)
+        n = C.c_int32()
+        self._check(self._L.sgfim_device_info(self._h, buf, 64, C.byref(n)), "device_info")
+        return buf.value.decode(), n.value
+
     def copy_probe(self, nbytes: int = 1 << 31, iters: int = 5) -> float:
         """On-device copy bandwidth in GB/s (bytes read + written)."""
         g = C.c_double()

@@ -116,6 +116,33 @@ def workload3(seed: int = 0, n_windows: int = 512, lo: int = 2_000, hi: int = 20
     return [sensor_stream(seed * 100_003 + w + 1, int(n), 32, 6, 4) for w, n in enumerate(lens)]
 
 
+def _workload3_window(args):
+    seed, w, n = args
+    T = sensor_stream(seed * 100_003 + w + 1, int(n), 32, 6, 4)
+    return T.offsets, T.items
+
+
+def workload3_csr(seed: int = 0, n_windows: int = 512, workers: int = 16):
+    """Workload 3 as one CSR: (win_off, txn_off, items, lengths), windows generated in a
+    process pool (window w as in workload3)."""
+    from concurrent.futures import ProcessPoolExecutor
+    lens = window_lengths(seed, n_windows)
+    args = [(seed, w, n) for w, n in enumerate(lens)]
+    if workers > 1:
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            parts = list(ex.map(_workload3_window, args))
+    else:
+        parts = [_workload3_window(a) for a in args]
+    win_off = np.zeros(len(parts) + 1, dtype=np.int64)
+    offs, its, base = [np.zeros(1, dtype=np.int64)], [], 0
+    for w, (o, it) in enumerate(parts):
+        win_off[w + 1] = win_off[w] + o.shape[0] - 1
+        offs.append(o[1:] + base)
+        its.append(it)
+        base += it.shape[0]
+    return win_off, np.concatenate(offs), np.concatenate(its), lens
+
+
 def workload4(seed: int, n_txn: int = 500_000) -> Transactions:
     """500,000 txns, 256 items shaped like workload 1 (3 regimes, N(0,1) means, phi 0.9,
     stay 0.98): 64 sensors x 4 bins. (32 x 8 bins would also give 256 items, but at 2%

@@ -58,35 +58,17 @@ def cpu_baseline(T, budget_s: float = 20.0):
                       f"(count-all reference search, {len(out)} MFIs)"}
 
 
-def _gen_w3(args):
-    from sgfim import gen
-    seed, w, n = args
-    T = gen.sensor_stream(seed * 100_003 + w + 1, int(n), 32, 6, 4)
-    return T.offsets, T.items
-
-
 def workload3_run(Context, seed=0, workers=16):
     """512 windows, lengths log-uniform in [2,000, 200,000], 32 sensors x 6 bins, 3% per window:
     one search over all windows (every counting launch mixes windows)."""
-    import numpy as np
-    from concurrent.futures import ProcessPoolExecutor
     from sgfim import gen
-    lens = gen.window_lengths(seed, 512)
     t0 = time.perf_counter()
-    with ProcessPoolExecutor(max_workers=workers) as ex:
-        parts = list(ex.map(_gen_w3, [(seed, w, n) for w, n in enumerate(lens)]))
-    win_off = np.zeros(len(parts) + 1, dtype=np.int64)
-    offs, its, base = [np.zeros(1, dtype=np.int64)], [], 0
-    for w, (o, it) in enumerate(parts):
-        win_off[w + 1] = win_off[w] + o.shape[0] - 1
-        offs.append(o[1:] + base)
-        its.append(it)
-        base += it.shape[0]
-    off = np.concatenate(offs)
-    items = np.concatenate(its)
+    win_off, off, items, lens = gen.workload3_csr(seed, 512, workers)
     log(f"workload 3 generated: {len(lens)} windows, {int(win_off[-1])} txns in {time.perf_counter() - t0:.1f} s")
     ctx = Context(device=0)
     ctx.load_windows(win_off, off, items)
+    from sgfim import autotune
+    tile = autotune.apply(ctx, len(lens))
     thr = [gen.min_count(0.03, int(n)) for n in lens]
     ctx.mine_windows(thr, convert=False)                   # warm
     t0 = time.perf_counter()
@@ -98,7 +80,8 @@ def workload3_run(Context, seed=0, workers=16):
     return {"workload": "workload 3: 512 windows, 2k-200k txns, 32 sensors x 6 bins, 3% per window",
             "seconds": dt, "mfis": len(r), "counts": counts, "counts_per_s": counts / dt,
             "count_kernel_ms": st["count_ms"], "launches": st["count_launches"],
-            "alg_gbs": 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None}
+            "alg_gbs": 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None,
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1]}}
 
 
 def workload5_run(Context, budget_s=8.0, seed=0):
@@ -193,6 +176,11 @@ def main():
     ctx.load_csr(T.offsets, T.items)
     log(f"loaded + transposed on GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words} words)")
     thrs = [gen.min_count(s, T.n_txn) for s in SUPPORTS]
+    from sgfim import autotune
+    tile_key = autotune.context_key(ctx, 1 if args.sequential else len(thrs))
+    tile = autotune.apply(ctx, 1 if args.sequential else len(thrs))
+    tile_src = "tile_cache.json" if autotune.lookup(tile_key) else "default"
+    log(f"counting tile {tile[0]} words x {tile[1]} chunks ({tile_src}, {tile_key})")
 
     pair_ms = []
 
@@ -270,7 +258,8 @@ def main():
         "config": {"workload": f"workload 2: {T.n_txn} txns, 64 sensors x 8 bins = {T.n_items} items, "
                                f"min supports 10%/5%/2%", "mfis_per_support": mfis,
                    "search": "three searches, one per support" if args.sequential
-                             else "one batched search over the three supports"},
+                             else "one batched search over the three supports",
+                   "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "source": tile_src}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": 1024.0 * (PMC_COUNT_KERNEL["fetch_scale"] * PMC_COUNT_KERNEL["fetch_kib"] +

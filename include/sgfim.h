@@ -88,9 +88,13 @@ typedef struct sgfim_round_stats {
     int64_t words;           /* algorithmic 64-bit words of the counting kernel so far:
                                 (needed counts + 1) * W per child, W = the child's window
                                 words; a needed count is a candidate f of U \ {e} whose
-                                pair (e, f) is frequent                                    */
+                                pair (e, f) is frequent and that is not a later child of
+                                the same record (that support is counted once, in the
+                                later child's row)                                        */
     int64_t pair_counts;     /* counts taken from the pair matrix (roots x items)          */
-    int64_t count_launches;  /* launches of the counting kernel                            */
+    int64_t count_launches;  /* launches of the counting kernel (one per round queued,
+                                including a last one queued after the frontier emptied,
+                                which returns at once)                                     */
     double count_ms;         /* summed duration of the counting kernel, from the device's
                                 constant-rate wall clock (first workgroup start to last end) */
     double event_ms;         /* the same duration from HIP events around the kernel; only
@@ -154,6 +158,19 @@ int sgfim_load_sensors(sgfim_ctx* ctx, const double* x, int64_t n_rows, int32_t 
  * window <= 64. */
 int sgfim_load_sensors_online(sgfim_ctx* ctx, const double* x, int64_t n_rows, int32_t n_sensors, int32_t n_bins,
                               int32_t window, int32_t stride, double* edges);
+
+/* Counting-tile shape of the search: chunk_words row words per chunk (the 16 child
+ * prefixes of one chunk sit in LDS: 128, 256 or 512; 0 = the default 256) and chunk_range
+ * chunks per work unit (1..64; 0 = the default 4). Results do not depend on it; it trades
+ * LDS and registers (occupancy) against set-up work per unit. sgfim/autotune.py picks it
+ * per device and input shape. No counterpart in the reference, whose engines take such
+ * knobs from environment variables read at start-up (kernel_bmma.cu:351-368). */
+int sgfim_set_tile(sgfim_ctx* ctx, int32_t chunk_words, int32_t chunk_range);
+int sgfim_get_tile(const sgfim_ctx* ctx, int32_t* chunk_words, int32_t* chunk_range);
+/* The context's device: its gcnArchName (NUL-terminated, at most len - 1 characters) and
+ * compute units; the key of the tile autotuner's cache. No counterpart in the reference
+ * (its Results header prints the device, kernel_bitmap.cu:608-690). */
+int sgfim_device_info(const sgfim_ctx* ctx, char* name, int32_t len, int32_t* n_cu);
 
 /* Items [windows][n_sensors] of the latest sensor load (either variant): *n receives their
  * number, and when items is not NULL the first min(cap, *n) are copied to it. */
