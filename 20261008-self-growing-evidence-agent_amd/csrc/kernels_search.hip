@@ -558,7 +558,6 @@ __device__ __forceinline__ uint32_t sg_wave_sum4(uint32_t v0, uint32_t v1, uint3
 }
 
 #define SG_TILE_THREADS 512
-#define SG_SORT_MAX 1024   // longest candidate list that a unit sorts
 
 // Popcounts of x & c over the PER words a lane holds (all PER, or the first qn).
 template <int PER, bool FULL>
@@ -654,6 +653,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     constexpr int PER = CW / 64;
     constexpr int NW = SG_TILE_THREADS / 64;
     static_assert(SG_RC * SG_CB == 64, "one lane per (child, candidate) pair of a wave step");
+    __shared__ uint64_t pre[CW];             // T(X) over the chunk
     __shared__ uint64_t cp[SG_CB][CW];       // child prefixes; the children's ok rows during set-up
     __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
     __shared__ int64_t s_unit;
@@ -716,33 +716,16 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         // a candidate that is itself a child (i < ntail) is counted only below the
         // diagonal: |T(X u {e_a, e_b})| is one support, counted once in cell (a, b),
         // a > b, and read from there for cell (b, a).
-        // room for the unsorted list past the ok rows; a list that may not fit goes
-        // straight to s_cand and keeps its order
-        constexpr int LST_CAP = 2 * SG_CB * (CW - 64);
-        const bool sort = nu <= LST_CAP && nu <= SG_SORT_MAX;
-        uint32_t* lst = sort ? reinterpret_cast<uint32_t*>(okr + SG_CB * 64) : s_cand;
         for (int i = threadIdx.x; i < nu; i += blockDim.x) {
             const int32_t f = S.clist[u0 + i];
             const int jlo = i < m.ntail ? i + 1 - j0 : 0;   // children j with j0 + j > i
             uint32_t mask = 0u;
             for (int j = max(jlo, 0); j < ncb; j++)
                 if ((okr[j * 64 + (f >> 6)] >> (f & 63)) & 1ull) mask |= 1u << j;
-            if (mask) lst[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
+            if (mask) s_cand[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
         }
         __syncthreads();
         const int nc = s_nc;
-        // listed in descending order of (children mask, position): candidates that the
-        // same children need share wave steps, so fewer child steps hold unneeded lanes
-        // (a rank sort; long lists keep their order)
-        if (sort) {
-            for (int a = threadIdx.x; a < nc; a += blockDim.x) {
-                const uint32_t v = lst[a];
-                int r = 0;
-                for (int b = 0; b < nc; b++) r += lst[b] > v;
-                s_cand[r] = v;
-            }
-            __syncthreads();
-        }
         const uint64_t* seg_rows = rows + sg.row_base;
         int32_t* cells = S.cells + S.celloff[g];
         const int32_t* clist = S.clist + u0;
@@ -758,10 +741,11 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
             const int qn = (wn + 63) >> 6;   // 64-word columns holding row words
             __syncthreads();   // every wave is done with the previous chunk's prefixes
             // child prefixes T(X) & T(e) (zero past the last word) and the children's
-            // supports: each wave ANDs the rows of X at its lanes' words itself, so the
-            // rows of X and of its children arrive in one round trip
+            // supports. Each wave loads its children's rows and, for its slice of the
+            // chunk's words, the rows of X, all in one round trip; T(X) over the slices
+            // meets in LDS.
             {
-                uint64_t pv[PER], rv[2][PER];
+                uint64_t rv[2][PER];
 #pragma unroll
                 for (int jj = 0; jj < 2; jj++) {
                     const int j = wave + NW * jj;
@@ -772,16 +756,15 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
                         rv[jj][q] = (j < ncb && w < wn) ? row[w] : 0ull;
                     }
                 }
-#pragma unroll
-                for (int q = 0; q < PER; q++) pv[q] = lane + 64 * q < wn ? ~0ull : 0ull;
-                for (int i = 0; i < m.k; i++) {
-                    const uint64_t* row = seg_rows + (int64_t)s_items[i] * sg.stride + w0;
-#pragma unroll
-                    for (int q = 0; q < PER; q++) {
-                        const int w = lane + 64 * q;
-                        if (w < wn) pv[q] &= row[w];
-                    }
+                constexpr int SW = CW / NW;   // words of a wave's slice
+                if (lane < SW) {
+                    const int w = wave * SW + lane;
+                    uint64_t v = w < wn ? ~0ull : 0ull;
+                    for (int i = 0; i < m.k; i++)
+                        if (w < wn) v &= seg_rows[(int64_t)s_items[i] * sg.stride + w0 + w];
+                    pre[w] = v;
                 }
+                __syncthreads();
 #pragma unroll
                 for (int jj = 0; jj < 2; jj++) {
                     const int j = wave + NW * jj;
@@ -789,7 +772,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
                         uint32_t acc = 0;
 #pragma unroll
                         for (int q = 0; q < PER; q++) {
-                            const uint64_t v = pv[q] & rv[jj][q];
+                            const uint64_t v = pre[lane + 64 * q] & rv[jj][q];
                             cp[j][lane + 64 * q] = v;
                             acc += (uint32_t)__popcll(v);
                         }
