@@ -1328,6 +1328,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         fill_stats(c, S);
         return SGFIM_OK;
     }
+    if (a->flags & SGFIM_FRESH_PAIR) c->pair_valid = false;
     if ((rc = ensure_pair(c))) return rc;
     S->pair_counts = n_roots * d;
     // search-segment table (thresholds, pair-ok blocks) + pair_ok masks

@@ -19,6 +19,7 @@ ROOT_STRICT = 0x1
 ROOT_TOPK = 0x2
 ORDER_LEARNED = 0x4
 NO_GRAPH = 0x8
+FRESH_PAIR = 0x10
 
 
 class SgfimError(RuntimeError):

@@ -54,6 +54,9 @@ enum sgfim_status {
                                     of by support; never changes the result */
 #define SGFIM_NO_GRAPH     0x8u  /* launch each round's kernels directly instead of replaying
                                     the round's captured hipGraph */
+#define SGFIM_FRESH_PAIR   0x10u /* recompute the pair matrix that seeds the search (on the
+                                    device, in the run's stream order) instead of reusing the
+                                    one kept from an earlier run; sgfim_pair_time times it */
 
 typedef struct sgfim_ctx sgfim_ctx;
 typedef struct sgfim_result sgfim_result;

@@ -205,3 +205,22 @@ def test_empty_and_degenerate_inputs(ctx):
     ctx.load_csr(np.asarray([0, 1, 2], dtype=np.int64), np.asarray([4, 4], dtype=np.int32))
     assert ctx.mine(1).as_dict() == {frozenset({4}): 2}
     assert ctx.mine(3).as_dict() == {}
+
+
+def test_fresh_pair_flag_recomputes_in_stream_order(ctx):
+    say("case: FRESH_PAIR, workload1 seed 7 n=30000")
+    from sgfim._lib import FRESH_PAIR
+    T = gen.workload1(7, 30_000)
+    ctx.load_csr(T.offsets, T.items)
+    thr = gen.min_count(0.02, T.n_txn)
+    ref = _oracle_mfis(T, thr)
+    assert ctx.mine(thr).as_dict() == ref
+    assert ctx.pair_time() > 0.0                   # the first run computed the pair matrix
+    for _ in range(2):
+        assert ctx.mine(thr, flags=FRESH_PAIR).as_dict() == ref
+        assert ctx.pair_time() > 0.0
+    res = ctx.mine_thresholds([thr, 2 * thr], flags=FRESH_PAIR)
+    got = [dict(), dict()]
+    for x, s, w in zip(res.itemsets, res.supports, res.windows):
+        got[w][frozenset(x)] = s
+    assert got[0] == ref and got[1] == _oracle_mfis(T, 2 * thr)
