@@ -36,10 +36,14 @@ __device__ __forceinline__ int sg_bucket_support(int32_t sup, int32_t thr)
     return b < 0 ? 0 : (b > SG_NBUCKETS - 1 ? SG_NBUCKETS - 1 : b);
 }
 
-__device__ __forceinline__ int sg_bucket_score(float s)
+// Learned order: the network's score splits the frontier in two (score >= 0.5 first), and
+// each half keeps the support order at half resolution. Siblings stay together within a
+// half, so their shared candidate rows stay in L2; a fully score-ordered frontier mixes
+// distant records in every round and loses more to cache misses than it saves in counts.
+__device__ __forceinline__ int sg_bucket_score(float s, int32_t sup, int32_t thr)
 {
-    int b = (int)(s * (float)SG_NBUCKETS);
-    return b < 0 ? 0 : (b > SG_NBUCKETS - 1 ? SG_NBUCKETS - 1 : b);
+    const int level = s >= 0.5f ? 1 : 0;
+    return level * (SG_NBUCKETS / 2) + (sg_bucket_support(sup, thr) >> 1);
 }
 
 // ------------------------------------------------------------------ init
@@ -945,7 +949,7 @@ __global__ void k_commit(SgSearchDev S, const SgSegment* __restrict__ segs, int3
         if (order_mode == 1) {
             const float sc = scores[i];
             S.meta[slot].score = sc;
-            b = sg_bucket_score(sc);
+            b = sg_bucket_score(sc, m.sup, segs[m.seg].thr);
         } else if (order_mode == 2) {
             b = min(m.k, SG_NBUCKETS - 1);
         } else {

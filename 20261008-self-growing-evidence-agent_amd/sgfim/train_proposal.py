@@ -18,6 +18,7 @@
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -52,19 +53,26 @@ def anytime_curve(ctx: Context, min_count: int, learned_order: bool):
 
     MFIs are taken as flat arrays each round (no per-itemset Python objects inside the
     timed loop); the final set is built once at the end for the identity check."""
-    ctx.anytime_begin(min_count, ORDER_LEARNED if learned_order else 0)
     rec = []
     parts = []
     n = 0
-    while True:
-        st = ctx.anytime_step(1)
-        r = ctx.anytime_take(convert=False)
-        if r.count:
-            parts.append(r.arrays)
-            n += r.count
-        rec.append((st["elapsed_s"], st["counts"], n))
-        if st["done"]:
-            break
+    # no cyclic garbage collection inside the timed loop: a full collection over the
+    # millions of objects an earlier result holds stalls a round by a quarter second
+    gc.collect()
+    gc.disable()
+    try:
+        ctx.anytime_begin(min_count, ORDER_LEARNED if learned_order else 0)
+        while True:
+            st = ctx.anytime_step(1)
+            r = ctx.anytime_take(convert=False)
+            if r.count:
+                parts.append(r.arrays)
+                n += r.count
+            rec.append((st["elapsed_s"], st["counts"], n))
+            if st["done"]:
+                break
+    finally:
+        gc.enable()
     found = {}
     for it, off, sup in parts:
         for i in range(len(sup)):
@@ -89,7 +97,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--batch-nodes", type=int, default=64)
+    ap.add_argument("--batch-nodes", type=int, default=1024)
     ap.add_argument("--label", choices=("yield", "direct", "ancestor"), default="yield",
                     help="yield: fraction of the record's children its round confirms as MFIs; "
                          "direct: the record's own round confirms an MFI; ancestor: also its ancestors")
@@ -154,6 +162,7 @@ def evaluate(ctx: Context, args, report: dict):
             anytime_curve(ctx, thr, False)
             anytime_curve(ctx, thr, True)
             warm = False
+        rp = fp = rl = fl = None   # release the previous stream's results before timing
         rp, fp = anytime_curve(ctx, thr, False)
         rl, fl = anytime_curve(ctx, thr, True)
         tot = len(fp)

@@ -50,8 +50,9 @@ enum sgfim_status {
                                     subtree (kernel_bitmap.cu:816,841-850); needs opts.rel_thr */
 #define SGFIM_ROOT_TOPK    0x2u  /* AnyFIM stage rule: only the first root_topk items of the
                                     stable descending-frequency order root a subtree (kernel.cu:850-859) */
-#define SGFIM_ORDER_LEARNED 0x4u /* order the frontier by the proposal network's score instead
-                                    of by support; never changes the result */
+#define SGFIM_ORDER_LEARNED 0x4u /* order the frontier by the proposal network's score: records
+                                    scoring >= 0.5 first, support order within each half;
+                                    never changes the result */
 #define SGFIM_NO_GRAPH     0x8u  /* launch each round's kernels directly instead of replaying
                                     the round's captured hipGraph */
 #define SGFIM_FRESH_PAIR   0x10u /* recompute the pair matrix that seeds the search (on the
