@@ -113,6 +113,45 @@ def workload5_run(Context, budget_s=8.0, seed=0):
             "alg_gbs": alg, "alg_frac_of_hbm_peak": alg / HBM_PEAK_GBS if alg else None}
 
 
+def workload4_run(Context, n_seeds=4, batch_nodes=1024):
+    """Anytime coverage on workload 4 (500,000 txns x 256 items, 2% floor): time until 95%
+    of the final MFI set is confirmed, learned frontier order against the plain support
+    order, on the first held-out seeds of the proposal network's evaluation (the committed
+    weights)."""
+    from sgfim import learned
+    from sgfim.train_proposal import anytime_curve, streams, t_at
+    from sgfim import gen
+    ctx = Context(device=0, batch_nodes=batch_nodes)
+    ctx.set_scorer(*learned.load_weights())
+    seeds = list(range(1_000_000, 1_000_000 + n_seeds))
+    rows, warm = [], True
+    for s, (off, items) in streams(seeds, 500_000, min(16, len(seeds))):
+        ctx.load_csr(off, items)
+        thr = gen.min_count(0.02, ctx.n_txn)
+        if warm:   # warm the buffers and the captured rounds of both orders
+            anytime_curve(ctx, thr, False)
+            anytime_curve(ctx, thr, True)
+            warm = False
+        rp = fp = rl = fl = None
+        rp, fp = anytime_curve(ctx, thr, False)
+        rl, fl = anytime_curve(ctx, thr, True)
+        tot = len(fp)
+        rows.append({"seed": s, "mfis": tot, "identical": fp == fl,
+                     "t95_plain_s": t_at(rp, 0.95, tot, 0), "t95_learned_s": t_at(rl, 0.95, tot, 0),
+                     "counts95_plain": t_at(rp, 0.95, tot, 1), "counts95_learned": t_at(rl, 0.95, tot, 1)})
+        log(f"workload 4 seed {s}: t95 plain {rows[-1]['t95_plain_s']:.4f} s, learned {rows[-1]['t95_learned_s']:.4f} s")
+    ctx.close()
+    med = lambda v: float(np.median(v))
+    return {"workload": f"workload 4: 500,000 txns x 256 items, 2% floor, held-out seeds {seeds[0]}..{seeds[-1]}, "
+                        f"anytime rounds of up to {batch_nodes} records",
+            "metric": "time to 95% MFI coverage, learned / plain",
+            "median_t95_ratio": med([r["t95_learned_s"] / r["t95_plain_s"] for r in rows]),
+            "median_counts95_ratio": med([r["counts95_learned"] / max(r["counts95_plain"], 1) for r in rows]),
+            "median_t95_plain_s": med([r["t95_plain_s"] for r in rows]),
+            "median_t95_learned_s": med([r["t95_learned_s"] for r in rows]),
+            "all_identical": all(r["identical"] for r in rows), "seeds": rows}
+
+
 def cpu_extra(T2=None, budget_s=15.0):
     """Single-thread restatements of the reference's Python miners: the set-based miner
     (skab_spike.py:84-135) and the NumPy packed-bitmap miner (exp_e6_cpuvec.py:43-107) on
@@ -148,11 +187,12 @@ def main():
     ap.add_argument("--n-txn", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the workload 3 and workload 5 measurements")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the workload 3, 4 and 5 measurements")
     ap.add_argument("--w5-budget", type=float, default=8.0)
     ap.add_argument("--sequential", action="store_true",
                     help="mine the three supports as three searches instead of one batched search")
-    ap.add_argument("--extra-only", choices=("workload3", "workload5"), default=None,
+    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5"), default=None,
                     help="run one extra workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -164,7 +204,8 @@ def main():
     from sgfim.core import Context
 
     if args.extra_only:
-        res = workload5_run(Context, args.w5_budget) if args.extra_only == "workload5" else workload3_run(Context)
+        res = {"workload3": lambda: workload3_run(Context), "workload4": lambda: workload4_run(Context),
+               "workload5": lambda: workload5_run(Context, args.w5_budget)}[args.extra_only]()
         print(json.dumps(res))
         return
 
@@ -280,7 +321,8 @@ def main():
     ctx.close()
     if not args.no_extra:
         out["extra"] = {"workload5": workload5_run(Context, args.w5_budget),
-                        "workload3": workload3_run(Context)}
+                        "workload3": workload3_run(Context),
+                        "workload4": workload4_run(Context)}
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(T)
