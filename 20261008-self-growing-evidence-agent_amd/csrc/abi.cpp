@@ -447,6 +447,8 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.unit_off, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &D.choff, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.celloff, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &D.cells, (size_t)cell_cap)) ||
+        (rc = dev_alloc_t(c, V, &D.chrec, (size_t)newborn_cap)) ||
+        (rc = dev_alloc_t(c, V, &D.blkrec, (size_t)(newborn_cap / 16 + bmax + 1))) ||
         (rc = dev_alloc_t(c, V, &D.clist, (size_t)bmax * d)) || (rc = dev_alloc_t(c, V, &D.ccount, (size_t)bmax * d)) ||
         (rc = dev_alloc_t(c, V, &D.newborn, (size_t)newborn_cap)) ||
         (rc = dev_alloc_t(c, V, &D.feat, (size_t)newborn_cap * SG_FEAT)) ||

@@ -434,11 +434,13 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         }
     }
     int64_t su = 0, sc = 0, scell = 0, sunit = 0;
+    int32_t mr = 0;   // longest window (row-chunk ranges) among the taken records
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = tid * PER + q;
         if (nid[q] >= 0) {
             const SgNodeMeta m = S.meta[nid[q]];
+            mr = max(mr, m.nrange);
             su += m.ncand;
             sc += m.ntail;
             scell += (int64_t)m.ncand * m.ntail;
@@ -473,17 +475,17 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             eunit += nun;
         }
     }
+    // units = blocks x the longest window's ranges (taken over every taken record: a
+    // longer record past the cut only adds units that every workgroup skips)
+    if (mr) atomicMax(&s_maxr, mr);
     __syncthreads();
-    {
-        // longest window of the records that made the cut: units = blocks x its ranges
+    if (tid < SG_NBUCKETS) {   // release the taken records from their buckets
         const int cut = s_cut;
-        int32_t mr = 0;
-#pragma unroll
-        for (int q = 0; q < PER; q++)
-            if (tid * PER + q < cut) mr = max(mr, S.meta[nid[q]].nrange);
-        if (mr) atomicMax(&s_maxr, mr);
+        const int o = tid, b = SG_NBUCKETS - 1 - o;
+        const int32_t lo = o > 0 ? ends[o - 1] : 0, hi = ends[o];
+        const int32_t n = max(0, min(hi, cut) - lo);
+        if (n) S.ctl[CTL_BUCKET_TOP0 + b] -= n;
     }
-    __syncthreads();
     if (tid == 0) {
         const int cut = s_cut;
         if (cut == total) {   // every taken record fits: the end offsets are the totals
@@ -493,11 +495,6 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             S.unit_off[cut] = tot[3];
         }
         if (cut == 0 && total > 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
-        for (int o = 0; o < SG_NBUCKETS; o++) {
-            const int b = SG_NBUCKETS - 1 - o;
-            const int32_t lo = o > 0 ? ends[o - 1] : 0, hi = ends[o];
-            S.ctl[CTL_BUCKET_TOP0 + b] -= max(0, min(hi, cut) - lo);
-        }
         const int64_t nch = S.choff[cut];
         S.ctl[CTL_BATCH_N] = cut;
         S.ctl[CTL_CHILDREN] = nch;
@@ -510,7 +507,9 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
 
 // ------------------------------------------------------------------ U lists
 // One wave per record: expand U into an ascending id list; the children are its
-// first ntail entries (U is ascending and children are the items below `last`).
+// first ntail entries (U is ascending and children are the items below `last`). The
+// wave also maps its children and its child blocks back to the record (chrec, blkrec),
+// so the counting and expansion kernels find a unit's record with one load.
 // The grid also clears the batch's count cells.
 __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
@@ -537,7 +536,13 @@ __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment
         t &= t - 1ull;
         S.clist[pos++] = lane * 64 + bit;
     }
-    for (int j = lane; j < m.ntail; j += 64) S.ccount[S.choff[b] + j] = 0;
+    const int32_t c0 = S.choff[b];
+    for (int j = lane; j < m.ntail; j += 64) {
+        S.ccount[c0 + j] = 0;
+        S.chrec[c0 + j] = b;
+    }
+    const int64_t u0 = S.unit_off[b];
+    for (int i = lane; i < m.nblk; i += 64) S.blkrec[u0 + i] = b;
 }
 
 // ------------------------------------------------------------------ counting tiles
@@ -685,12 +690,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         // together, so the rows of one range stay in L2 while every record uses them
         const int cr = (int)(u / n_blocks);
         const int64_t bq = u - (int64_t)cr * n_blocks;
-        int lo = 0, hi = nb - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (S.unit_off[mid] <= bq) lo = mid; else hi = mid - 1;
-        }
-        const int g = lo;
+        const int g = S.blkrec[bq];
         const int32_t slot = S.batch[g];
         const SgNodeMeta m = S.meta[slot];
         if (cr >= m.nrange) {   // past this record's (shorter) window
@@ -843,12 +843,7 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
         sE[wv][lane] = 0ull;
         __syncthreads();
         if (q < nchild) {
-            int lo = 0, hi = nb - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (S.choff[mid] <= q) lo = mid; else hi = mid - 1;
-            }
-            g = lo;
+            g = S.chrec[q];
             const int j = (int)(q - S.choff[g]);
             const int32_t slot = S.batch[g];
             const SgNodeMeta m = S.meta[slot];

@@ -105,6 +105,8 @@ struct SgSearchDev {
     int32_t* choff;      // [bmax + 1] child offsets
     int64_t* celloff;    // [bmax + 1] count-matrix offsets
     int32_t* cells;      // [cell_cap] child x candidate counts
+    int32_t* chrec;      // [newborn_cap] batch record of each child of the round
+    int32_t* blkrec;     // [newborn_cap / SG_CB + bmax] batch record of each child block
     int64_t cell_cap;
     // children born in the current round
     int32_t* newborn;    // [newborn_cap]
