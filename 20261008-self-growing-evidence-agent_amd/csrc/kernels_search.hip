@@ -285,21 +285,25 @@ __device__ __forceinline__ int32_t sg_take_slot(const SgSearchDev& S, int64_t nb
 }
 
 // ------------------------------------------------------------------ seed (roots from the pair matrix)
-// One wave per root (seg, r): F = ok[r] \ {r}; equivalents from the pair-matrix row.
-__global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __restrict__ segs,
-                                              const int32_t* __restrict__ pair,
-                                              const uint64_t* __restrict__ ok,
-                                              const int32_t* __restrict__ roots, int32_t n_roots,
-                                              const int32_t* __restrict__ sup_all,
-                                              const int64_t* __restrict__ seg_item_base)
+// One wave per root (seg, r): F = ok[r] \ {r}; equivalents from the pair-matrix row. MFI
+// ring positions, record slots and keys are reserved once per workgroup of 16 roots.
+#define SG_SEED_WAVES 16
+__global__ __launch_bounds__(SG_SEED_WAVES * 64) void k_seed(SgSearchDev S, const SgSegment* __restrict__ segs,
+                                                             const int32_t* __restrict__ pair,
+                                                             const uint64_t* __restrict__ ok,
+                                                             const int32_t* __restrict__ roots, int32_t n_roots,
+                                                             const int32_t* __restrict__ sup_all,
+                                                             const int64_t* __restrict__ seg_item_base)
 {
-    __shared__ uint64_t sF[4][64];
-    __shared__ uint64_t sE[4][64];
-    __shared__ uint16_t sX[4][SG_MAX_DEPTH];
+    __shared__ uint64_t sF[SG_SEED_WAVES][64];
+    __shared__ uint64_t sE[SG_SEED_WAVES][64];
+    __shared__ uint16_t sX[SG_SEED_WAVES][SG_MAX_DEPTH];
+    __shared__ int32_t s_act[SG_SEED_WAVES];
+    __shared__ int64_t s_ring, s_nb, s_top, s_key;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t q = (int64_t)blockIdx.x * 4 + wv;
+    const int64_t q = (int64_t)blockIdx.x * SG_SEED_WAVES + wv;
     const bool live = q < n_roots;
-    int seg = 0, r = 0;
+    int seg = 0, r = 0, act = SG_ACT_NONE, k = 1;
     sE[wv][lane] = 0ull;
     __syncthreads();
     if (live) {
@@ -320,32 +324,40 @@ __global__ __launch_bounds__(256) void k_seed(SgSearchDev S, const SgSegment* __
             if (e != r && prow[e] == sr) atomicOr((unsigned long long*)&sE[wv][e >> 6], 1ull << (e & 63));
     }
     __syncthreads();
-    if (!live) return;
+    if (live) act = sg_node_decide(S, segs[seg], sF[wv], sE[wv], sX[wv], k, r);   // k grows by merged items
+    if (lane == 0) s_act[wv] = act;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ne = 0, nr = 0, nl = 0;
+        for (int w = 0; w < SG_SEED_WAVES; w++) {
+            ne += s_act[w] == SG_ACT_EMIT;
+            nr += s_act[w] == SG_ACT_RECORD;
+            nl += blockIdx.x * SG_SEED_WAVES + w < n_roots;
+        }
+        atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], (unsigned long long)nl);
+        if (ne) s_ring = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], (unsigned long long)ne);
+        if (nr) {
+            s_nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], (unsigned long long)nr);
+            s_top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-(int64_t)nr));
+            s_key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], (unsigned long long)nr);
+        }
+    }
+    __syncthreads();
+    if (act == SG_ACT_NONE) return;   // wave-uniform
+    int rank = 0;
+    for (int w = 0; w < wv; w++) rank += s_act[w] == act;
     const SgSegment sg = segs[seg];
     const int32_t* isup = sup_all + seg_item_base[seg];
-    int k = 1;
-    const int act = sg_node_decide(S, sg, sF[wv], sE[wv], sX[wv], k, r);
-    if (lane == 0) atomicAdd((unsigned long long*)&S.ctl[CTL_NODES], 1ull);
     if (act == SG_ACT_EMIT) {
-        if (lane == 0) {
-            const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], 1ull);
-            sg_emit_at(S, pos, sX[wv], k, isup[r], seg, -1);
-        }
-    } else if (act == SG_ACT_RECORD) {
-        int64_t nb = 0, key = 0;
+        if (lane == 0) sg_emit_at(S, s_ring + rank, sX[wv], k, isup[r], seg, -1);
+    } else {
+        const int64_t nb = s_nb + rank;
         int32_t slot = -1;
-        if (lane == 0) {
-            nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], 1ull);
-            const int64_t top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-1ll));
-            slot = sg_take_slot(S, nb, top);
-            if (slot >= 0) key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], 1ull);
-        }
-        nb = __shfl(nb, 0, 64);
-        key = __shfl(key, 0, 64);
+        if (lane == 0) slot = sg_take_slot(S, nb, s_top - rank);
         slot = __shfl(slot, 0, 64);
         if (slot >= 0) {
             float* fo = S.feat_on ? S.feat + nb * SG_FEAT : nullptr;
-            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, key, -1);
+            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, s_key + rank, -1);
             if (lane == 0) S.newborn[nb] = slot;
         }
     }
@@ -1013,7 +1025,8 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 {
     if (n_roots <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_newborn, dim3(1), dim3(1), 0, s, S);
-    hipLaunchKernelGGL(k_seed, dim3((unsigned)((n_roots + 3) / 4)), dim3(256), 0, s, S, segs, pair, ok, roots, n_roots,
+    hipLaunchKernelGGL(k_seed, dim3((unsigned)((n_roots + SG_SEED_WAVES - 1) / SG_SEED_WAVES)), dim3(SG_SEED_WAVES * 64), 0,
+                       s, S, segs, pair, ok, roots, n_roots,
                        sup_all, seg_item_base);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
