@@ -29,7 +29,7 @@ INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
 # passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
 # what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
 # profiles/k_count_group_workload2_pmc.md holds the commands and the table.
-PMC_COUNT_KERNEL = {"fetch_kib": 116158.7, "write_kib": 61608.3, "fetch_scale": 2.0, "write_scale": 1.0}
+PMC_COUNT_KERNEL = {"fetch_kib": 117841.4, "write_kib": 61441.7, "fetch_scale": 2.0, "write_scale": 1.0}
 SUPPORTS = (0.10, 0.05, 0.02)
 
 
