@@ -1,6 +1,7 @@
-"""GPU parity at the benchmark's full sizes: workload 2 (1,000,000 txns x 512 items at 10%,
-5% and 2%, one batched search) and workload 3 (512 windows, 24.7M txns, 3% per window,
-one search), each against the C oracle's exact MFI families."""
+"""GPU parity at the benchmark's full sizes against the C oracle's exact MFI families:
+workload 2 (1,000,000 txns x 512 items at 10%, 5% and 2%, one batched search), workload 3
+(512 windows, 24.7M txns, 3% per window, one search) and workload 4 (500,000 txns x 256
+items at 2%, plain and learned frontier order)."""
 import numpy as np
 import pytest
 
@@ -62,3 +63,20 @@ def test_workload3_full_size_matches_oracle():
         assert got[w] == ref, w
         total += len(ref)
     assert total == 2_702_719
+
+
+def test_workload4_full_size_both_orders_match_oracle():
+    print("case: workload 4 full size (500,000 txns x 256 items, seed 1000004) at 2%, plain and learned order",
+          flush=True)
+    from sgfim import learned
+    from sgfim._lib import ORDER_LEARNED
+    T = gen.workload4(1_000_004)
+    thr = gen.min_count(0.02, T.n_txn)
+    ref = _oracle(T.offsets, T.items, T.n_items, thr)
+    assert len(ref) == 544_863
+    with Context(device=0, batch_nodes=1024) as ctx:
+        ctx.load_csr(T.offsets, T.items)
+        ctx.set_scorer(*learned.load_weights())
+        for flags in (0, ORDER_LEARNED):
+            got = _by_set(ctx.mine(thr, flags=flags, convert=False), 1)[0]
+            assert got == ref, flags
