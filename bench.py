@@ -121,12 +121,15 @@ def workload4_run(Context, n_seeds=4, batch_nodes=1024):
     from sgfim import learned
     from sgfim.train_proposal import anytime_curve, streams, t_at
     from sgfim import gen
+    from sgfim import autotune
     ctx = Context(device=0, batch_nodes=batch_nodes)
     ctx.set_scorer(*learned.load_weights())
+    tile = None
     seeds = list(range(1_000_000, 1_000_000 + n_seeds))
     rows, warm = [], True
     for s, (off, items) in streams(seeds, 500_000, min(16, len(seeds))):
         ctx.load_csr(off, items)
+        tile = tile or autotune.apply(ctx, 1)
         thr = gen.min_count(0.02, ctx.n_txn)
         if warm:   # warm the buffers and the captured rounds of both orders
             anytime_curve(ctx, thr, False)
@@ -149,7 +152,8 @@ def workload4_run(Context, n_seeds=4, batch_nodes=1024):
             "median_counts95_ratio": med([r["counts95_learned"] / max(r["counts95_plain"], 1) for r in rows]),
             "median_t95_plain_s": med([r["t95_plain_s"] for r in rows]),
             "median_t95_learned_s": med([r["t95_learned_s"] for r in rows]),
-            "all_identical": all(r["identical"] for r in rows), "seeds": rows}
+            "all_identical": all(r["identical"] for r in rows),
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1]}, "seeds": rows}
 
 
 def cpu_extra(T2=None, budget_s=15.0):
