@@ -84,29 +84,34 @@ def workload3_run(Context, seed=0, workers=16):
             "tile": {"chunk_words": tile[0], "chunk_range": tile[1]}}
 
 
-def workload5_run(Context, budget_s=8.0, seed=0):
+def workload5_run(Context, budget_s=8.0, seed=0, learned_order=False):
     """10M txns x 2,048 Zipf items (2.56 GB of bitmaps, 10x the MALL): a budgeted anytime run
-    down to the 0.5% floor; reports the counting kernel against the HBM roofline."""
-    from sgfim import gen
+    down to the 0.5% floor; reports the counting kernel against the HBM roofline. With
+    learned_order the proposal network orders the frontier."""
+    from sgfim import gen, learned
+    from sgfim._lib import ORDER_LEARNED
     ctx = Context(device=0, frontier_cap=1 << 23)   # 8M frontier records (~5 GB at 2,048 items)
     t0 = time.perf_counter()
     ctx.load_zipf(seed=seed, **gen.WORKLOAD5)
     log(f"workload 5 generated on the GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words})")
+    if learned_order:
+        ctx.set_scorer(*learned.load_weights())
     thr = gen.min_count(0.005, ctx.n_txn)
     t0 = time.perf_counter()
-    ctx.anytime_begin(thr)
+    ctx.anytime_begin(thr, ORDER_LEARNED if learned_order else 0)
     mfis = 0
+    tag = "learned" if learned_order else "plain"
     while True:
         st = ctx.anytime_step(1)
-        mfis += len(ctx.anytime_take())
-        log(f"workload 5 round {st['round']}: {st['counts']} counts, {mfis} MFIs, frontier {st['frontier']}")
+        mfis += len(ctx.anytime_take(convert=False))
+        log(f"workload 5 ({tag}) round {st['round']}: {st['counts']} counts, {mfis} MFIs, frontier {st['frontier']}")
         if st["done"] or time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
     ctx.close()
     alg = 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None
     return {"workload": "workload 5: 10M txns x 2048 Zipf(1.1) items, mean length 60, 40 planted sets, "
-                        "floor 0.5%, budgeted anytime run",
+                        f"floor 0.5%, budgeted anytime run, {tag} frontier order",
             "seconds": dt, "rounds": st["round"], "done": bool(st["done"]), "mfis_confirmed": mfis,
             "counts": st["counts"] + st["pair_counts"], "counts_per_s": (st["counts"] + st["pair_counts"]) / dt,
             "count_kernel_ms": st["count_ms"], "launches": st["count_launches"],
@@ -196,7 +201,7 @@ def main():
     ap.add_argument("--w5-budget", type=float, default=8.0)
     ap.add_argument("--sequential", action="store_true",
                     help="mine the three supports as three searches instead of one batched search")
-    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5"), default=None,
+    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5", "workload5_learned"), default=None,
                     help="run one extra workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -209,7 +214,8 @@ def main():
 
     if args.extra_only:
         res = {"workload3": lambda: workload3_run(Context), "workload4": lambda: workload4_run(Context),
-               "workload5": lambda: workload5_run(Context, args.w5_budget)}[args.extra_only]()
+               "workload5": lambda: workload5_run(Context, args.w5_budget),
+               "workload5_learned": lambda: workload5_run(Context, args.w5_budget, learned_order=True)}[args.extra_only]()
         print(json.dumps(res))
         return
 
@@ -325,6 +331,7 @@ def main():
     ctx.close()
     if not args.no_extra:
         out["extra"] = {"workload5": workload5_run(Context, args.w5_budget),
+                        "workload5_learned": workload5_run(Context, args.w5_budget, learned_order=True),
                         "workload3": workload3_run(Context),
                         "workload4": workload4_run(Context)}
     if not args.no_cpu:
