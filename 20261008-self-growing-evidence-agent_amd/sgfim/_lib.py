@@ -113,7 +113,7 @@ def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise SgfimError(E_STATE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+            raise SgfimError(E_STATE, f"{LIB_PATH} is not built; run python -m sgfim.build")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name, None)
