@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sgfim.h"
@@ -298,7 +299,10 @@ double now_s(const SearchState* S)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t0).count();
 }
 
-// Copy MFIs [ring_read, pos) from the ring into S->pending.
+// Copy MFIs [ring_read, pos) from the ring into S->pending. The ring lives in mapped host
+// memory and the stream has been synchronised, so entries are read in place. The outputs
+// are sized once (a prefix over the itemset lengths); large hand-overs are filled by
+// several threads, each converting a contiguous range of entries.
 int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
 {
     const int64_t n = pos - S->ring_read;
@@ -306,30 +310,47 @@ int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
     const int64_t cap = S->B->D.ring_cap;
     if (n > cap) return fail(c, SGFIM_E_CAPACITY, "MFI ring overflow (%lld new entries, capacity %lld)",
                              (long long)n, (long long)cap);
-    std::vector<uint16_t> it((size_t)n * SG_MAX_DEPTH);
-    std::vector<int32_t> meta((size_t)n * 4);
-    const int64_t a = S->ring_read % cap;
-    const int64_t first = std::min(n, cap - a);
-    // the ring lives in mapped host memory and the stream has been synchronised
-    memcpy(it.data(), S->B->h_ring_items + a * SG_MAX_DEPTH, (size_t)first * SG_MAX_DEPTH * 2);
-    memcpy(meta.data(), S->B->h_ring_meta + a * 4, (size_t)first * 16);
-    if (first < n) {
-        memcpy(it.data() + first * SG_MAX_DEPTH, S->B->h_ring_items, (size_t)(n - first) * SG_MAX_DEPTH * 2);
-        memcpy(meta.data() + first * 4, S->B->h_ring_meta, (size_t)(n - first) * 16);
-    }
+    const int32_t* ring_meta = S->B->h_ring_meta;      // [cap][4]: k, sup, seg, round
+    const uint16_t* ring_items = S->B->h_ring_items;   // [cap][SG_MAX_DEPTH]
+    const int64_t r0 = S->ring_read % cap;
+    auto slot = [&](int64_t i) { const int64_t s = r0 + i; return s < cap ? s : s - cap; };
     sgfim_result& R = S->pending;
-    for (int64_t i = 0; i < n; i++) {
-        const int32_t k = meta[i * 4], sup = meta[i * 4 + 1], seg = meta[i * 4 + 2], rnd = meta[i * 4 + 3];
-        // descending item ids: the order in which the reference's search adds items (the
-        // root is the largest id, every child a smaller one), hence its Results lines
-        uint16_t* x = it.data() + i * SG_MAX_DEPTH;
-        std::sort(x, x + k, [](uint16_t a, uint16_t b) { return a > b; });
-        for (int j = 0; j < k; j++) R.items.push_back(c->ids[x[j]]);
-        R.off.push_back((int64_t)R.items.size());
-        R.sup.push_back(sup);
-        R.round.push_back(rnd);
-        R.time_s.push_back(t);
-        R.seg.push_back(seg);
+    const size_t n0 = R.sup.size();
+    R.off.resize(n0 + 1 + (size_t)n);
+    for (int64_t i = 0; i < n; i++) R.off[n0 + 1 + i] = R.off[n0 + i] + ring_meta[slot(i) * 4];
+    R.items.resize((size_t)R.off[n0 + n]);
+    R.sup.resize(n0 + (size_t)n);
+    R.round.resize(n0 + (size_t)n);
+    R.time_s.resize(n0 + (size_t)n);
+    R.seg.resize(n0 + (size_t)n);
+    const int32_t* ids = c->ids.data();
+    auto convert = [&](int64_t lo, int64_t hi) {
+        uint16_t x[SG_MAX_DEPTH];
+        for (int64_t i = lo; i < hi; i++) {
+            const int64_t s = slot(i);
+            const int32_t* m = ring_meta + s * 4;
+            const int k = m[0];
+            std::copy(ring_items + s * SG_MAX_DEPTH, ring_items + s * SG_MAX_DEPTH + k, x);
+            // descending item ids: the order in which the reference's search adds items (the
+            // root is the largest id, every child a smaller one), hence its Results lines
+            std::sort(x, x + k, [](uint16_t a, uint16_t b) { return a > b; });
+            int32_t* out = R.items.data() + R.off[n0 + i];
+            for (int j = 0; j < k; j++) out[j] = ids[x[j]];
+            R.sup[n0 + i] = m[1];
+            R.seg[n0 + i] = m[2];
+            R.round[n0 + i] = m[3];
+            R.time_s[n0 + i] = t;
+        }
+    };
+    const int64_t per_thread = 1 << 15;
+    const int nt = (int)std::min<int64_t>({(n + per_thread - 1) / per_thread,
+                                           (int64_t)std::max(1u, std::thread::hardware_concurrency()), 16});
+    if (nt <= 1) {
+        convert(0, n);
+    } else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nt; k++) pool.emplace_back(convert, n * k / nt, n * (k + 1) / nt);
+        for (auto& th : pool) th.join();
     }
     S->ring_read = pos;
     return SGFIM_OK;
