@@ -942,42 +942,69 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
 // Newborn records go onto their priority bucket; the batch's record slots go back to
 // the free list (their children have all been expanded). The two index spaces are
 // independent, so one grid-stride launch does both. The host sums the bucket tops.
-__global__ void k_commit(SgSearchDev S, const SgSegment* __restrict__ segs, int32_t order_mode,
-                         const float* __restrict__ scores, int32_t release)
+__global__ __launch_bounds__(256) void k_commit(SgSearchDev S, const SgSegment* __restrict__ segs,
+                                                int32_t order_mode, const float* __restrict__ scores,
+                                                int32_t release)
 {
+    // bucket positions are reserved once per workgroup and bucket (an LDS histogram gives
+    // each newborn its rank), not once per newborn
+    __shared__ int32_t s_cnt[SG_NBUCKETS];
+    __shared__ int64_t s_base[SG_NBUCKETS];
+    __shared__ int64_t s_free;
     const int64_t n = min(S.ctl[CTL_NEWBORN], S.newborn_cap);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int64_t i = t0; i < n; i += stride) {
-        const int32_t slot = S.newborn[i];
-        if (slot < 0 || slot >= S.cap) continue;
-        const SgNodeMeta m = S.meta[slot];
-        int b;
-        if (order_mode == 1) {
-            const float sc = scores[i];
-            S.meta[slot].score = sc;
-            b = sg_bucket_score(sc, m.sup, segs[m.seg].thr);
-        } else if (order_mode == 2) {
-            b = min(m.k, SG_NBUCKETS - 1);
-        } else {
-            b = sg_bucket_support(m.sup, segs[m.seg].thr);
-        }
-        const int32_t pos = (int32_t)atomicAdd((unsigned long long*)&S.ctl[CTL_BUCKET_TOP0 + b], 1ull);
-        S.bucket[(int64_t)b * S.cap + pos] = slot;
-        if (S.trace_cap) {
-            const int64_t t = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_TRACE_BIRTH], 1ull);
-            if (t < S.trace_cap) {
-                S.trace_key[2 * t] = S.key[slot];
-                S.trace_key[2 * t + 1] = S.pkey[slot];
-                for (int f = 0; f < SG_FEAT; f++) S.trace_feat[t * SG_FEAT + f] = S.feat[i * SG_FEAT + f];
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {   // workgroup-uniform
+        const int64_t i = i0 + threadIdx.x;
+        if (threadIdx.x < SG_NBUCKETS) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        int b = -1, rank = 0;
+        int32_t slot = -1;
+        if (i < n) {
+            slot = S.newborn[i];
+            if (slot >= 0 && slot < S.cap) {
+                const SgNodeMeta m = S.meta[slot];
+                if (order_mode == 1) {
+                    const float sc = scores[i];
+                    S.meta[slot].score = sc;
+                    b = sg_bucket_score(sc, m.sup, segs[m.seg].thr);
+                } else if (order_mode == 2) {
+                    b = min(m.k, SG_NBUCKETS - 1);
+                } else {
+                    b = sg_bucket_support(m.sup, segs[m.seg].thr);
+                }
+                rank = atomicAdd(&s_cnt[b], 1);
             }
         }
+        __syncthreads();
+        if (threadIdx.x < SG_NBUCKETS && s_cnt[threadIdx.x])
+            s_base[threadIdx.x] = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_BUCKET_TOP0 + threadIdx.x],
+                                                     (unsigned long long)s_cnt[threadIdx.x]);
+        __syncthreads();
+        if (b >= 0) {
+            S.bucket[(int64_t)b * S.cap + s_base[b] + rank] = slot;
+            if (S.trace_cap) {
+                const int64_t t = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_TRACE_BIRTH], 1ull);
+                if (t < S.trace_cap) {
+                    S.trace_key[2 * t] = S.key[slot];
+                    S.trace_key[2 * t + 1] = S.pkey[slot];
+                    for (int f = 0; f < SG_FEAT; f++) S.trace_feat[t * SG_FEAT + f] = S.feat[i * SG_FEAT + f];
+                }
+            }
+        }
+        __syncthreads();   // s_cnt and s_base are reused by the next iteration
     }
-    if (release) {
+    if (release) {   // the batch's record slots go back onto the free list, one reservation per workgroup
         const int64_t nb = S.ctl[CTL_BATCH_N];
-        for (int64_t i = t0; i < nb; i += stride) {
-            const int64_t pos = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], 1ull);
-            if (pos >= 0 && pos < S.cap) S.free_idx[pos] = S.batch[i];   // out of range only after an overflow
+        for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nb; i0 += stride) {
+            const int64_t cnt = min((int64_t)blockDim.x, nb - i0);
+            if (threadIdx.x == 0) s_free = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP],
+                                                              (unsigned long long)cnt);
+            __syncthreads();
+            if (threadIdx.x < cnt) {
+                const int64_t pos = s_free + threadIdx.x;
+                if (pos >= 0 && pos < S.cap) S.free_idx[pos] = S.batch[i0 + threadIdx.x];   // out of range only after an overflow
+            }
+            __syncthreads();
         }
     }
 }
