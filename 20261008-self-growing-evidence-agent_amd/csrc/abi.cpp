@@ -46,6 +46,7 @@ struct SearchBufs {
     std::vector<DevBuf> bufs;
     hipGraph_t graph[2] = {nullptr, nullptr};       // one captured round per frontier order
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int32_t graph_feat[2] = {0, 0};                 // the D.feat_on each round was captured with
     hipEvent_t ev0 = nullptr, ev1 = nullptr;        // around the counting kernel (direct launches)
     int64_t* h_ctl = nullptr;                       // pinned: processed copy of the control block, then
                                                     // two snapshots of rounds in flight
@@ -200,14 +201,20 @@ void search_free(sgfim_ctx* c)
     c->S = nullptr;
 }
 
+// Drop captured round g: its kernel arguments name the loaded data's buffers.
+void graph_free(SearchBufs* B, int g)
+{
+    if (B->exec[g]) (void)hipGraphExecDestroy(B->exec[g]);
+    if (B->graph[g]) (void)hipGraphDestroy(B->graph[g]);
+    B->exec[g] = nullptr;
+    B->graph[g] = nullptr;
+}
+
 void bufs_free(sgfim_ctx* c)
 {
     SearchBufs* B = c->SB;
     if (!B) return;
-    for (int i = 0; i < 2; i++) {
-        if (B->exec[i]) (void)hipGraphExecDestroy(B->exec[i]);
-        if (B->graph[i]) (void)hipGraphDestroy(B->graph[i]);
-    }
+    for (int i = 0; i < 2; i++) graph_free(B, i);
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->h_ring_items) (void)hipHostFree(B->h_ring_items);
     if (B->h_ring_meta) (void)hipHostFree(B->h_ring_meta);
@@ -220,10 +227,14 @@ void bufs_free(sgfim_ctx* c)
     c->SB = nullptr;
 }
 
+// Frees the loaded data. The search buffers (frontier, buckets, pinned MFI ring) stay for
+// the next search on new data when they fit it; only the captured rounds, whose arguments
+// name the old data's buffers, are dropped.
 void data_free(sgfim_ctx* c)
 {
     search_free(c);
-    bufs_free(c);
+    if (c->SB)
+        for (int i = 0; i < 2; i++) graph_free(c->SB, i);
     dev_free_all(c, c->data_bufs);
     c->d_rows = nullptr;
     c->d_sup = nullptr;
@@ -416,7 +427,9 @@ int launch_round(sgfim_ctx* c, SearchState* S)
         return SGFIM_OK;
     }
     const int g = S->order_mode;
+    if (B->exec[g] && B->graph_feat[g] != B->D.feat_on) graph_free(B, g);
     if (!B->exec[g]) {
+        B->graph_feat[g] = B->D.feat_on;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = sg_search_round(B->D, c->d_rows, c->d_vseg, c->d_pair, c->d_ok, c->d_sup,
                                        c->d_vseg_ibase, uniform, S->order_mode, &c->scorer, nullptr, nullptr,
@@ -567,6 +580,7 @@ void sgfim_destroy(sgfim_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     data_free(c);
+    bufs_free(c);
     dev_free_all(c, c->scorer_bufs);
     if (c->h_pair_pin) (void)hipHostFree(c->h_pair_pin);
     for (int i = 0; i < 2; i++)
@@ -1398,16 +1412,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         return rc;
     S->B = c->SB;
     SgSearchDev& D = S->B->D;
-    const int32_t feat_on = (S->order_mode == 1 || D.trace_cap > 0) ? 1 : 0;
-    if (D.feat_on != feat_on) {  // captured rounds hold the old value: capture again
-        for (int i = 0; i < 2; i++) {
-            if (S->B->exec[i]) (void)hipGraphExecDestroy(S->B->exec[i]);
-            if (S->B->graph[i]) (void)hipGraphDestroy(S->B->graph[i]);
-            S->B->exec[i] = nullptr;
-            S->B->graph[i] = nullptr;
-        }
-        D.feat_on = feat_on;
-    }
+    D.feat_on = (S->order_mode == 1 || D.trace_cap > 0) ? 1 : 0;   // a round captured with another value is captured again
     HIPCHK(c, sg_search_init(D, c->stream));
     S->n_roots = (int32_t)n_roots;
     S->roots = std::move(rp);
@@ -1638,7 +1643,6 @@ int sgfim_set_scorer(sgfim_ctx* c, const float* w1, const float* b1, const float
 {
     if (!c || !w1 || !b1 || !w2) return SGFIM_E_INVAL;
     (void)hipSetDevice(c->device);
-    dev_free_all(c, c->scorer_bufs);
     c->scorer_set = false;
     auto bf16 = [](float f) -> uint16_t {
         uint32_t u;
@@ -1652,9 +1656,16 @@ int sgfim_set_scorer(sgfim_ctx* c, const float* w1, const float* b1, const float
     for (int i = 0; i < SG_HID; i++) w2b[i] = bf16(w2[i]);
     int rc;
     SgScorer& s = c->scorer;
-    if ((rc = dev_alloc_t(c, c->scorer_bufs, &s.w1, w1t.size())) || (rc = dev_alloc_t(c, c->scorer_bufs, &s.b1, SG_HID)) ||
-        (rc = dev_alloc_t(c, c->scorer_bufs, &s.w2, SG_HID)) || (rc = dev_alloc_t(c, c->scorer_bufs, &s.b2, 1)))
+    // the weights live in buffers allocated once per context and overwritten in stream
+    // order: a captured learned-order round names these addresses
+    if (!s.w1 && ((rc = dev_alloc_t(c, c->scorer_bufs, &s.w1, w1t.size())) ||
+                  (rc = dev_alloc_t(c, c->scorer_bufs, &s.b1, SG_HID)) ||
+                  (rc = dev_alloc_t(c, c->scorer_bufs, &s.w2, SG_HID)) ||
+                  (rc = dev_alloc_t(c, c->scorer_bufs, &s.b2, 1)))) {
+        dev_free_all(c, c->scorer_bufs);
+        s = SgScorer{};
         return rc;
+    }
     HIPCHK(c, hipMemcpyAsync(s.w1, w1t.data(), w1t.size() * 2, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.b1, b1, SG_HID * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.w2, w2b.data(), SG_HID * 2, hipMemcpyHostToDevice, c->stream));

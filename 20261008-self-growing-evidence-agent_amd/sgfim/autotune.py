@@ -146,7 +146,7 @@ def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None,
 
 def main(argv=None):
     """Tune the benchmark's workload 2 (three supports in one search), workload 3 (512
-    windows) and workload 4 (rounds of up to 1024 records) on device 0 and write the cache."""
+    windows) and workload 4 (rounds of up to 4096 records) on device 0 and write the cache."""
     import argparse
     from . import gen
     from .core import Context
@@ -157,7 +157,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     say = lambda m: print(m, flush=True)
     for w in a.workloads.split(","):
-        ctx = Context(device=0, batch_nodes=1024 if w == "4" else 0)
+        ctx = Context(device=0, batch_nodes=4096 if w == "4" else 0)
         if w == "2":
             T = gen.workload2(0)
             ctx.load_csr(T.offsets, T.items)
@@ -170,7 +170,7 @@ def main(argv=None):
             thrs = [gen.min_count(0.03, int(n)) for n in lens]
             run = lambda: ctx.mine_windows(thrs, convert=False)
             n_sets = len(lens)
-        elif w == "4":   # anytime rounds of up to 1024 records, as the coverage evaluation runs them
+        elif w == "4":   # anytime rounds of up to 4096 records, as the coverage evaluation runs them
             T = gen.workload4(1_000_000)
             ctx.load_csr(T.offsets, T.items)
             thr = gen.min_count(0.02, T.n_txn)

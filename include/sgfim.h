@@ -50,9 +50,8 @@ enum sgfim_status {
                                     subtree (kernel_bitmap.cu:816,841-850); needs opts.rel_thr */
 #define SGFIM_ROOT_TOPK    0x2u  /* AnyFIM stage rule: only the first root_topk items of the
                                     stable descending-frequency order root a subtree (kernel.cu:850-859) */
-#define SGFIM_ORDER_LEARNED 0x4u /* order the frontier by the proposal network's score: records
-                                    scoring >= 0.5 first, support order within each half;
-                                    never changes the result */
+#define SGFIM_ORDER_LEARNED 0x4u /* order the frontier by the proposal network's score (64
+                                    score buckets, highest first); never changes the result */
 #define SGFIM_NO_GRAPH     0x8u  /* launch each round's kernels directly instead of replaying
                                     the round's captured hipGraph */
 #define SGFIM_FRESH_PAIR   0x10u /* recompute the pair matrix that seeds the search (on the
@@ -119,7 +118,8 @@ const char* sgfim_version(void);
  * (kernel_bitmap.cu:205-298, 342-388). txn_off has n_txn + 1 entries; row t is
  * items[txn_off[t] .. txn_off[t+1]); duplicates inside a row are allowed (a row
  * is a set). An empty row is an empty transaction and counts in n_txn
- * (kernel_bitmap.cu:219). */
+ * (kernel_bitmap.cu:219). Every load replaces the loaded data; the search buffers of
+ * earlier searches stay allocated for the next search when they fit it. */
 int sgfim_load_csr(sgfim_ctx* ctx, const int64_t* txn_off, int64_t n_txn,
                    const int32_t* items, int64_t nnz);
 
@@ -276,8 +276,9 @@ int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
 
 /* Proposal network (24 -> 256 ReLU -> 1): load BF16 weights used by
  * SGFIM_ORDER_LEARNED. w1 is [24][256] row-major, b1 [256], w2 [256], b2 [1], all
- * float32 here and rounded to BF16 on the device. No reference counterpart: the
- * reference has no learned component. */
+ * float32 here and rounded to BF16 on the device. A later call overwrites the weights in
+ * stream order: rounds already enqueued use the old weights, later ones the new. No
+ * reference counterpart: the reference has no learned component. */
 int sgfim_set_scorer(sgfim_ctx* ctx, const float* w1, const float* b1, const float* w2, float b2);
 
 /* Search trace for training the proposal network. With tracing on, the next
