@@ -47,6 +47,7 @@ struct SearchBufs {
     hipGraph_t graph[2] = {nullptr, nullptr};       // one captured round per frontier order
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int32_t graph_feat[2] = {0, 0};                 // the D.feat_on each round was captured with
+    uint64_t graph_gen[2] = {0, 0};                 // the search universe each round was captured on
     hipEvent_t ev0 = nullptr, ev1 = nullptr;        // around the counting kernel (direct launches)
     int64_t* h_ctl = nullptr;                       // pinned: processed copy of the control block, then
                                                     // two snapshots of rounds in flight
@@ -55,6 +56,24 @@ struct SearchBufs {
     int32_t* h_ring_meta = nullptr;                 // device: kernels write MFIs straight to the host
     int32_t* d_roots = nullptr;
     int64_t roots_cap = 0;
+};
+
+// The item universe a search runs on: the loaded items, or a compact copy of the items
+// frequent at the search's thresholds (when more than SG_DMAX distinct items are loaded).
+// Ids inside the search are dense ids of this universe.
+struct Universe {
+    int32_t d = 0, dpad = 0;
+    int64_t W = 0, Wp = 0;
+    uint64_t* rows = nullptr;           // rows arena (segment row_base / stride)
+    int32_t* sup = nullptr;             // [nseg][dpad] item supports
+    int32_t* pair = nullptr;            // [nseg][dpad][dpad] pair matrix
+    bool* pair_valid = nullptr;
+    uint64_t* ok = nullptr;             // pair-ok masks, one block per search segment
+    const int64_t* h_sup_seg = nullptr; // host [nseg][d]
+    const SgSegment* segs = nullptr;    // host segment table
+    SgSegment* d_seg = nullptr;         // device segment table
+    const int32_t* ids = nullptr;       // dense -> original id
+    uint64_t gen = 0;                   // changes whenever these buffers do
 };
 
 // One search run.
@@ -77,6 +96,7 @@ struct SearchState {
     double event_ms = 0.0;           // HIP-event duration (direct launches only)
     int64_t count_launches = 0;
     SearchBufs* B = nullptr;         // the context's buffers used by this run
+    Universe U;                      // the items it searches
     // host sources of the run's asynchronous uploads, alive until the next run
     std::vector<SgSegment> vsegs;
     std::vector<int64_t> vibase;
@@ -138,6 +158,28 @@ struct sgfim_ctx {
     SearchBufs* SB = nullptr;
     int64_t cap_hint = 0;               // frontier capacity learned from an earlier overflow
     sgfim_round_stats last{};
+    // compact search universe (more than SG_DMAX distinct items loaded): the items frequent
+    // at the latest such search's thresholds, gathered from the loaded rows
+    struct {
+        std::vector<int32_t> keep;      // compact -> loaded dense id
+        std::vector<int32_t> ids;       // compact -> original id
+        std::vector<int64_t> h_sup_seg; // [nseg][d]
+        std::vector<SgSegment> segs;
+        int32_t d = 0, dpad = 0;
+        int64_t rows_cap = 0, sup_cap = 0, pair_cap = 0, ok_cap = 0;
+        int32_t seg_cap = 0;
+        uint64_t* rows = nullptr;
+        int32_t* sup = nullptr;
+        int32_t* pair = nullptr;
+        uint64_t* ok = nullptr;
+        SgSegment* d_seg = nullptr;
+        bool pair_valid = false;
+        bool built = false;
+        uint64_t gen = 0;
+        std::vector<DevBuf> bufs;
+    } cu;
+    uint64_t gen_counter = 0;           // source of universe generations
+    uint64_t loaded_gen = 0;            // generation of the loaded items' buffers
 };
 
 namespace {
@@ -230,11 +272,29 @@ void bufs_free(sgfim_ctx* c)
 // Frees the loaded data. The search buffers (frontier, buckets, pinned MFI ring) stay for
 // the next search on new data when they fit it; only the captured rounds, whose arguments
 // name the old data's buffers, are dropped.
+void universe_free(sgfim_ctx* c)
+{
+    auto& u = c->cu;
+    dev_free_all(c, u.bufs);
+    u.rows = nullptr;
+    u.sup = nullptr;
+    u.pair = nullptr;
+    u.ok = nullptr;
+    u.d_seg = nullptr;
+    u.rows_cap = u.sup_cap = u.pair_cap = u.ok_cap = 0;
+    u.seg_cap = 0;
+    u.keep.clear();
+    u.pair_valid = false;
+    u.built = false;
+}
+
 void data_free(sgfim_ctx* c)
 {
     search_free(c);
     if (c->SB)
         for (int i = 0; i < 2; i++) graph_free(c->SB, i);
+    universe_free(c);
+    c->loaded_gen = ++c->gen_counter;
     dev_free_all(c, c->data_bufs);
     c->d_rows = nullptr;
     c->d_sup = nullptr;
@@ -275,6 +335,67 @@ int32_t search_chunk(const sgfim_ctx* c)
 
 int32_t search_range(const sgfim_ctx* c) { return c->tile_cr ? c->tile_cr : 4; }
 
+// The loaded items as a search universe (pair matrix and pair-ok masks as allocated so far).
+Universe loaded_universe(sgfim_ctx* c)
+{
+    Universe U;
+    U.d = c->d;
+    U.dpad = c->dpad;
+    U.W = c->W;
+    U.Wp = c->Wp;
+    U.rows = c->d_rows;
+    U.sup = c->d_sup;
+    U.pair = c->d_pair;
+    U.pair_valid = &c->pair_valid;
+    U.ok = c->d_ok;
+    U.h_sup_seg = c->h_sup_seg.data();
+    U.segs = c->segs.data();
+    U.d_seg = c->d_seg;
+    U.ids = c->ids.data();
+    U.gen = c->loaded_gen;
+    return U;
+}
+
+Universe compact_universe(sgfim_ctx* c)
+{
+    auto& u = c->cu;
+    Universe U;
+    U.d = u.d;
+    U.dpad = u.dpad;
+    U.W = c->W;
+    U.Wp = c->Wp;
+    U.rows = u.rows;
+    U.sup = u.sup;
+    U.pair = u.pair;
+    U.pair_valid = &u.pair_valid;
+    U.ok = u.ok;
+    U.h_sup_seg = u.h_sup_seg.data();
+    U.segs = u.segs.data();
+    U.d_seg = u.d_seg;
+    U.ids = u.ids.data();
+    U.gen = u.gen;
+    return U;
+}
+
+// Pair matrix of universe U (unless still valid), timed with the context's pair events.
+int ensure_pair_u(sgfim_ctx* c, const Universe& U)
+{
+    if (*U.pair_valid) return SGFIM_OK;
+    const bool timed = c->pair_ev[0] && hipEventRecord(c->pair_ev[0], c->stream) == hipSuccess;
+    if (c->nseg == 1) {
+        HIPCHK(c, sg_launch_pair_mfma(U.rows, U.dpad, U.Wp, U.W, U.pair, c->stream));
+    } else {
+        int64_t maxw = 0;
+        for (int32_t w = 0; w < c->nseg; w++) maxw = std::max(maxw, U.segs[w].words);
+        HIPCHK(c, hipMemsetAsync(U.pair, 0, (size_t)U.dpad * U.dpad * c->nseg * 4, c->stream));
+        HIPCHK(c, sg_launch_pair_mfma_segs(U.rows, U.d_seg, c->nseg, U.dpad, maxw, U.pair, c->stream));
+    }
+    c->pair_timed = timed && hipEventRecord(c->pair_ev[1], c->stream) == hipSuccess;
+    *U.pair_valid = true;
+    return SGFIM_OK;
+}
+
+// Pair matrix of the loaded items.
 int ensure_pair(sgfim_ctx* c)
 {
     if (c->pair_valid) return SGFIM_OK;
@@ -282,17 +403,77 @@ int ensure_pair(sgfim_ctx* c)
         int rc = dev_alloc_t(c, c->data_bufs, &c->d_pair, (size_t)c->dpad * c->dpad * c->nseg);
         if (rc) return rc;
     }
-    const bool timed = c->pair_ev[0] && hipEventRecord(c->pair_ev[0], c->stream) == hipSuccess;
-    if (c->nseg == 1) {
-        HIPCHK(c, sg_launch_pair_mfma(c->d_rows, c->dpad, c->Wp, c->W, c->d_pair, c->stream));
-    } else {
-        int64_t maxw = 0;
-        for (const SgSegment& g : c->segs) maxw = std::max(maxw, g.words);
-        HIPCHK(c, hipMemsetAsync(c->d_pair, 0, (size_t)c->dpad * c->dpad * c->nseg * 4, c->stream));
-        HIPCHK(c, sg_launch_pair_mfma_segs(c->d_rows, c->d_seg, c->nseg, c->dpad, maxw, c->d_pair, c->stream));
+    return ensure_pair_u(c, loaded_universe(c));
+}
+
+// Compact search universe of the loaded items `keep` (ascending dense ids, at most SG_DMAX):
+// their rows and supports in every window, gathered on the device. The same items as the
+// current copy keep it (and its pair matrix); buffers only grow.
+int compact_build(sgfim_ctx* c, std::vector<int32_t>& keep)
+{
+    auto& u = c->cu;
+    if (u.built && u.keep == keep) return SGFIM_OK;
+    const int32_t dc = (int32_t)keep.size();
+    const int32_t dcpad = std::max(128, (dc + 127) / 128 * 128);
+    const int32_t dwc = (dc + 63) / 64;
+    const int32_t nseg = c->nseg;
+    std::vector<SgSegment> segs(c->segs.begin(), c->segs.begin() + nseg);
+    int64_t base = 0;
+    for (int32_t w = 0; w < nseg; w++) {
+        SgSegment& g = segs[w];
+        g.row_base = base;
+        base += (int64_t)dcpad * g.stride;
+        g.d = dc;
+        g.dpad = dcpad;
+        g.dw = dwc;
+        g.pair_base = (int64_t)w * dcpad * dcpad;
+        g.ok_base = (int64_t)w * dc * dwc;
     }
-    c->pair_timed = timed && hipEventRecord(c->pair_ev[1], c->stream) == hipSuccess;
-    c->pair_valid = true;
+    const int64_t sup_need = (int64_t)dcpad * nseg, pair_need = (int64_t)dcpad * dcpad * nseg;
+    const int64_t ok_need = (int64_t)dc * dwc * c->vseg_cap;
+    if (base > u.rows_cap || sup_need > u.sup_cap || pair_need > u.pair_cap || ok_need > u.ok_cap ||
+        nseg > u.seg_cap) {
+        universe_free(c);
+        int rc;
+        if ((rc = dev_alloc_t(c, u.bufs, &u.rows, (size_t)base)) || (rc = dev_alloc_t(c, u.bufs, &u.sup, (size_t)sup_need)) ||
+            (rc = dev_alloc_t(c, u.bufs, &u.pair, (size_t)pair_need)) || (rc = dev_alloc_t(c, u.bufs, &u.ok, (size_t)ok_need)) ||
+            (rc = dev_alloc_t(c, u.bufs, &u.d_seg, (size_t)nseg))) {
+            universe_free(c);
+            return rc;
+        }
+        u.rows_cap = base;
+        u.sup_cap = sup_need;
+        u.pair_cap = pair_need;
+        u.ok_cap = ok_need;
+        u.seg_cap = nseg;
+    }
+    u.built = false;
+    std::vector<DevBuf> tmp;
+    int32_t* d_keep = nullptr;
+    int rc;
+    if ((rc = dev_alloc_t(c, tmp, &d_keep, (size_t)dc))) return rc;
+    hipError_t e = hipMemcpyAsync(d_keep, keep.data(), (size_t)dc * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(u.d_seg, segs.data(), (size_t)nseg * sizeof(SgSegment), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(u.rows, 0, (size_t)base * 8, c->stream);   // pad rows stay zero
+    if (e == hipSuccess) e = hipMemsetAsync(u.sup, 0, (size_t)sup_need * 4, c->stream);
+    if (e == hipSuccess)
+        e = sg_launch_gather_rows(c->d_rows, c->d_sup, c->d_seg, c->dpad, u.rows, u.sup, u.d_seg, dcpad, d_keep, dc, nseg,
+                                  c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free_all(c, tmp);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "compacting the search's items failed: %s", hipGetErrorString(e));
+    u.h_sup_seg.assign((size_t)dc * nseg, 0);
+    u.ids.resize(dc);
+    for (int32_t i = 0; i < dc; i++) u.ids[i] = c->ids[keep[i]];
+    for (int32_t w = 0; w < nseg; w++)
+        for (int32_t i = 0; i < dc; i++) u.h_sup_seg[(size_t)w * dc + i] = c->h_sup_seg[(size_t)w * c->d + keep[i]];
+    u.keep = keep;
+    u.segs = std::move(segs);
+    u.d = dc;
+    u.dpad = dcpad;
+    u.pair_valid = false;
+    u.built = true;
+    u.gen = ++c->gen_counter;
     return SGFIM_OK;
 }
 
@@ -334,7 +515,7 @@ int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
     R.round.resize(n0 + (size_t)n);
     R.time_s.resize(n0 + (size_t)n);
     R.seg.resize(n0 + (size_t)n);
-    const int32_t* ids = c->ids.data();
+    const int32_t* ids = S->U.ids;
     auto convert = [&](int64_t lo, int64_t hi) {
         uint16_t x[SG_MAX_DEPTH];
         for (int64_t i = lo; i < hi; i++) {
@@ -422,16 +603,17 @@ int launch_round(sgfim_ctx* c, SearchState* S)
     SearchBufs* B = S->B;
     (void)hipGetLastError();
     if (!S->use_graph) {
-        HIPCHK(c, sg_search_round(B->D, c->d_rows, c->d_vseg, c->d_pair, c->d_ok, c->d_sup, c->d_vseg_ibase,
+        HIPCHK(c, sg_search_round(B->D, S->U.rows, c->d_vseg, S->U.pair, S->U.ok, S->U.sup, c->d_vseg_ibase,
                                   uniform, S->order_mode, &c->scorer, B->ev0, B->ev1, c->stream));
         return SGFIM_OK;
     }
     const int g = S->order_mode;
-    if (B->exec[g] && B->graph_feat[g] != B->D.feat_on) graph_free(B, g);
+    if (B->exec[g] && (B->graph_feat[g] != B->D.feat_on || B->graph_gen[g] != S->U.gen)) graph_free(B, g);
     if (!B->exec[g]) {
         B->graph_feat[g] = B->D.feat_on;
+        B->graph_gen[g] = S->U.gen;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = sg_search_round(B->D, c->d_rows, c->d_vseg, c->d_pair, c->d_ok, c->d_sup,
+        hipError_t e = sg_search_round(B->D, S->U.rows, c->d_vseg, S->U.pair, S->U.ok, S->U.sup,
                                        c->d_vseg_ibase, uniform, S->order_mode, &c->scorer, nullptr, nullptr,
                                        c->stream);
         hipError_t e2 = hipStreamEndCapture(c->stream, &B->graph[g]);
@@ -1300,7 +1482,6 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         return fail(c, SGFIM_E_STATE, "SGFIM_ORDER_LEARNED needs sgfim_set_scorer first");
     if ((a->flags & SGFIM_ROOT_STRICT) && (a->flags & SGFIM_ROOT_TOPK))
         return fail(c, SGFIM_E_INVAL, "SGFIM_ROOT_STRICT and SGFIM_ROOT_TOPK are exclusive");
-    if (c->d > 4096) return fail(c, SGFIM_E_INVAL, "mining supports at most 4096 distinct items (got %d)", c->d);
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     search_free(c);
@@ -1311,9 +1492,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     S->order_mode = (a->flags & SGFIM_ORDER_LEARNED) ? 1 : 0;
     S->use_graph = !(a->flags & SGFIM_NO_GRAPH);
     S->t0 = std::chrono::steady_clock::now();
-    const int32_t d = c->d;
     const int32_t nseg = c->nseg;
-    const int32_t dw = (d + 63) / 64;
     int rc;
     // search segments: one per window (its own threshold), or one per threshold of a
     // single set (one batched search over several thresholds; roots and results per
@@ -1332,12 +1511,41 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     for (int64_t t : thr)
         if (t < 1 || t > (int64_t)INT32_MAX) return fail(c, SGFIM_E_INVAL, "min_count out of range");
     auto data_seg = [&](int32_t v) { return multi ? 0 : v; };
+    // the universe: the loaded items, or -- past SG_DMAX of them -- a compact copy of the
+    // items frequent at some search segment's threshold (no other item can be in an MFI,
+    // a frontier record or a candidate list)
+    if (c->d <= SG_DMAX) {
+        S->U = loaded_universe(c);
+    } else {
+        std::vector<int32_t> keep;
+        for (int32_t x = 0; x < c->d; x++)
+            for (int32_t v = 0; v < nv; v++)
+                if (c->h_sup_seg[(size_t)data_seg(v) * c->d + x] >= thr[v]) {
+                    keep.push_back(x);
+                    break;
+                }
+        if ((int64_t)keep.size() > SG_DMAX)
+            return fail(c, SGFIM_E_INVAL, "mining supports at most %d items frequent at the search's thresholds (got %lld of %d)",
+                        SG_DMAX, (long long)keep.size(), c->d);
+        if (keep.empty()) {
+            S->B = nullptr;   // nothing frequent: no search buffers are touched
+            S->active = true;
+            S->done = true;
+            fill_stats(c, S);
+            return SGFIM_OK;
+        }
+        if ((rc = compact_build(c, keep))) return rc;
+        S->U = compact_universe(c);
+    }
+    Universe& U = S->U;
+    const int32_t d = U.d;
+    const int32_t dw = (d + 63) / 64;
     // roots (search segment, dense id): frequent items passing the root rule
     std::vector<int32_t> rp;
     for (int32_t v = 0; v < nv; v++) {
         const int32_t w = data_seg(v);
-        const int64_t* sup = c->h_sup_seg.data() + (size_t)w * d;
-        const int64_t n = c->segs[w].n;
+        const int64_t* sup = U.h_sup_seg + (size_t)w * d;
+        const int64_t n = U.segs[w].n;
         if (a->flags & SGFIM_ROOT_TOPK) {
             const int32_t topk = (multi && !c->multi_topk.empty()) ? c->multi_topk[v] : a->root_topk;
             std::vector<int32_t> order(d);
@@ -1365,8 +1573,15 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         fill_stats(c, S);
         return SGFIM_OK;
     }
-    if (a->flags & SGFIM_FRESH_PAIR) c->pair_valid = false;
-    if ((rc = ensure_pair(c))) return rc;
+    if (a->flags & SGFIM_FRESH_PAIR) *U.pair_valid = false;
+    if (U.gen == c->loaded_gen) {   // the loaded items' pair matrix and pair-ok masks are allocated on first use
+        if ((rc = ensure_pair(c))) return rc;
+        if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * c->vseg_cap))) return rc;
+        U.pair = c->d_pair;
+        U.ok = c->d_ok;
+    } else if ((rc = ensure_pair_u(c, U))) {
+        return rc;
+    }
     S->pair_counts = n_roots * d;
     // search-segment table (thresholds, pair-ok blocks) + pair_ok masks
     std::vector<SgSegment>& vs = S->vsegs;
@@ -1374,15 +1589,14 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     vs.resize(nv);
     vib.resize(nv);
     for (int32_t v = 0; v < nv; v++) {
-        vs[v] = c->segs[data_seg(v)];
+        vs[v] = U.segs[data_seg(v)];
         vs[v].thr = (int32_t)thr[v];
         vs[v].ok_base = (int64_t)v * d * dw;
-        vib[v] = (int64_t)data_seg(v) * c->dpad;
+        vib[v] = (int64_t)data_seg(v) * U.dpad;
     }
     HIPCHK(c, hipMemcpyAsync(c->d_vseg, vs.data(), (size_t)nv * sizeof(SgSegment), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_vseg_ibase, vib.data(), (size_t)nv * 8, hipMemcpyHostToDevice, c->stream));
-    if (!c->d_ok && (rc = dev_alloc_t(c, c->data_bufs, &c->d_ok, (size_t)d * dw * c->vseg_cap))) return rc;
-    HIPCHK(c, sg_launch_pair_ok(c->d_pair, c->d_vseg, nv, d, c->d_ok, c->stream));
+    HIPCHK(c, sg_launch_pair_ok(U.pair, c->d_vseg, nv, d, U.ok, c->stream));
     // the seed turns every root into at most one frontier record
     const int64_t seed_records = n_roots;
     // search buffers (reused across runs when they fit)
@@ -1417,7 +1631,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     S->n_roots = (int32_t)n_roots;
     S->roots = std::move(rp);
     HIPCHK(c, hipMemcpyAsync(S->B->d_roots, S->roots.data(), S->roots.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, sg_search_seed(D, c->d_vseg, c->d_pair, c->d_ok, S->B->d_roots, S->n_roots, c->d_sup, c->d_vseg_ibase,
+    HIPCHK(c, sg_search_seed(D, c->d_vseg, U.pair, U.ok, S->B->d_roots, S->n_roots, U.sup, c->d_vseg_ibase,
                              S->order_mode, &c->scorer, c->stream));
     // no synchronisation here: the seed's MFIs, errors and frontier are read after the
     // first round (a round over an empty frontier does nothing)

@@ -71,6 +71,35 @@ hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const i
     return hipGetLastError();
 }
 
+// Compact search universe: one wave per (kept row, segment) copies the row and its support.
+__global__ __launch_bounds__(256) void k_gather_rows(const uint64_t* __restrict__ src, const int32_t* __restrict__ src_sup,
+                                                     const SgSegment* __restrict__ ssegs, int32_t sdpad,
+                                                     uint64_t* __restrict__ dst, int32_t* __restrict__ dst_sup,
+                                                     const SgSegment* __restrict__ dsegs, int32_t ddpad,
+                                                     const int32_t* __restrict__ keep, int32_t dc)
+{
+    const int w = blockIdx.y;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= dc) return;
+    const SgSegment sg = ssegs[w], tg = dsegs[w];
+    const int32_t x = keep[i];
+    const uint64_t* a = src + sg.row_base + (int64_t)x * sg.stride;
+    uint64_t* b = dst + tg.row_base + (int64_t)i * tg.stride;
+    for (int64_t k = lane; k < sg.stride; k += 64) b[k] = a[k];
+    if (lane == 0) dst_sup[(int64_t)w * ddpad + i] = src_sup[(int64_t)w * sdpad + x];
+}
+
+hipError_t sg_launch_gather_rows(const uint64_t* src, const int32_t* src_sup, const SgSegment* src_segs,
+                                 int32_t src_dpad, uint64_t* dst, int32_t* dst_sup, const SgSegment* dst_segs,
+                                 int32_t dst_dpad, const int32_t* keep, int32_t dc, int32_t nseg, hipStream_t s)
+{
+    if (dc <= 0 || nseg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((dc + 3) / 4), (unsigned)nseg), dim3(256), 0, s, src, src_sup,
+                       src_segs, src_dpad, dst, dst_sup, dst_segs, dst_dpad, keep, dc);
+    return hipGetLastError();
+}
+
 // Plain device copy (16-byte loads and stores, grid-stride): the on-box bandwidth probe
 // and the known-size reference for hardware byte counters.
 __global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)

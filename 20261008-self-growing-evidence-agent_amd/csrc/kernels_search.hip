@@ -99,7 +99,6 @@ __device__ __forceinline__ void sg_features(const SgFeatIn& f, float* out)
 
 // ------------------------------------------------------------------ records
 #define SG_CB 16      // children per counting tile
-#define SG_DMAX 4096  // most items a search handles (the host refuses more)
 
 __device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
 {

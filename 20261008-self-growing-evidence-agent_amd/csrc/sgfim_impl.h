@@ -19,6 +19,8 @@
 #define SG_NBUCKETS 64
 #define SG_WAVE 64
 #define SG_BMAX 16384   // largest batch of frontier nodes per round
+#define SG_DMAX 4096    // most items one search handles (a larger loaded set is compacted to
+                        // its frequent items first)
 
 struct SgSegment {
     int64_t row_base;   // word offset of row 0 of this segment in the rows arena
@@ -55,6 +57,11 @@ struct SgNodeMeta {
 hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const int32_t* idmap,
                                int64_t n_txn, int32_t id_max, uint64_t* rows, int32_t d,
                                int32_t dpad, int64_t Wp, int32_t* supports, hipStream_t s);
+// Row keep[i] of every segment of (src, src_sup) -> row i of (dst, dst_sup), for i < dc;
+// the segment tables give both arenas' row bases (equal strides).
+hipError_t sg_launch_gather_rows(const uint64_t* src, const int32_t* src_sup, const SgSegment* src_segs,
+                                 int32_t src_dpad, uint64_t* dst, int32_t* dst_sup, const SgSegment* dst_segs,
+                                 int32_t dst_dpad, const int32_t* keep, int32_t dc, int32_t nseg, hipStream_t s);
 
 hipError_t sg_launch_gen_zipf(uint64_t* rows, int64_t Wp, int64_t n_txn, int32_t d, const uint64_t* q53,
                               const uint64_t* r53, const int32_t* item_sets_off, const int32_t* item_sets,

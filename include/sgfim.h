@@ -239,7 +239,10 @@ int sgfim_count_batch(sgfim_ctx* ctx, const int32_t* prefix_ids, const int64_t* 
  * (kernel_bitmap.cu:796-910, kernel_bmma.cu:1179-1295) and one AnyFIM stage
  * (AnyTime kernel.cu:844-907). Without root flags the result is the exact MFI
  * family {X : sup(X) >= min_count, no single-item extension frequent}, the
- * family code/pipeline/skab_spike.py:84-135 returns. */
+ * family code/pipeline/skab_spike.py:84-135 returns. One search handles at most
+ * 4096 items: with more loaded, it runs on a device copy of the items frequent at
+ * its thresholds (the reference compacts to those items, kernel_bitmap.cu:692-703),
+ * and more than 4096 frequent items is SGFIM_E_INVAL. */
 int sgfim_mine(sgfim_ctx* ctx, const sgfim_mine_args* args, sgfim_result** out);
 
 /* Anytime search: begin, advance by rounds, poll, cancel. MFIs are confirmed
