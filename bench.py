@@ -25,6 +25,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
+# 32-bit VALU lane-ops/s: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (a wave64 instruction
+# issues over four cycles)
+VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 # HBM bytes per k_count_group dispatch for this command, from the two rocprofv3 --pmc
 # passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
 # what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
@@ -301,6 +304,8 @@ def main():
     avg_ms = tot_cms / max(tot_launch, 1)
     bytes_per_launch = 8.0 * tot_words / max(tot_launch, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # the AND + popcount a count needs: per 64-bit word two 32-bit ANDs and two popcounts
+    valu_achieved = 4.0 * tot_words / (tot_cms * 1e-3) / 1e12 if tot_cms > 0 else 0.0
     out = {
         "metric": "support counts/s (exact MFI mining, workload 2 at 10%/5%/2%)",
         "value": value,
@@ -327,7 +332,13 @@ def main():
                                        "calibrated by the copy probe (profiles/k_count_group_workload2_pmc.md)",
                      "kernel": "k_count_group", "avg_launch_ms": avg_ms,
                      "alg_bytes_per_launch": bytes_per_launch, "timing_check": event_check,
-                     "copy_probe_gbs": copy_gbs},
+                     "copy_probe_gbs": copy_gbs,
+                     # the bound that holds: candidate rows are reread from L2 and the MALL, so
+                     # the algorithmic bytes exceed the HBM peak; the kernel is bound by VALU
+                     # issue, and this is the share of it spent on the AND + popcount itself
+                     "valu": {"achieved": valu_achieved, "peak": VALU_PEAK_TLANE, "unit": "T lane-op/s",
+                              "frac": valu_achieved / VALU_PEAK_TLANE,
+                              "counted": "4 lane-ops (2 AND, 2 popcount) per 64-bit word of every count"}},
         # the pair matrix (part of every step): 2 d^2 N int8 ops (the full square) per launch
         "pair_roofline": {"bound": "mfma", "kernel": "k_pair_tile", "avg_launch_ms": pair_avg,
                           "achieved": pair_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
