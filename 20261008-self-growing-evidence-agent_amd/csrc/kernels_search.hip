@@ -687,7 +687,6 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     uint64_t* okr = &cp[0][0];
     const int64_t n_units = S.ctl[CTL_UNITS];
     const int64_t n_blocks = S.ctl[CTL_BLOCKS];
-    const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
     if (n_units == 0) return;   // a round queued after the frontier emptied: nothing to count
     if (threadIdx.x == 0)
         atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
@@ -846,7 +845,6 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
     __shared__ unsigned long long s_counts, s_words;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t nchild = S.ctl[CTL_CHILDREN];
-    const int32_t nb = (int32_t)S.ctl[CTL_BATCH_N];
     if (threadIdx.x == 0) { s_counts = 0; s_words = 0; }
     for (int64_t q0 = (int64_t)blockIdx.x * SG_EXP_WAVES; q0 < nchild; q0 += (int64_t)gridDim.x * SG_EXP_WAVES) {
         const int64_t q = q0 + wv;
