@@ -113,6 +113,7 @@ struct sgfim_ctx {
     int n_cu = 256;                     // compute units of the device
     std::string dev_name;               // gcnArchName of the device
     int32_t tile_cw = 0, tile_cr = 0;   // counting-tile shape of the search (0: default)
+    int64_t sib_words = 0;              // sibling-ring words of the search (0: default)
     sgfim_opts opts{};
     // loaded data
     bool loaded = false;
@@ -629,10 +630,13 @@ int launch_round(sgfim_ctx* c, SearchState* S)
 int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap, int32_t d, int32_t dw,
                 int32_t bmax, int32_t chunk, int32_t range, int64_t trace_cap, int64_t n_roots, int64_t cell_cap)
 {
+    // sibling ring: four rows per frontier record, 8 MB .. 2 GB, unless set
+    const int64_t sib_cap = c->sib_words > 0 ? c->sib_words
+                                             : std::min<int64_t>(std::max<int64_t>(cap * dw * 4, (int64_t)1 << 20), (int64_t)1 << 28);
     SearchBufs* B = c->SB;
     if (B) {
         const SgSearchDev& D = B->D;
-        const bool fits = D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
+        const bool fits = D.sib_cap == sib_cap && D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
                           D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.chunk_range == range && D.trace_cap == trace_cap &&
                           B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + 2 * D.newborn_cap;
         if (fits) return SGFIM_OK;
@@ -652,6 +656,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     D.count_grid = sg_count_group_blocks_per_cu(chunk) * c->n_cu;   // one persistent wave of workgroups
     D.trace_cap = trace_cap;
     D.cell_cap = cell_cap;
+    D.sib_cap = sib_cap;
     B->roots_cap = std::max<int64_t>(n_roots, 1);
     auto& V = B->bufs;
     int rc;
@@ -670,6 +675,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.feat, (size_t)newborn_cap * SG_FEAT)) ||
         (rc = dev_alloc_t(c, V, &D.scores, (size_t)newborn_cap)) ||
         (rc = dev_alloc_t(c, V, &D.ctl, CTL_SLOTS)) ||
+        (rc = dev_alloc_t(c, V, &D.sib, (size_t)sib_cap)) || (rc = dev_alloc_t(c, V, &D.sib_off, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &B->d_roots, (size_t)B->roots_cap * 2))) {
         bufs_free(c);
         return rc;
@@ -1108,6 +1114,15 @@ int sgfim_set_tile(sgfim_ctx* c, int32_t chunk_words, int32_t chunk_range)
     if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "tile: an anytime search is in progress");
     c->tile_cw = chunk_words;
     c->tile_cr = chunk_range;
+    return SGFIM_OK;
+}
+
+int sgfim_set_sib_ring(sgfim_ctx* c, int64_t words)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (words < 0) return fail(c, SGFIM_E_INVAL, "sibling ring: words must be >= 0");
+    if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "sibling ring: an anytime search is in progress");
+    c->sib_words = words;
     return SGFIM_OK;
 }
 

@@ -136,6 +136,30 @@ __device__ __forceinline__ float sg_wave_fsum(float v)
     return v;
 }
 
+// Records whose rows span more chunk ranges than this get no sibling block: each counting
+// unit (one chunk range) reloads its block's rows, which pays on short rows, where a search
+// is bound by the AND + popcount, and not on long ones, where the unit streams the rows
+// from HBM (10M transactions at 2,048 items: 153 ranges, and fewer MFIs in a time budget).
+#define SG_SIB_RANGES 16
+
+// A sibling block at absolute offset off is intact while no later write has wrapped onto
+// it: every word written so far lies below off + sib_cap.
+__device__ __forceinline__ bool sg_sib_live(const SgSearchDev& S, int64_t off)
+{
+    return off >= 0 && S.ctl[CTL_SIB_NEXT] - off <= S.sib_cap;
+}
+
+// Offset of child e's row F(X u {e}) in the sibling block of X at off: the block's first
+// row is U(X), and children are the items of U(X) in ascending order. One wave.
+__device__ __forceinline__ int64_t sg_sib_row(const SgSearchDev& S, int64_t off, int32_t e, int dw)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t h = lane < dw ? S.sib[(off + lane) % S.sib_cap] : 0ull;
+    const int w = e >> 6;
+    const uint64_t part = lane < w ? h : (lane == w ? (h & ((1ull << (e & 63)) - 1ull)) : 0ull);
+    return off + (int64_t)(1 + sg_wave_isum(__popcll(part))) * S.dw_max;
+}
+
 // Write record X (items px[0..k), support, tail bound `last`, extensions U in LDS) to
 // slot. Called by all 64 lanes of a wave (dw <= 64 words of U, k <= 32 items: one per
 // lane); lane 0 writes the metadata and the features.
@@ -143,7 +167,7 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
                                const int32_t* __restrict__ pair, int32_t slot, const uint16_t* px, int k,
                                int32_t sup, int32_t last, int seg, const uint64_t* U, int32_t sup_parent,
                                int32_t nf_parent, int32_t ntail_parent, int parent_is_root, float* feat_out,
-                               int64_t key, int64_t parent_key)
+                               int64_t key, int64_t parent_key, int64_t sib)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t v = lane < sg.dw ? U[lane] : 0ull;
@@ -176,6 +200,7 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     m.nblk = (nt + SG_CB - 1) / SG_CB;
     m.nrange = sg_ranges(sg.words, S.chunk_words, S.chunk_range);
     m.score = 0.0f;
+    m.sib = sib;
     S.meta[slot] = m;
     S.key[slot] = key;
     S.pkey[slot] = parent_key;
@@ -358,7 +383,7 @@ __global__ __launch_bounds__(SG_SEED_WAVES * 64) void k_seed(SgSearchDev S, cons
         slot = __shfl(slot, 0, 64);
         if (slot >= 0) {
             float* fo = S.feat_on ? S.feat + nb * SG_FEAT : nullptr;
-            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, s_key + rank, -1);
+            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, s_key + rank, -1, -1);
             if (lane == 0) S.newborn[nb] = slot;
         }
     }
@@ -401,7 +426,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
 {
     __shared__ int32_t take[SG_NBUCKETS], tops[SG_NBUCKETS], ends[SG_NBUCKETS];
     __shared__ int64_t wsum[SEL_THREADS / 64];
-    __shared__ int64_t s_budget, tot[4];
+    __shared__ int64_t s_budget, tot[5];
     __shared__ int32_t s_total, s_cut, s_maxr;
     const int tid = threadIdx.x;
     if (tid < SG_NBUCKETS) tops[tid] = (int32_t)S.ctl[CTL_BUCKET_TOP0 + tid];
@@ -446,7 +471,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             nid[q] = S.bucket[(int64_t)b * S.cap + (tops[b] - 1 - pos)];
         }
     }
-    int64_t su = 0, sc = 0, scell = 0, sunit = 0;
+    int64_t su = 0, sc = 0, scell = 0, sunit = 0, ssib = 0;
     int32_t mr = 0;   // longest window (row-chunk ranges) among the taken records
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -458,6 +483,9 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             sc += m.ntail;
             scell += (int64_t)m.ncand * m.ntail;
             sunit += m.nblk;
+            const bool blk = m.nrange <= SG_SIB_RANGES;
+            if (blk) ssib += (int64_t)(1 + m.ntail) * S.dw_max;
+            S.sib_off[i] = blk ? 0 : -1;   // parked: whether the record gets a block
             S.batch[i] = nid[q];
             S.coff[i] = m.ncand;
             S.choff[i] = m.ntail;
@@ -468,6 +496,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
     int64_t ec = sg_block_excl_scan(sc, wsum, &tot[1]);
     int64_t ecell = sg_block_excl_scan(scell, wsum, &tot[2]);
     int64_t eunit = sg_block_excl_scan(sunit, wsum, &tot[3]);
+    int64_t esib = sg_block_excl_scan(ssib, wsum, &tot[4]);
+    const int64_t sib_base = S.ctl[CTL_SIB_NEXT];
     // pass 2: offsets; the cut is the first record past a budget (offsets only grow)
     const int64_t budget = s_budget;
 #pragma unroll
@@ -482,6 +512,10 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             S.choff[i] = (int32_t)ec;
             S.celloff[i] = ecell;
             S.unit_off[i] = eunit;
+            // a block that would wrap onto another of this round gets none
+            const int64_t sz = S.sib_off[i] >= 0 ? (int64_t)(1 + nt) * S.dw_max : 0;
+            S.sib_off[i] = sz > 0 && esib + sz <= S.sib_cap ? sib_base + esib : -1;
+            esib += sz;
             eu += nu;
             ec += nt;
             ecell += cells;
@@ -515,6 +549,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         S.ctl[CTL_BLOCKS] = S.unit_off[cut];
         S.ctl[CTL_UNITS] = S.unit_off[cut] * (int64_t)s_maxr;
         S.ctl[CTL_NODES] += nch;
+        S.ctl[CTL_SIB_NEXT] = S.ctl[CTL_SIB_NEXT] + tot[4];   // blocks of records past the cut stay unused
     }
 }
 
@@ -536,6 +571,8 @@ __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment
     const SgNodeMeta m = S.meta[slot];
     const int dw = segs[m.seg].dw;
     const uint64_t v = lane < dw ? S.cand[(int64_t)slot * S.dw_max + lane] : 0ull;
+    const int64_t sb = S.sib_off[b];
+    if (sb >= 0 && lane < S.dw_max) S.sib[(sb + lane) % S.sib_cap] = v;   // U(X) heads X's sibling block
     const int c = __popcll(v);
     int x = c;
     for (int o = 1; o < 64; o <<= 1) {
@@ -721,11 +758,20 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         if (threadIdx.x < m.k) s_items[threadIdx.x] = S.items[(int64_t)slot * SG_MAX_DEPTH + threadIdx.x];
         if (threadIdx.x < ncb) s_child[threadIdx.x] = S.clist[u0 + j0 + threadIdx.x];
         __syncthreads();
-        // set-up: children's ok rows, then the list of needed candidates
-        const uint64_t* okb = ok + sg.ok_base;
-        for (int t = threadIdx.x; t < ncb * sg.dw; t += blockDim.x) {
-            const int j = t / sg.dw, w = t - j * sg.dw;
-            okr[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
+        // set-up: the children's need rows, then the list of needed candidates. A child e
+        // needs f if f is in F(P u {e}) of the parent P's sibling block (while it is intact),
+        // else if the pair (e, f) is frequent.
+        if (sg_sib_live(S, m.sib)) {
+            for (int j = wave; j < ncb; j += NW) {
+                const int64_t r = sg_sib_row(S, m.sib, s_child[j], sg.dw);
+                if (lane < sg.dw) okr[j * 64 + lane] = S.sib[(r + lane) % S.sib_cap];
+            }
+        } else {
+            const uint64_t* okb = ok + sg.ok_base;
+            for (int t = threadIdx.x; t < ncb * sg.dw; t += blockDim.x) {
+                const int j = t / sg.dw, w = t - j * sg.dw;
+                okr[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
+            }
         }
         __syncthreads();
         // Candidate i needs counting for child j0 + j when the pair is frequent, except that
@@ -839,6 +885,7 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
 {
     __shared__ uint64_t sF[SG_EXP_WAVES][64];
     __shared__ uint64_t sE[SG_EXP_WAVES][64];
+    __shared__ uint64_t sN[SG_EXP_WAVES][64];   // the child's need row (the counting tile's rule)
     __shared__ uint16_t sX[SG_EXP_WAVES][SG_MAX_DEPTH];
     __shared__ int32_t s_act[SG_EXP_WAVES];
     __shared__ int64_t s_ring, s_nb, s_top, s_key;
@@ -864,9 +911,17 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             e = S.clist[u0 + j];
             sup_e = S.ccount[S.choff[g] + j];
             const int32_t* crow = S.cells + S.celloff[g] + (int64_t)j * nu;
-            const uint64_t* oke = ok + sg.ok_base + (int64_t)e * sg.dw;
+            if (sg_sib_live(S, m.sib)) {
+                const int64_t r = sg_sib_row(S, m.sib, e, sg.dw);
+                sN[wv][lane] = lane < sg.dw ? S.sib[(r + lane) % S.sib_cap] : 0ull;
+            } else {
+                sN[wv][lane] = lane < sg.dw ? ok[sg.ok_base + (int64_t)e * sg.dw + lane] : 0ull;
+            }
             if (lane < m.k) sX[wv][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
             if (lane == 0) sX[wv][m.k] = (uint16_t)e;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             int needed = 0;
             for (int i = lane; i < nu; i += 64) {
                 if (i == j) continue;
@@ -874,7 +929,7 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
                 // a later child's support with e sits in that child's row (counted once)
                 const bool mirror = i < m.ntail && i > j;
                 const int32_t cnt = mirror ? crow[(int64_t)(i - j) * nu + j] : crow[i];
-                if (!mirror && ((oke[f >> 6] >> (f & 63)) & 1ull)) needed++;
+                if (!mirror && ((sN[wv][f >> 6] >> (f & 63)) & 1ull)) needed++;
                 if (cnt >= sg.thr) {
                     atomicOr((unsigned long long*)&sF[wv][f >> 6], 1ull << (f & 63));
                     if (cnt == sup_e) atomicOr((unsigned long long*)&sE[wv][f >> 6], 1ull << (f & 63));
@@ -888,6 +943,10 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // F(X u {e}) is child e's row of X's sibling block (before the node step takes
+            // the equal-support items out of it)
+            const int64_t sb = S.sib_off[g];
+            if (sb >= 0 && lane < S.dw_max) S.sib[(sb + (int64_t)(1 + j) * S.dw_max + lane) % S.sib_cap] = sF[wv][lane];
             k = m.k + 1;
             act = sg_node_decide(S, sg, sF[wv], sE[wv], sX[wv], k, e);
         }
@@ -923,7 +982,8 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
                 if (nslot >= 0) {
                     float* fo = S.feat_on ? S.feat + nbi * SG_FEAT : nullptr;
                     sg_make_record(S, sg, sup_all + seg_item_base[m.seg], pair, nslot, sX[wv], k, sup_e, e, m.seg,
-                                   sF[wv], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot]);
+                                   sF[wv], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot],
+                                   S.sib_off[g]);
                     if (lane == 0) S.newborn[nbi] = nslot;
                 }
             }

@@ -51,6 +51,7 @@ struct SgNodeMeta {
     int32_t nblk;       // blocks of children in the record's counting tiles
     int32_t nrange;     // row-chunk ranges of the record's window
     float score;        // priority score
+    int64_t sib;        // the parent's sibling block in the sibling ring (-1: none, a root)
 };
 
 // Launchers (kernels_*.hip); all launch on the given stream and return hipError_t.
@@ -115,6 +116,15 @@ struct SgSearchDev {
     int32_t* chrec;      // [newborn_cap] batch record of each child of the round
     int32_t* blkrec;     // [newborn_cap / SG_CB + bmax] batch record of each child block
     int64_t cell_cap;
+    // Sibling ring: a record X expanded in a round writes a block of (1 + ntail) rows of
+    // dw_max words: its U, then F(X u {e}) for each child e. A child record Y of X keeps the
+    // block's offset; when Y is counted, child e of Y needs candidate f only if f is in
+    // F(X u {e}) (|T(Y u {e, f})| <= |T(X u {e, f})|), a tighter test than the pair matrix.
+    // Offsets are absolute positions in an endless stream of words, stored modulo sib_cap;
+    // a block is readable while it lies in the latest sib_cap words written.
+    uint64_t* sib;       // [sib_cap]
+    int64_t* sib_off;    // [bmax + 1] block of each batch record (-1: none)
+    int64_t sib_cap;
     // children born in the current round
     int32_t* newborn;    // [newborn_cap]
     float* feat;         // [newborn_cap][SG_FEAT]
@@ -164,7 +174,8 @@ enum SgCtl {
     CTL_T1 = 16 + SG_NBUCKETS, // latest wall_clock64() at which a counting workgroup finished
     CTL_CHILDREN,         // children to expand in the current batch
     CTL_CELLS,            // count-matrix cells of the current batch
-    CTL_SLOTS = 19 + SG_NBUCKETS
+    CTL_SIB_NEXT,         // sibling-ring words written so far (end of the current round's blocks)
+    CTL_SLOTS = 20 + SG_NBUCKETS
 };
 
 struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)

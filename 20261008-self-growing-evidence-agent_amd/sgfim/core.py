@@ -308,6 +308,11 @@ class Context:
         """Counting-tile shape of later searches (0 = default); results do not depend on it."""
         self._check(self._L.sgfim_set_tile(self._h, int(chunk_words), int(chunk_range)), "set_tile")
 
+    def set_sib_ring(self, words: int = 0):
+        """Words of the sibling ring of later searches (0 = default); results do not depend
+        on it, a larger ring prunes more counts on deep searches."""
+        self._check(self._L.sgfim_set_sib_ring(self._h, int(words)), "set_sib_ring")
+
     def tile(self) -> tuple:
         """(chunk_words, chunk_range) the next search uses."""
         cw, cr = C.c_int32(), C.c_int32()
