@@ -395,26 +395,34 @@ __global__ void k_zero_newborn(SgSearchDev S)
 }
 
 // ------------------------------------------------------------------ select
-// Block-wide exclusive scan of one int64 per thread (SEL_THREADS threads).
-__device__ __forceinline__ int64_t sg_block_excl_scan(int64_t v, int64_t* wsum, int64_t* total)
+// Block-wide exclusive scans of N int64s per thread at once (SEL_THREADS threads): one
+// pass of wave scans, wave totals scanned by wave 0, two barriers for all N.
+template <int N>
+__device__ __forceinline__ void sg_block_excl_scan(int64_t (&v)[N], int64_t (*wsum)[SEL_THREADS / 64], int64_t* total)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int64_t x = v;
+    int64_t x[N];
+#pragma unroll
+    for (int n = 0; n < N; n++) x[n] = v[n];
     for (int o = 1; o < 64; o <<= 1) {
-        const int64_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+            const int64_t y = __shfl_up(x[n], o, 64);
+            if (lane >= o) x[n] += y;
+        }
     }
-    if (lane == 63) wsum[wv] = x;
+    if (lane == 63)
+#pragma unroll
+        for (int n = 0; n < N; n++) wsum[n][wv] = x[n];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < N) {   // one thread per scan runs over the SEL_THREADS / 64 wave totals
         int64_t a = 0;
-        for (int w = 0; w < SEL_THREADS / 64; w++) { const int64_t t = wsum[w]; wsum[w] = a; a += t; }
-        *total = a;
+        for (int w = 0; w < SEL_THREADS / 64; w++) { const int64_t t = wsum[threadIdx.x][w]; wsum[threadIdx.x][w] = a; a += t; }
+        total[threadIdx.x] = a;
     }
     __syncthreads();
-    const int64_t r = wsum[wv] + x - v;
-    __syncthreads();
-    return r;
+#pragma unroll
+    for (int n = 0; n < N; n++) v[n] = wsum[n][wv] + x[n] - v[n];
 }
 
 // Take up to bmax records from the highest buckets, as many as fit the budgets: new
@@ -425,7 +433,7 @@ __device__ __forceinline__ int64_t sg_block_excl_scan(int64_t v, int64_t* wsum, 
 __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
 {
     __shared__ int32_t take[SG_NBUCKETS], tops[SG_NBUCKETS], ends[SG_NBUCKETS];
-    __shared__ int64_t wsum[SEL_THREADS / 64];
+    __shared__ int64_t wsum[5][SEL_THREADS / 64];
     __shared__ int64_t s_budget, tot[5];
     __shared__ int32_t s_total, s_cut, s_maxr;
     const int tid = threadIdx.x;
@@ -492,11 +500,9 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             S.unit_off[i] = m.nblk;
         }
     }
-    int64_t eu = sg_block_excl_scan(su, wsum, &tot[0]);
-    int64_t ec = sg_block_excl_scan(sc, wsum, &tot[1]);
-    int64_t ecell = sg_block_excl_scan(scell, wsum, &tot[2]);
-    int64_t eunit = sg_block_excl_scan(sunit, wsum, &tot[3]);
-    int64_t esib = sg_block_excl_scan(ssib, wsum, &tot[4]);
+    int64_t ex[5] = {su, sc, scell, sunit, ssib};
+    sg_block_excl_scan<5>(ex, wsum, tot);
+    int64_t eu = ex[0], ec = ex[1], ecell = ex[2], eunit = ex[3], esib = ex[4];
     const int64_t sib_base = S.ctl[CTL_SIB_NEXT];
     // pass 2: offsets; the cut is the first record past a budget (offsets only grow)
     const int64_t budget = s_budget;
