@@ -135,6 +135,8 @@ struct sgfim_ctx {
     bool pair_timed = false;
     int32_t* h_pair_pin = nullptr;      // pinned staging buffer for pair-matrix read-back (dpad x dpad)
     int64_t h_pair_pin_n = 0;
+    char* h_stage = nullptr;            // pinned staging of a search's uploads (segments, item bases, roots)
+    size_t h_stage_cap = 0;
     uint64_t* d_ok = nullptr;           // pair-ok masks, one block per search segment (vseg_cap blocks)
     SgSegment* d_seg = nullptr;         // data segments (windows) + one for count_batch
     int64_t* d_seg_item_base = nullptr;
@@ -242,6 +244,24 @@ void search_free(sgfim_ctx* c)
 {
     delete c->S;
     c->S = nullptr;
+}
+
+// Pinned staging of at least `bytes` for a search's uploads. Called after the search's
+// stream is synchronized, so earlier uploads from it have completed.
+int stage_reserve(sgfim_ctx* c, size_t bytes)
+{
+    if (bytes <= c->h_stage_cap) return SGFIM_OK;
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    c->h_stage_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, (size_t)1 << 16);
+    if (hipHostMalloc((void**)&c->h_stage, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->h_stage = nullptr;
+        return fail(c, SGFIM_E_NOMEM, "pinned staging allocation failed");
+    }
+    c->h_stage_cap = cap;
+    return SGFIM_OK;
 }
 
 // Drop captured round g: its kernel arguments name the loaded data's buffers.
@@ -388,7 +408,7 @@ int ensure_pair_u(sgfim_ctx* c, const Universe& U)
     } else {
         int64_t maxw = 0;
         for (int32_t w = 0; w < c->nseg; w++) maxw = std::max(maxw, U.segs[w].words);
-        HIPCHK(c, hipMemsetAsync(U.pair, 0, (size_t)U.dpad * U.dpad * c->nseg * 4, c->stream));
+        HIPCHK(c, sg_launch_zero32(U.pair, (int64_t)U.dpad * U.dpad * c->nseg, c->stream));
         HIPCHK(c, sg_launch_pair_mfma_segs(U.rows, U.d_seg, c->nseg, U.dpad, maxw, U.pair, c->stream));
     }
     c->pair_timed = timed && hipEventRecord(c->pair_ev[1], c->stream) == hipSuccess;
@@ -771,6 +791,7 @@ void sgfim_destroy(sgfim_ctx* c)
     bufs_free(c);
     dev_free_all(c, c->scorer_bufs);
     if (c->h_pair_pin) (void)hipHostFree(c->h_pair_pin);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     for (int i = 0; i < 2; i++)
         if (c->pair_ev[i]) (void)hipEventDestroy(c->pair_ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1609,8 +1630,14 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         vs[v].ok_base = (int64_t)v * d * dw;
         vib[v] = (int64_t)data_seg(v) * U.dpad;
     }
-    HIPCHK(c, hipMemcpyAsync(c->d_vseg, vs.data(), (size_t)nv * sizeof(SgSegment), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_vseg_ibase, vib.data(), (size_t)nv * 8, hipMemcpyHostToDevice, c->stream));
+    // uploads go through pinned staging: [segments | item bases | roots], 256-byte aligned
+    const size_t st_seg = 0, st_ib = (st_seg + (size_t)nv * sizeof(SgSegment) + 255) / 256 * 256;
+    const size_t st_roots = (st_ib + (size_t)nv * 8 + 255) / 256 * 256;
+    if ((rc = stage_reserve(c, st_roots + rp.size() * 4))) return rc;
+    memcpy(c->h_stage + st_seg, vs.data(), (size_t)nv * sizeof(SgSegment));
+    memcpy(c->h_stage + st_ib, vib.data(), (size_t)nv * 8);
+    HIPCHK(c, hipMemcpyAsync(c->d_vseg, c->h_stage + st_seg, (size_t)nv * sizeof(SgSegment), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_vseg_ibase, c->h_stage + st_ib, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, sg_launch_pair_ok(U.pair, c->d_vseg, nv, d, U.ok, c->stream));
     // the seed turns every root into at most one frontier record
     const int64_t seed_records = n_roots;
@@ -1645,7 +1672,8 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     HIPCHK(c, sg_search_init(D, c->stream));
     S->n_roots = (int32_t)n_roots;
     S->roots = std::move(rp);
-    HIPCHK(c, hipMemcpyAsync(S->B->d_roots, S->roots.data(), S->roots.size() * 4, hipMemcpyHostToDevice, c->stream));
+    memcpy(c->h_stage + st_roots, S->roots.data(), S->roots.size() * 4);
+    HIPCHK(c, hipMemcpyAsync(S->B->d_roots, c->h_stage + st_roots, S->roots.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, sg_search_seed(D, c->d_vseg, U.pair, U.ok, S->B->d_roots, S->n_roots, U.sup, c->d_vseg_ibase,
                              S->order_mode, &c->scorer, c->stream));
     // no synchronisation here: the seed's MFIs, errors and frontier are read after the

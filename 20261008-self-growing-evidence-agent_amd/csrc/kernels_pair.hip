@@ -200,10 +200,29 @@ static int32_t pair_splits(int32_t tiles, int64_t words)
     return (int32_t)(splits < 1 ? 1 : splits);
 }
 
+// Zero n int32s (16-byte stores, grid-stride): the split-N tiles add into the matrix. A
+// kernel launch costs the host less than a memset call, and it may sit in a captured graph.
+__global__ __launch_bounds__(256) void k_zero32(int32_t* __restrict__ p, int64_t n)
+{
+    const int64_t n4 = n / 4;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        ((int4*)p)[i] = int4{0, 0, 0, 0};
+    for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+hipError_t sg_launch_zero32(int32_t* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(1024, (n / 4 + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_zero32, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
                                int32_t* pair, hipStream_t s)
 {
-    hipError_t e = hipMemsetAsync(pair, 0, sizeof(int32_t) * (size_t)dpad * dpad, s);
+    hipError_t e = sg_launch_zero32(pair, (int64_t)dpad * dpad, s);
     if (e != hipSuccess || W <= 0) return e;
     SgSegment one{};
     one.row_base = 0;

@@ -71,6 +71,8 @@ hipError_t sg_launch_copy_probe(const void* src, void* dst, int64_t bytes, hipSt
 
 hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
                                int32_t* pair, hipStream_t s);
+// p[0 .. n) = 0 with a kernel (16-byte stores).
+hipError_t sg_launch_zero32(int32_t* p, int64_t n, hipStream_t s);
 hipError_t sg_launch_pair_valu(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
                                int32_t* pair, hipStream_t s);
 // Batched pair matrices over segments (one launch for all windows).
