@@ -696,6 +696,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.scores, (size_t)newborn_cap)) ||
         (rc = dev_alloc_t(c, V, &D.ctl, CTL_SLOTS)) ||
         (rc = dev_alloc_t(c, V, &D.sib, (size_t)sib_cap)) || (rc = dev_alloc_t(c, V, &D.sib_off, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.need, (size_t)newborn_cap * dw)) ||
         (rc = dev_alloc_t(c, V, &B->d_roots, (size_t)B->roots_cap * 2))) {
         bufs_free(c);
         return rc;

@@ -136,12 +136,6 @@ __device__ __forceinline__ float sg_wave_fsum(float v)
     return v;
 }
 
-// Records whose rows span more chunk ranges than this get no sibling block: each counting
-// unit (one chunk range) reloads its block's rows, which pays on short rows, where a search
-// is bound by the AND + popcount, and not on long ones, where the unit streams the rows
-// from HBM (10M transactions at 2,048 items: 153 ranges, and fewer MFIs in a time budget).
-#define SG_SIB_RANGES 16
-
 // A sibling block at absolute offset off is intact while no later write has wrapped onto
 // it: every word written so far lies below off + sib_cap.
 __device__ __forceinline__ bool sg_sib_live(const SgSearchDev& S, int64_t off)
@@ -491,9 +485,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             sc += m.ntail;
             scell += (int64_t)m.ncand * m.ntail;
             sunit += m.nblk;
-            const bool blk = m.nrange <= SG_SIB_RANGES;
-            if (blk) ssib += (int64_t)(1 + m.ntail) * S.dw_max;
-            S.sib_off[i] = blk ? 0 : -1;   // parked: whether the record gets a block
+            ssib += (int64_t)(1 + m.ntail) * S.dw_max;
             S.batch[i] = nid[q];
             S.coff[i] = m.ncand;
             S.choff[i] = m.ntail;
@@ -519,8 +511,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             S.celloff[i] = ecell;
             S.unit_off[i] = eunit;
             // a block that would wrap onto another of this round gets none
-            const int64_t sz = S.sib_off[i] >= 0 ? (int64_t)(1 + nt) * S.dw_max : 0;
-            S.sib_off[i] = sz > 0 && esib + sz <= S.sib_cap ? sib_base + esib : -1;
+            const int64_t sz = (int64_t)(1 + nt) * S.dw_max;
+            S.sib_off[i] = esib + sz <= S.sib_cap ? sib_base + esib : -1;
             esib += sz;
             eu += nu;
             ec += nt;
@@ -599,6 +591,32 @@ __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment
     }
     const int64_t u0 = S.unit_off[b];
     for (int i = lane; i < m.nblk; i += 64) S.blkrec[u0 + i] = b;
+}
+
+// ------------------------------------------------------------------ need rows
+// One wave per child e of the round's records: the row of candidates e needs, F(P u {e})
+// from the record's parent P's sibling block while it is intact, else e's pair-ok row.
+// Every counting unit of the child's block and the child's expansion read it from here.
+__global__ __launch_bounds__(256) void k_need_rows(SgSearchDev S, const SgSegment* __restrict__ segs,
+                                                  const uint64_t* __restrict__ ok)
+{
+    const int64_t nchild = S.ctl[CTL_CHILDREN];
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nchild; q += stride) {   // wave-uniform
+        const int g = S.chrec[q];
+        const SgNodeMeta m = S.meta[S.batch[g]];
+        const SgSegment sg = segs[m.seg];
+        const int32_t e = S.clist[S.coff[g] + (int)(q - S.choff[g])];
+        uint64_t v;
+        if (sg_sib_live(S, m.sib)) {
+            const int64_t r = sg_sib_row(S, m.sib, e, sg.dw);
+            v = lane < sg.dw ? S.sib[(r + lane) % S.sib_cap] : 0ull;
+        } else {
+            v = lane < sg.dw ? ok[sg.ok_base + (int64_t)e * sg.dw + lane] : 0ull;
+        }
+        if (lane < S.dw_max) S.need[q * S.dw_max + lane] = v;
+    }
 }
 
 // ------------------------------------------------------------------ counting tiles
@@ -764,19 +782,12 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         if (threadIdx.x < m.k) s_items[threadIdx.x] = S.items[(int64_t)slot * SG_MAX_DEPTH + threadIdx.x];
         if (threadIdx.x < ncb) s_child[threadIdx.x] = S.clist[u0 + j0 + threadIdx.x];
         __syncthreads();
-        // set-up: the children's need rows, then the list of needed candidates. A child e
-        // needs f if f is in F(P u {e}) of the parent P's sibling block (while it is intact),
-        // else if the pair (e, f) is frequent.
-        if (sg_sib_live(S, m.sib)) {
-            for (int j = wave; j < ncb; j += NW) {
-                const int64_t r = sg_sib_row(S, m.sib, s_child[j], sg.dw);
-                if (lane < sg.dw) okr[j * 64 + lane] = S.sib[(r + lane) % S.sib_cap];
-            }
-        } else {
-            const uint64_t* okb = ok + sg.ok_base;
+        // set-up: the children's need rows (k_need_rows), then the list of needed candidates
+        {
+            const uint64_t* nr = S.need + (int64_t)(S.choff[g] + j0) * S.dw_max;
             for (int t = threadIdx.x; t < ncb * sg.dw; t += blockDim.x) {
                 const int j = t / sg.dw, w = t - j * sg.dw;
-                okr[j * 64 + w] = okb[(int64_t)s_child[j] * sg.dw + w];
+                okr[j * 64 + w] = nr[(int64_t)j * S.dw_max + w];
             }
         }
         __syncthreads();
@@ -917,12 +928,7 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             e = S.clist[u0 + j];
             sup_e = S.ccount[S.choff[g] + j];
             const int32_t* crow = S.cells + S.celloff[g] + (int64_t)j * nu;
-            if (sg_sib_live(S, m.sib)) {
-                const int64_t r = sg_sib_row(S, m.sib, e, sg.dw);
-                sN[wv][lane] = lane < sg.dw ? S.sib[(r + lane) % S.sib_cap] : 0ull;
-            } else {
-                sN[wv][lane] = lane < sg.dw ? ok[sg.ok_base + (int64_t)e * sg.dw + lane] : 0ull;
-            }
+            sN[wv][lane] = lane < sg.dw ? S.need[q * S.dw_max + lane] : 0ull;   // the counting tile's rule
             if (lane < m.k) sX[wv][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
             if (lane == 0) sX[wv][m.k] = (uint16_t)e;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1134,6 +1140,7 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     hipError_t e;
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
+    hipLaunchKernelGGL(k_need_rows, dim3(1024), dim3(256), 0, s, S, segs, ok);
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
     switch (S.chunk_words) {
     case 128: hipLaunchKernelGGL(k_count_group<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;

@@ -127,6 +127,7 @@ struct SgSearchDev {
     uint64_t* sib;       // [sib_cap]
     int64_t* sib_off;    // [bmax + 1] block of each batch record (-1: none)
     int64_t sib_cap;
+    uint64_t* need;      // [newborn_cap][dw_max] need row of each child of the round
     // children born in the current round
     int32_t* newborn;    // [newborn_cap]
     float* feat;         // [newborn_cap][SG_FEAT]

@@ -176,9 +176,8 @@ int sgfim_set_tile(sgfim_ctx* ctx, int32_t chunk_words, int32_t chunk_range);
  * children's frequent-extension masks F(X u {e}) to the ring; when a child record is
  * counted, child e of it needs candidate f only if f is in F(X u {e}), a tighter test than
  * the pair matrix, exact since |T(Y u {e, f})| <= |T(X u {e, f})| for Y >= X. A block that
- * later writes have wrapped onto falls back to the pair test, and records whose rows span
- * more than 16 counting chunk ranges write no block. Results do not depend on it; a larger
- * ring prunes more counts on deep searches (1 = none). No reference counterpart. */
+ * later writes have wrapped onto falls back to the pair test. Results do not depend on it;
+ * a larger ring prunes more counts on deep searches (1 = none). No reference counterpart. */
 int sgfim_set_sib_ring(sgfim_ctx* ctx, int64_t words);
 int sgfim_get_tile(const sgfim_ctx* ctx, int32_t* chunk_words, int32_t* chunk_range);
 /* The context's device: its gcnArchName (NUL-terminated, at most len - 1 characters) and
