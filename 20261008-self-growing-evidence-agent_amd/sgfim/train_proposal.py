@@ -27,7 +27,7 @@ from concurrent.futures import ProcessPoolExecutor
 
 import numpy as np
 
-from . import gen, learned
+from . import autotune, gen, learned
 from ._lib import ORDER_LEARNED
 from .core import Context
 
@@ -172,6 +172,7 @@ def evaluate(ctx: Context, args, report: dict):
     warm = True
     for k, (s, (off, items)) in enumerate(streams(seeds, args.n_txn, args.workers)):
         ctx.load_csr(off, items)
+        autotune.apply(ctx, 1)   # the counting tile tuned for this device and input shape
         thr = gen.min_count(args.floor, ctx.n_txn)
         if warm:  # first held-out stream: warm the captured rounds of both orders
             anytime_curve(ctx, thr, True)
