@@ -32,7 +32,7 @@ VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 # passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
 # what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
 # profiles/k_count_group_workload2_pmc.md holds the commands and the table.
-PMC_COUNT_KERNEL = {"fetch_kib": 131481.0, "write_kib": 73592.8, "fetch_scale": 2.0, "write_scale": 1.0}
+PMC_COUNT_KERNEL = {"fetch_kib": 116982.0, "write_kib": 61325.3, "fetch_scale": 2.0, "write_scale": 1.0}
 SUPPORTS = (0.10, 0.05, 0.02)
 
 
