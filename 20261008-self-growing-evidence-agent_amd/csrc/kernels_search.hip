@@ -18,13 +18,13 @@
 // candidate row chunk is loaded once and ANDed with up to 16 child prefixes held in
 // LDS. By downward closure every frequent extension of X u {e} lies in U, so the
 // reference's local maximality test (frequent-extension count == |X u {e}|) is
-// exact on the tile's row; cells whose pair (e, f) is infrequent are skipped and
-// stay 0 (no superset of an infrequent pair is frequent).
-// Parent-equivalence pruning (exact): see sg_expand_node.
+// exact on the tile's row; cells that cannot be frequent are skipped and stay 0: the
+// pair (e, f) is infrequent, or f is not in F(P u {e}) of X's parent P (the sibling ring).
+// Parent-equivalence pruning (exact): see sg_node_decide.
 //
-// One round = k_select -> k_candlist -> k_count_group -> k_expand_children -> [score]
-// -> k_commit; every launch has a fixed shape and reads its sizes from the control
-// block, so the round is captured as one hipGraph.
+// One round = k_select -> k_candlist -> k_need_rows -> k_count_group -> k_expand_children
+// -> [score] -> k_commit; every launch has a fixed shape and reads its sizes from the
+// control block, so the round is captured as one hipGraph.
 #include "count_core.h"
 
 #define SEL_THREADS 1024
@@ -717,15 +717,16 @@ __device__ __forceinline__ void sg_step_plan(const uint32_t* s_cand, const int32
 }
 
 // Unit (record g, block of up to SG_CB children, range of up to chunk_range row chunks).
-// Set-up, once per unit: the children's pair-ok rows give each candidate f of U the
-// mask of children e that need it (pair (e, f) frequent, f != e); candidates no child
-// needs are dropped, the others listed in LDS. Then per chunk: the prefix of X and the
+// Set-up, once per unit: the children's need rows (k_need_rows) give each candidate f of
+// U the mask of children e that need it (f in F(P u {e}) of the parent's sibling block,
+// or the pair (e, f) frequent; f != e); candidates no child needs are dropped, the others
+// listed in LDS. Then per chunk: the prefix of X and the
 // prefixes of the children go to LDS, and each wave takes SG_RC listed candidates at a
 // time (sg_tile_step); lane (j, r) = (lane >> 2, lane & 3) of a step holds whether
 // child j needs candidate r, so the ballot keeps child j's four bits at 4j. One integer
 // atomic per needed (child, candidate, chunk). Skipped cells stay 0, which the
-// expansion reads as infrequent -- exact, since an infrequent pair has no frequent
-// superset. The child prefixes' popcounts give the children's supports.
+// expansion reads as infrequent -- exact, since a skipped cell's itemset has an
+// infrequent subset. The child prefixes' popcounts give the children's supports.
 // Registers are capped for 6 waves per SIMD (three workgroups per CU, as the LDS of a
 // 256-word chunk allows); 128-word chunks leave LDS for four workgroups, and ask for 8.
 template <int CW>
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     constexpr int NW = SG_TILE_THREADS / 64;
     static_assert(SG_RC * SG_CB == 64, "one lane per (child, candidate) pair of a wave step");
     __shared__ uint64_t pre[CW];             // T(X) over the chunk
-    __shared__ uint64_t cp[SG_CB][CW];       // child prefixes; the children's ok rows during set-up
+    __shared__ uint64_t cp[SG_CB][CW];       // child prefixes; the children's need rows during set-up
     __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
     __shared__ int64_t s_unit;
     __shared__ int32_t s_items[SG_MAX_DEPTH];
