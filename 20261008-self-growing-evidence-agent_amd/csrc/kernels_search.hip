@@ -426,25 +426,32 @@ __device__ __forceinline__ void sg_block_excl_scan(int64_t (&v)[N], int64_t (*ws
 // its own entries), so record metadata is loaded once.
 __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
 {
-    __shared__ int32_t take[SG_NBUCKETS], tops[SG_NBUCKETS], ends[SG_NBUCKETS];
+    __shared__ int32_t tops[SG_NBUCKETS], ends[SG_NBUCKETS];
     __shared__ int64_t wsum[5][SEL_THREADS / 64];
     __shared__ int64_t s_budget, tot[5];
     __shared__ int32_t s_total, s_cut, s_maxr;
     const int tid = threadIdx.x;
-    if (tid < SG_NBUCKETS) tops[tid] = (int32_t)S.ctl[CTL_BUCKET_TOP0 + tid];
-    __syncthreads();
-    if (tid == 0) {
-        // order o = 63 - b: the highest bucket first; ends[o] = records taken through o
-        int32_t taken = 0;
-        for (int o = 0; o < SG_NBUCKETS; o++) {
-            const int b = SG_NBUCKETS - 1 - o;
-            const int32_t t = min(tops[b], S.bmax - taken);
-            take[b] = t;
-            taken += t;
-            ends[o] = taken;
+    static_assert(SG_NBUCKETS == 64, "one lane per bucket");
+    if (tid < SG_NBUCKETS) {
+        // lane o holds order o = 63 - b, the highest bucket first; ends[o] = records taken
+        // through o = min(records in buckets of order <= o, bmax), an inclusive scan
+        const int o = tid, b = SG_NBUCKETS - 1 - o;
+        const int32_t t = (int32_t)S.ctl[CTL_BUCKET_TOP0 + b];
+        tops[b] = t;
+        int32_t cum = t;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t y = __shfl_up(cum, d, 64);
+            if (o >= d) cum += y;
         }
-        s_total = taken;
-        s_cut = taken;
+        const int32_t end = min(cum, S.bmax);
+        ends[o] = end;
+        const int32_t taken = __shfl(end, SG_NBUCKETS - 1, 64);
+        if (o == 0) {
+            s_total = taken;
+            s_cut = taken;
+        }
+    }
+    if (tid == 0) {
         s_maxr = 0;
         s_budget = min(S.ctl[CTL_FREE_TOP], S.newborn_cap);
         S.ctl[CTL_ROUND] += 1;   // MFIs of this round carry its number; the seed's carry 0
@@ -456,13 +463,16 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
     __syncthreads();
     const int total = s_total;
     constexpr int PER = SG_BMAX / SEL_THREADS;
+    // thread tid takes batch positions tid * pe .. tid * pe + pe - 1: contiguous, so the
+    // scans below stay in batch order, and spread over every thread of the block
+    const int pe = (total + SEL_THREADS - 1) / SEL_THREADS;
     // pass 1: record slots, then sizes (independent loads within each phase)
     int32_t nid[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int i = tid * PER + q;
+        const int i = tid * pe + q;
         nid[q] = -1;
-        if (i < total) {
+        if (q < pe && i < total) {
             int lo = 0, hi = SG_NBUCKETS - 1;   // first order o with ends[o] > i
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
@@ -477,7 +487,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
     int32_t mr = 0;   // longest window (row-chunk ranges) among the taken records
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int i = tid * PER + q;
+        const int i = tid * pe + q;
         if (nid[q] >= 0) {
             const SgNodeMeta m = S.meta[nid[q]];
             mr = max(mr, m.nrange);
@@ -500,7 +510,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
     const int64_t budget = s_budget;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int i = tid * PER + q;
+        const int i = tid * pe + q;
         if (nid[q] >= 0) {
             const int32_t nu = S.coff[i], nt = S.choff[i];
             const int64_t nun = S.unit_off[i];
