@@ -174,21 +174,20 @@ __global__ __launch_bounds__(256) void k_pair_valu(const uint64_t* __restrict__ 
     if (lane == 0) out[p] = (int32_t)acc;
 }
 
-// ok[i][w] bit b = pair(i, 64w+b) >= thr, for 64w+b < d.
-__global__ void k_pair_ok(const int32_t* __restrict__ pair, const SgSegment* __restrict__ segs,
-                          int32_t max_d, uint64_t* __restrict__ ok)
+// ok[i][w] bit b = pair(i, 64w+b) >= thr, for 64w+b < d. One wave per word: lane b
+// tests column 64w+b (one coalesced 256-byte read) and the ballot is the word.
+__global__ __launch_bounds__(256) void k_pair_ok(const int32_t* __restrict__ pair, const SgSegment* __restrict__ segs,
+                                                 int32_t max_d, uint64_t* __restrict__ ok)
 {
     const SgSegment sg = segs[blockIdx.y];
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;   // the same on every lane
     if (q >= (int64_t)sg.d * sg.dw) return;
+    const int lane = threadIdx.x & 63;
     const int i = (int)(q / sg.dw), w = (int)(q % sg.dw);
-    const int32_t* pr = pair + sg.pair_base + (int64_t)i * sg.dpad;
-    uint64_t v = 0;
-    for (int b = 0; b < 64; b++) {
-        const int j = w * 64 + b;
-        if (j < sg.d && pr[j] >= sg.thr) v |= 1ull << b;
-    }
-    ok[sg.ok_base + (int64_t)i * sg.dw + w] = v;
+    const int j = w * 64 + lane;
+    const bool hit = j < sg.d && pair[sg.pair_base + (int64_t)i * sg.dpad + j] >= sg.thr;
+    const uint64_t v = __ballot(hit);
+    if (lane == 0) ok[sg.ok_base + (int64_t)i * sg.dw + w] = v;
 }
 
 // Transaction splits so that about two workgroups per CU (512) are in flight.
@@ -269,8 +268,8 @@ hipError_t sg_launch_pair_ok(const int32_t* pair, const SgSegment* segs, int32_t
                              int32_t max_d, uint64_t* ok, hipStream_t s)
 {
     if (nseg <= 0 || max_d <= 0) return hipSuccess;
-    const int64_t per = (int64_t)max_d * ((max_d + 63) / 64);
-    dim3 grid((unsigned)((per + 255) / 256), (unsigned)nseg);
+    const int64_t per = (int64_t)max_d * ((max_d + 63) / 64);   // words; a wave each
+    dim3 grid((unsigned)((per * 64 + 255) / 256), (unsigned)nseg);
     hipLaunchKernelGGL(k_pair_ok, grid, dim3(256), 0, s, pair, segs, max_d, ok);
     return hipGetLastError();
 }
