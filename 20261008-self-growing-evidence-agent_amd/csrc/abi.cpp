@@ -527,6 +527,25 @@ int drain_ring(sgfim_ctx* c, SearchState* S, int64_t pos, double t)
     const uint16_t* ring_items = S->B->h_ring_items;   // [cap][SG_MAX_DEPTH]
     const int64_t r0 = S->ring_read % cap;
     auto slot = [&](int64_t i) { const int64_t s = r0 + i; return s < cap ? s : s - cap; };
+    // Every entry is checked before anything is sized or copied from it: an itemset length
+    // outside 1..SG_MAX_DEPTH, an item outside the universe or a segment outside the search
+    // would make the copies below read or write out of bounds. Such an entry can only come
+    // from a device-side fault, so it stops the run with SGFIM_E_DEVICE.
+    const int32_t d = S->U.d;
+    const int32_t nseg_search = (int32_t)S->vsegs.size();
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t s = slot(i);
+        if (s < 0 || s >= cap) return fail(c, SGFIM_E_DEVICE, "MFI ring slot %lld outside [0, %lld)", (long long)s, (long long)cap);
+        const int32_t* m = ring_meta + s * 4;
+        if (m[0] < 1 || m[0] > SG_MAX_DEPTH || m[2] < 0 || m[2] >= nseg_search)
+            return fail(c, SGFIM_E_DEVICE, "corrupt MFI ring entry %lld (length %d, segment %d)", (long long)(S->ring_read + i),
+                        m[0], m[2]);
+        const uint16_t* it = ring_items + s * SG_MAX_DEPTH;
+        for (int j = 0; j < m[0]; j++)
+            if ((int32_t)it[j] >= d)
+                return fail(c, SGFIM_E_DEVICE, "corrupt MFI ring entry %lld (item %d of %d)", (long long)(S->ring_read + i),
+                            (int)it[j], d);
+    }
     sgfim_result& R = S->pending;
     const size_t n0 = R.sup.size();
     R.off.resize(n0 + 1 + (size_t)n);

@@ -11,6 +11,12 @@ engines' `computation_time(s):` / `total_time(s):` lines on stdout, which the
 reference adapter parses (engine_adapter.py:28-30). The files under sgfim/bin/ carry
 the executables' names, so pointing the reference's TENSORFIM_DIR / ANYFIM_EXE at that
 directory swaps the engines without touching the reference adapter.
+
+Exit codes are the engines': 1 after a run, whether it wrote its Results file or not
+(kernel_bitmap.cu:801,910; AnyTime kernel.cu:803,950; "success is the produced
+Results file, not the exit code", TensorFIM README.md:85-86), and 0 when AnytimeMining
+is asked for more stages than there are items (kernel.cu:804-806). A wrong argument
+count exits 2 (the engines read their arguments without checking them).
 """
 from __future__ import annotations
 
@@ -39,21 +45,23 @@ def main(argv=None) -> int:
         t0 = time.perf_counter()
         r = EA.run_tensorfim(mode.split("_", 1)[1], fimi, thr)
         if not r["ok"]:
+            print(r.get("log", "run failed"), file=sys.stderr)
             return 1
         print(f"MFIs: {len(r['mfis'])}")
         print(f"computation_time(s): {r['comp_times'][-1]:.6f}")
         print(f"total_time(s): {max(r['tot_times'][-1], time.perf_counter() - t0):.6f}")
-        return 0
+        return 1
     if mode == "AnytimeMining":
         if len(args) != 2:
             return _usage("usage: AnytimeMining <fimi> <upto_stage>")
         r = EA.run_anyfim(args[0], int(args[1]))
         if not r["ok"]:
-            # the engine refuses more stages than items and writes no file (kernel.cu:801-804)
-            print("UptoStage exceeds the number of items", file=sys.stderr)
-            return 1
+            print(r.get("log", "run failed"))
+            # the engine refuses more stages than items, writes no file and exits 0
+            # (kernel.cu:804-806); an unreadable input exits 1 (kernel.cu:803)
+            return 0 if r.get("refused") else 1
         print(f"stages: {len(r['stages'])}, results: {r['results_path']}")
-        return 0
+        return 1
     return _usage(f"unknown engine {mode!r}")
 
 

@@ -234,16 +234,28 @@ def parse_anyfim_results(path):
 def run_tensorfim(engine, fimi_path, rel_thr, reps=1, timeout=1800, exact=False):
     """engine: 'baseline' | 'bmma' (both map to the same exact GPU search). Returns
     {'ok', 'mfis': {frozenset: freq}, 'comp_times': [...], 'tot_times': [...]} and writes
-    <fimi>-<rel_thr:.6f>=Results.txt."""
+    <fimi>-<rel_thr:.6f>=Results.txt.
+
+    Failures follow the reference adapter (engine_adapter.py:82-99): a run that times out
+    or produces no Results file sets ok=False and is skipped; nothing is raised. Here a
+    library error of any kind (time-out, more frequent items or a longer itemset than the
+    library supports, capacity, device) is such a run; its message goes under 'log'."""
     if engine not in ("baseline", "bmma"):
         raise ValueError(f"unknown engine {engine!r}")
     rp = f"{fimi_path}-{rel_thr:.6f}=Results.txt"
     comp, tot = [], []
     ok, mfis = True, {}
+    log = []
+    ctx = _ctx()   # no usable device or library: raised, as a missing engine executable is
     t_pre0 = time.perf_counter()
-    off, items = _csr_without_zero(*read_fimi(fimi_path))
-    ctx = _ctx()
-    ctx.load_csr(off, items)
+    try:
+        off, items = _csr_without_zero(*read_fimi(fimi_path))
+        ctx.load_csr(off, items)
+    except (OSError, L.SgfimError) as e:
+        # the engine cannot read its input and writes no Results file (kernel_bitmap.cu:801)
+        if os.path.exists(rp):
+            os.remove(rp)
+        return {"ok": False, "mfis": {}, "comp_times": [], "tot_times": [], "log": str(e)}
     pre = time.perf_counter() - t_pre0
     n = ctx.n_txn
     thr = max(1, math.ceil(n * rel_thr)) if n else 1
@@ -254,22 +266,32 @@ def run_tensorfim(engine, fimi_path, rel_thr, reps=1, timeout=1800, exact=False)
         try:
             res = ctx.mine(thr, flags=0 if exact else L.ROOT_STRICT, rel_thr=rel_thr, deadline_s=float(timeout))
         except L.SgfimError as e:
-            if e.code == L.E_TIMEOUT:
-                ok = False
-                continue
-            raise
+            ok = False
+            log.append(str(e))
+            continue
         dt = time.perf_counter() - t0
         comp.append(dt)
         tot.append(dt + pre)
         mfis = res.as_dict()
         write_tensorfim_results(rp, fimi_path, list(zip(res.itemsets, res.supports)), n, rel_thr, dt, pre,
                                 {"transNum": n, "dimension": ctx.n_items})
-    return {"ok": ok, "mfis": mfis, "comp_times": comp, "tot_times": tot}
+    out = {"ok": ok, "mfis": mfis, "comp_times": comp, "tot_times": tot}
+    if log:
+        out["log"] = "\n".join(log)
+    return out
 
 
 def anyfim_thresholds(supports: np.ndarray, upto_stage: int):
     """Stage thresholds: the k-th highest item count, k = 1..upto_stage (kernel.cu:846)."""
     return [int(x) for x in np.sort(supports)[::-1][:upto_stage]]
+
+
+def _anyfim_failed(rp, why):
+    """The reference adapter's answer when the engine leaves no Results file
+    (engine_adapter.py:123-129): ok=False, no stages, the engine's output under 'log'."""
+    if os.path.exists(rp):
+        os.remove(rp)
+    return {"ok": False, "stages": {}, "meta": {}, "log": why, "results_path": rp}
 
 
 def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
@@ -279,18 +301,27 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
     batched=True mines every stage in one search (sgfim_set_thresholds) instead of
     re-mining stage by stage as the engine does (kernel.cu:844-907); the stage sets are
     the same, and runtimePerStage then holds the time at which each stage's last MFI was
-    confirmed in that shared run."""
+    confirmed in that shared run.
+
+    Failures follow the reference adapter (engine_adapter.py:115-131): a time-out returns
+    ok=False with empty stages, and so does any library error (its message under 'log',
+    as the reference keeps the engine's stdout when the Results file is missing)."""
     rp = f"{fimi_path}-{upto_stage}-stages=Results.txt"
     if os.path.exists(rp):
         os.remove(rp)
+    ctx = _ctx()   # no usable device or library: raised, as a missing engine executable is
     t_pre0 = time.perf_counter()
-    off, items = _csr_without_zero(*read_fimi(fimi_path))
-    ctx = _ctx()
-    ctx.load_csr(off, items)
+    try:
+        off, items = _csr_without_zero(*read_fimi(fimi_path))
+        ctx.load_csr(off, items)
+    except (OSError, L.SgfimError) as e:
+        return _anyfim_failed(rp, str(e))
     pre = time.perf_counter() - t_pre0
     if upto_stage > ctx.n_items:
-        # the engine refuses (kernel.cu:801-804) and writes no file
-        return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+        # the engine refuses (kernel.cu:804-806) and writes no file
+        out = _anyfim_failed(rp, "Up to the stage number being greater than the dimension !")
+        out["refused"] = True
+        return out
     sup = ctx.item_supports()
     thrs = anyfim_thresholds(sup, upto_stage)
     stages, runtimes = {}, []
@@ -300,12 +331,10 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
             res = ctx.mine_thresholds(thrs, flags=0 if exact else L.ROOT_TOPK,
                                       root_topk=list(range(1, upto_stage + 1)))
         except L.SgfimError as e:
-            if e.code == L.E_TIMEOUT:
-                return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
-            raise
+            return _anyfim_failed(rp, str(e))
         total = time.perf_counter() - t_start
         if total > timeout:
-            return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+            return _anyfim_failed(rp, f"time-out after {total:.3f} s")
         for k in range(1, upto_stage + 1):
             stages[k] = []
         done_at = [0.0] * upto_stage
@@ -319,14 +348,12 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
     for k in range(1, upto_stage + 1):
         left = timeout - (time.perf_counter() - t_start)
         if left <= 0:
-            return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
+            return _anyfim_failed(rp, f"time-out before stage {k}")
         t0 = time.perf_counter()
         try:
             res = ctx.mine(thrs[k - 1], flags=0 if exact else L.ROOT_TOPK, root_topk=k, deadline_s=left)
         except L.SgfimError as e:
-            if e.code == L.E_TIMEOUT:
-                return {"ok": False, "stages": {}, "meta": {}, "results_path": rp}
-            raise
+            return _anyfim_failed(rp, str(e))
         runtimes.append(time.perf_counter() - t0)
         stages[k] = list(zip(res.itemsets, res.supports))
     write_anyfim_results(rp, fimi_path, stages, thrs, runtimes, ctx.n_txn, pre)

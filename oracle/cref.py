@@ -58,12 +58,25 @@ def lib():
         L.kat_csr.argtypes = [P, C.c_int64, P, P]
         L.kat_levelwise.argtypes = [P, C.c_int64, C.c_int64, P, P, C.c_int64]
         L.kat_levelwise.restype = C.c_int64
+        L.ref_simd_path.restype = C.c_int
+        L.ref_set_simd.argtypes = [C.c_int]; L.ref_set_simd.restype = C.c_int
         _lib = L
     return _lib
 
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def simd_path() -> str:
+    """The counter's inner loop in use: "avx512-vpopcntdq" or "scalar-popcnt"."""
+    return "avx512-vpopcntdq" if lib().ref_simd_path() else "scalar-popcnt"
+
+
+def set_simd(best: bool) -> str:
+    """Use the host's best counter path (best=True) or force the scalar popcnt loop."""
+    lib().ref_set_simd(1 if best else 0)
+    return simd_path()
 
 
 def bitmap(offsets: np.ndarray, items: np.ndarray, n_items: int) -> np.ndarray:

@@ -34,6 +34,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <immintrin.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -44,6 +45,87 @@ enum { REF_COUNT_ALL = 0, REF_COUNT_INHERIT = 1 };
 enum { REF_ROOTS_ALL = 0, REF_ROOTS_STRICT = 1, REF_ROOTS_TOPK = 2 };
 
 static inline int popc64(uint64_t x) { return __builtin_popcountll(x); }
+
+/* ---------------------------------------------------------------- vectorised AND + popcount
+ * The node counter's inner loop, sum_w popcount(m[w] & r[w]) (kernel_bitmap.cu:400-405;
+ * the CPU-vectorised miner does the same with a byte LUT, exp_e6_cpuvec.py:43-107). Two
+ * implementations, chosen once at run time: AVX-512 VPOPCNTDQ (eight 64-bit popcounts
+ * per instruction, four independent accumulators) on hosts that have it, else the scalar
+ * popcnt loop. Both return the same integer; the tests check them against each other. */
+static int64_t andpop_scalar(const uint64_t* m, const uint64_t* r, int64_t W)
+{
+    int64_t s = 0;
+    for (int64_t w = 0; w < W; w++) s += popc64(m[w] & r[w]);
+    return s;
+}
+
+__attribute__((target("avx512f,avx512vpopcntdq")))
+static int64_t andpop_avx512(const uint64_t* m, const uint64_t* r, int64_t W)
+{
+    __m512i a0 = _mm512_setzero_si512(), a1 = a0, a2 = a0, a3 = a0;
+    int64_t w = 0;
+    for (; w + 32 <= W; w += 32) {
+        a0 = _mm512_add_epi64(a0, _mm512_popcnt_epi64(_mm512_and_si512(_mm512_loadu_si512(m + w), _mm512_loadu_si512(r + w))));
+        a1 = _mm512_add_epi64(a1, _mm512_popcnt_epi64(_mm512_and_si512(_mm512_loadu_si512(m + w + 8), _mm512_loadu_si512(r + w + 8))));
+        a2 = _mm512_add_epi64(a2, _mm512_popcnt_epi64(_mm512_and_si512(_mm512_loadu_si512(m + w + 16), _mm512_loadu_si512(r + w + 16))));
+        a3 = _mm512_add_epi64(a3, _mm512_popcnt_epi64(_mm512_and_si512(_mm512_loadu_si512(m + w + 24), _mm512_loadu_si512(r + w + 24))));
+    }
+    for (; w + 8 <= W; w += 8)
+        a0 = _mm512_add_epi64(a0, _mm512_popcnt_epi64(_mm512_and_si512(_mm512_loadu_si512(m + w), _mm512_loadu_si512(r + w))));
+    int64_t s = _mm512_reduce_add_epi64(_mm512_add_epi64(_mm512_add_epi64(a0, a1), _mm512_add_epi64(a2, a3)));
+    for (; w < W; w++) s += popc64(m[w] & r[w]);
+    return s;
+}
+
+/* dst[w] = a[w] & b[w] (the prefix masks); the compiler vectorises it for the host ISA */
+__attribute__((target("avx512f")))
+static void and_rows_avx512(uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t W)
+{
+    int64_t w = 0;
+    for (; w + 8 <= W; w += 8)
+        _mm512_storeu_si512(dst + w, _mm512_and_si512(_mm512_loadu_si512(a + w), _mm512_loadu_si512(b + w)));
+    for (; w < W; w++) dst[w] = a[w] & b[w];
+}
+
+static void and_rows_scalar(uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t W)
+{
+    for (int64_t w = 0; w < W; w++) dst[w] = a[w] & b[w];
+}
+
+static int g_simd = -1;   /* -1: not chosen yet; 0: scalar; 1: AVX-512 VPOPCNTDQ */
+
+static int simd_path(void)
+{
+    if (g_simd < 0) {
+        __builtin_cpu_init();
+        g_simd = (__builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512vpopcntdq")) ? 1 : 0;
+    }
+    return g_simd;
+}
+
+/* The counter path in use (1 = AVX-512 VPOPCNTDQ, 0 = scalar popcnt). */
+int ref_simd_path(void) { return simd_path(); }
+
+/* Force the scalar path (0) or the host's best path (1; scalar where AVX-512 VPOPCNTDQ is
+ * absent). Returns the path now in use. */
+int ref_set_simd(int want)
+{
+    g_simd = -1;
+    if (want) return simd_path();
+    g_simd = 0;
+    return 0;
+}
+
+static inline int64_t andpop(const uint64_t* m, const uint64_t* r, int64_t W)
+{
+    return simd_path() ? andpop_avx512(m, r, W) : andpop_scalar(m, r, W);
+}
+
+static inline void and_rows(uint64_t* dst, const uint64_t* a, const uint64_t* b, int64_t W)
+{
+    if (simd_path()) and_rows_avx512(dst, a, b, W);
+    else and_rows_scalar(dst, a, b, W);
+}
 
 /* ---------------------------------------------------------------- bitmap build */
 /* rows[(id-1) * W + (t >> 6)] |= 1 << (t & 63) for every item id in 1..n_items. */
@@ -70,12 +152,7 @@ void ref_count(const uint64_t* rows, int64_t W, const int32_t* prefix, int32_t k
         const uint64_t* r = rows + (size_t)prefix[i] * W;
         for (int64_t w = 0; w < W; w++) mask[w] &= r[w];
     }
-    for (int32_t j = 0; j < nc; j++) {
-        const uint64_t* r = rows + (size_t)cands[j] * W;
-        int64_t s = 0;
-        for (int64_t w = 0; w < W; w++) s += popc64(mask[w] & r[w]);
-        out[j] = s;
-    }
+    for (int32_t j = 0; j < nc; j++) out[j] = andpop(mask, rows + (size_t)cands[j] * W, W);
     free(mask);
 }
 
@@ -191,9 +268,7 @@ static void visit(const ref_ctx* c, ref_worker* w, int k, int64_t freq)
     }
     for (int j = 0; j < nsrc; j++) {
         int32_t e = src ? src[j] : j;
-        const uint64_t* r = c->rows + (size_t)e * W;
-        int64_t s = 0;
-        for (int64_t q = 0; q < W; q++) s += popc64(m[q] & r[q]);
+        const int64_t s = andpop(m, c->rows + (size_t)e * W, W);
         res[e] = s;
         w->counts++;
         if ((double)s >= c->thr) fl[nf++] = e;
@@ -213,9 +288,7 @@ static void visit(const ref_ctx* c, ref_worker* w, int k, int64_t freq)
     for (int j = nf - 1; j >= 0; j--) {
         int32_t i = fl[j];
         if (i >= progress) continue;
-        uint64_t* cm = w->masks + (size_t)k * W;
-        const uint64_t* r = c->rows + (size_t)i * W;
-        for (int64_t q = 0; q < W; q++) cm[q] = m[q] & r[q];
+        and_rows(w->masks + (size_t)k * W, m, c->rows + (size_t)i * W, W);
         w->x[k] = i;
         const int64_t saved = w->tr_parent;
         w->tr_parent = me;

@@ -163,15 +163,57 @@ def test_engine_argv_shims(fimi):
         os.remove(rp)
     p = subprocess.run([sys.executable, os.path.join(bindir, "CoParaCG_bmma.exe"), path, "0.05"],
                        capture_output=True, text=True, timeout=300)
-    assert p.returncode == 0, p.stderr
+    assert p.returncode == 1, p.stderr                            # the engines exit 1 after a run
     assert "computation_time(s):" in p.stdout and "total_time(s):" in p.stdout
     assert EA.parse_tensorfim_results(rp) == O.tensorfim_family(rows, 0.05)
     p = subprocess.run([sys.executable, os.path.join(bindir, "AnytimeMining.exe"), path, "5"],
                        capture_output=True, text=True, timeout=300)
-    assert p.returncode == 0, p.stderr
+    assert p.returncode == 1, p.stderr
     st, meta = EA.parse_anyfim_results(f"{path}-5-stages=Results.txt")
     for k in range(1, 6):
         assert dict(st[k]) == O.anyfim_stage_family(rows, k)[0]
     p = subprocess.run([sys.executable, os.path.join(bindir, "AnytimeMining.exe"), path, "41"],
                        capture_output=True, text=True, timeout=300)
-    assert p.returncode == 1                                      # more stages than items
+    assert p.returncode == 0                                      # more stages than items: refused, exit 0
+    assert not os.path.exists(f"{path}-41-stages=Results.txt")
+    p = subprocess.run([sys.executable, os.path.join(bindir, "AnytimeMining.exe"), path + ".missing", "5"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 1                                      # unreadable input: exit 1, no file
+    assert not os.path.exists(f"{path}.missing-5-stages=Results.txt")
+
+
+def test_adapter_failures_return_not_ok(tmp_path):
+    """Library failures come back as ok=False with empty results and a 'log', as the
+    reference adapter reports an engine that left no Results file (engine_adapter.py:82-99,
+    123-129); nothing is raised."""
+    print("case: adapter failures -> ok=False", flush=True)
+    # 40 items that always occur together: at 150 of 200 one MFI of 40 items, past the
+    # longest itemset the search supports (E_DEPTH)
+    rng = np.random.default_rng(0)
+    rows = [tuple(range(1, 41)) + tuple(int(x) for x in rng.choice(np.arange(41, 61), 3, replace=False))
+            for _ in range(200)]
+    deep = tmp_path / "deep.txt"
+    deep.write_text("\n".join(" ".join(map(str, r)) for r in rows) + "\n")
+    r = EA.run_tensorfim("bmma", str(deep), 0.75, reps=2)
+    assert r["ok"] is False and r["mfis"] == {} and r["comp_times"] == [] and "longer than" in r["log"]
+    assert not os.path.exists(f"{deep}-{0.75:.6f}=Results.txt")
+    a = EA.run_anyfim(str(deep), 3)
+    assert a["ok"] is False and a["stages"] == {} and a["log"]
+    assert not os.path.exists(a["results_path"])
+    # 4,200 distinct items, every one frequent: more than the 4,096 a search supports (E_INVAL)
+    wide = tmp_path / "wide.txt"
+    wide.write_text("\n".join(" ".join(str(x) for x in range(1 + (t % 2), 4201, 2)) for t in range(4)) + "\n")
+    r = EA.run_tensorfim("baseline", str(wide), 0.25, exact=True)
+    assert r["ok"] is False and r["mfis"] == {} and "4096" in r["log"]
+    r = EA.run_tensorfim("bmma", str(tmp_path / "absent.txt"), 0.1)
+    assert r["ok"] is False and r["mfis"] == {}
+    p = subprocess_run_shim("CoParaCG_bmma.exe", str(deep), "0.75")
+    assert p.returncode == 1 and not os.path.exists(f"{deep}-{0.75:.6f}=Results.txt")
+
+
+def subprocess_run_shim(name, *args):
+    import subprocess
+    import sys
+    bindir = os.path.join(os.path.dirname(EA.__file__), "bin")
+    return subprocess.run([sys.executable, os.path.join(bindir, name), *args], capture_output=True, text=True,
+                          timeout=300)
