@@ -114,6 +114,8 @@ struct sgfim_ctx {
     std::string dev_name;               // gcnArchName of the device
     int32_t tile_cw = 0, tile_cr = 0;   // counting-tile shape of the search (0: default)
     int64_t sib_words = 0;              // sibling-ring words of the search (0: default)
+    int32_t proj_factor = 4;            // projected blocks: factor (0: off)
+    int64_t proj_arena = 0;             // projected-block arena words (0: default)
     sgfim_opts opts{};
     // loaded data
     bool loaded = false;
@@ -614,6 +616,10 @@ void fill_stats(sgfim_ctx* c, SearchState* S)
         st.mfis = h[CTL_RING_POS];
         st.frontier = frontier_size(h);
         st.words = h[CTL_WORDS];
+        st.projections = h[CTL_PROJS];
+        st.proj_words = h[CTL_PROJ_WORDS];
+        st.proj_from_rows = h[CTL_PROJ_ROWS];
+        st.proj_block_words = h[CTL_PROJ_ARENA];
     }
     st.round = S->rounds;
     st.pair_counts = S->pair_counts;
@@ -629,6 +635,7 @@ void fill_stats(sgfim_ctx* c, SearchState* S)
 int check_err(sgfim_ctx* c, SearchState* S)
 {
     const int64_t e = S->B->h_ctl[CTL_ERR];
+    if (e & 4) return fail(c, SGFIM_E_DEVICE, "projected block inconsistent (transaction list or item outside its U)");
     if (e & 2) return fail(c, SGFIM_E_DEPTH, "an itemset longer than %d items would be needed", SG_MAX_DEPTH);
     if (e & 1)
         return fail(c, SGFIM_E_CAPACITY,
@@ -666,8 +673,14 @@ int launch_round(sgfim_ctx* c, SearchState* S)
 }
 
 // (Re)use the context's search buffers for a run needing the given sizes.
+struct ProjSizes {
+    int32_t factor = 0;
+    int64_t arena = 0, pdesc = 0, ptid = 0;
+};
+
 int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap, int32_t d, int32_t dw,
-                int32_t bmax, int32_t chunk, int32_t range, int64_t trace_cap, int64_t n_roots, int64_t cell_cap)
+                int32_t bmax, int32_t chunk, int32_t range, int64_t trace_cap, int64_t n_roots, int64_t cell_cap,
+                const ProjSizes& ps)
 {
     // sibling ring: four rows per frontier record, 8 MB .. 2 GB, unless set
     const int64_t sib_cap = c->sib_words > 0 ? c->sib_words
@@ -677,7 +690,9 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         const SgSearchDev& D = B->D;
         const bool fits = D.sib_cap == sib_cap && D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
                           D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.chunk_range == range && D.trace_cap == trace_cap &&
-                          B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + 2 * D.newborn_cap;
+                          B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + 2 * D.newborn_cap &&
+                          D.proj_factor == ps.factor && D.parena_cap == ps.arena && D.pdesc_cap == ps.pdesc &&
+                          D.ptid_cap == ps.ptid;
         if (fits) return SGFIM_OK;
         bufs_free(c);
     }
@@ -696,6 +711,12 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     D.trace_cap = trace_cap;
     D.cell_cap = cell_cap;
     D.sib_cap = sib_cap;
+    D.proj_factor = ps.factor;
+    D.parena_cap = ps.arena;
+    D.proj_round_words = ps.arena / 4;
+    D.proj_max_block = D.proj_round_words;
+    D.pdesc_cap = ps.pdesc;
+    D.ptid_cap = ps.ptid;
     B->roots_cap = std::max<int64_t>(n_roots, 1);
     auto& V = B->bufs;
     int rc;
@@ -716,9 +737,24 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.ctl, CTL_SLOTS)) ||
         (rc = dev_alloc_t(c, V, &D.sib, (size_t)sib_cap)) || (rc = dev_alloc_t(c, V, &D.sib_off, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &D.need, (size_t)newborn_cap * dw)) ||
-        (rc = dev_alloc_t(c, V, &B->d_roots, (size_t)B->roots_cap * 2))) {
+        (rc = dev_alloc_t(c, V, &B->d_roots, (size_t)B->roots_cap * 2)) ||
+        (rc = dev_alloc_t(c, V, &D.bview, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.bsrc, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.bpnew, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.bnr, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.bunitB, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.bunitT, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.bunitG, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.bpa, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.bpd, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.ptoff, (size_t)bmax + 1)) ||
+        (rc = dev_alloc_t(c, V, &D.pfill, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.crow, (size_t)bmax * d))) {
         bufs_free(c);
         return rc;
+    }
+    if (ps.factor > 0) {   // the block arena holds a ring of arena words plus one largest block past its end
+        if ((rc = dev_alloc_t(c, V, &D.parena, (size_t)(ps.arena + D.proj_max_block))) ||
+            (rc = dev_alloc_t(c, V, &D.pdesc, (size_t)ps.pdesc)) ||
+            (rc = dev_alloc_t(c, V, &D.pmask, (size_t)ps.pdesc * dw)) ||
+            (rc = dev_alloc_t(c, V, &D.prank, (size_t)ps.pdesc * dw)) || (rc = dev_alloc_t(c, V, &D.ptid, (size_t)ps.ptid))) {
+            bufs_free(c);
+            return rc;
+        }
     }
     if (trace_cap) {
         if ((rc = dev_alloc_t(c, V, &D.trace_key, (size_t)trace_cap * 2)) ||
@@ -1164,6 +1200,25 @@ int sgfim_set_sib_ring(sgfim_ctx* c, int64_t words)
     if (words < 0) return fail(c, SGFIM_E_INVAL, "sibling ring: words must be >= 0");
     if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "sibling ring: an anytime search is in progress");
     c->sib_words = words;
+    return SGFIM_OK;
+}
+
+int sgfim_set_projection(sgfim_ctx* c, int32_t factor, int64_t arena_words)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (factor < 0 || factor == 1 || factor > 1024 || arena_words < 0 || (arena_words > 0 && arena_words < 1024))
+        return fail(c, SGFIM_E_INVAL, "projection: factor must be 0 or 2..1024, arena_words 0 or >= 1024");
+    if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "projection: an anytime search is in progress");
+    c->proj_factor = factor;
+    c->proj_arena = arena_words;
+    return SGFIM_OK;
+}
+
+int sgfim_get_projection(const sgfim_ctx* c, int32_t* factor, int64_t* arena_words)
+{
+    if (!c || !factor || !arena_words) return SGFIM_E_INVAL;
+    *factor = c->proj_factor;
+    *arena_words = c->proj_arena;
     return SGFIM_OK;
 }
 
@@ -1683,8 +1738,30 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         ring_cap = c->opts.ring_cap;
         newborn_cap = std::min(newborn_cap, (ring_cap - n_roots) / 2);
     }
+    // projected blocks: arena ring (default 32x the loaded rows, at least 2^28 words, at most a
+    // quarter of the device memory), descriptors for every frontier record up to 2^20,
+    // transaction lists of one round (at least 2^27 entries, and two blocks projected from
+    // the longest window)
+    ProjSizes ps;
+    if (c->proj_factor > 0) {
+        ps.factor = c->proj_factor;
+        int64_t a = c->proj_arena;
+        if (a <= 0) {
+            a = std::max<int64_t>((int64_t)1 << 28, 32 * c->rows_words);
+            size_t fr = 0, tt = 0;   // the total (not the free) memory keeps the size stable across runs
+            if (hipMemGetInfo(&fr, &tt) == hipSuccess)
+                a = std::max<int64_t>((int64_t)1 << 22, std::min<int64_t>(a, (int64_t)(tt / 4 / 8 * 15 / 16)));
+            else
+                (void)hipGetLastError();
+        }
+        ps.arena = a;
+        ps.pdesc = std::min<int64_t>(std::max<int64_t>(cap, (int64_t)1 << 16), (int64_t)1 << 20);
+        int64_t nmax = 0;
+        for (int32_t v = 0; v < nv; v++) nmax = std::max<int64_t>(nmax, vs[v].n);
+        ps.ptid = std::max<int64_t>((int64_t)1 << 27, 2 * nmax / ps.factor + 1);
+    }
     if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, search_chunk(c), search_range(c),
-                          c->trace ? c->trace_cap : 0, n_roots, cell_cap)))
+                          c->trace ? c->trace_cap : 0, n_roots, cell_cap, ps)))
         return rc;
     S->B = c->SB;
     SgSearchDev& D = S->B->D;

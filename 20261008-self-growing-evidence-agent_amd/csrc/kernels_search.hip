@@ -161,7 +161,7 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
                                const int32_t* __restrict__ pair, int32_t slot, const uint16_t* px, int k,
                                int32_t sup, int32_t last, int seg, const uint64_t* U, int32_t sup_parent,
                                int32_t nf_parent, int32_t ntail_parent, int parent_is_root, float* feat_out,
-                               int64_t key, int64_t parent_key, int64_t sib)
+                               int64_t key, int64_t parent_key, int64_t sib, int64_t proj, int32_t pk)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t v = lane < sg.dw ? U[lane] : 0ull;
@@ -195,6 +195,9 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     m.nrange = sg_ranges(sg.words, S.chunk_words, S.chunk_range);
     m.score = 0.0f;
     m.sib = sib;
+    m.proj = proj;
+    m.pk = pk;
+    m.pad_ = 0;
     S.meta[slot] = m;
     S.key[slot] = key;
     S.pkey[slot] = parent_key;
@@ -377,7 +380,8 @@ __global__ __launch_bounds__(SG_SEED_WAVES * 64) void k_seed(SgSearchDev S, cons
         slot = __shfl(slot, 0, 64);
         if (slot >= 0) {
             float* fo = S.feat_on ? S.feat + nb * SG_FEAT : nullptr;
-            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, s_key + rank, -1, -1);
+            sg_make_record(S, sg, isup, pair, slot, sX[wv], k, isup[r], r, seg, sF[wv], 0, 0, 0, 0, fo, s_key + rank, -1, -1,
+                           -1, 0);
             if (lane == 0) S.newborn[nb] = slot;
         }
     }
@@ -419,16 +423,97 @@ __device__ __forceinline__ void sg_block_excl_scan(int64_t (&v)[N], int64_t (*ws
     for (int n = 0; n < N; n++) v[n] = wsum[n][wv] + x[n] - v[n];
 }
 
+// ------------------------------------------------------------------ projected blocks
+// A block p is usable this round when neither ring can wrap onto it before the round ends:
+// the round allocates at most SG_BMAX descriptors and proj_round_words arena words, and
+// (pn0, an0) are the ring positions before it. The descriptor of a usable block is p's.
+__device__ __forceinline__ bool sg_block_usable(const SgSearchDev& S, int64_t p, int64_t pn0, int64_t an0, SgProj& D)
+{
+    if (p < 0 || p >= pn0 || pn0 + SG_BMAX - p > S.pdesc_cap) return false;
+    D = S.pdesc[p % S.pdesc_cap];
+    return an0 + S.proj_round_words - D.off <= S.parena_cap;
+}
+
+// Row of item f in block p (its rank in U of the block's itemset). Every item a record
+// of the block's subtree names is in that U (U shrinks down the tree, and every later item
+// of the record came from it); an item outside it would address past the block, so it
+// flags an error (the search stops with SGFIM_E_DEVICE) and row 0 is used instead.
+__device__ __forceinline__ int32_t sg_block_row(const SgSearchDev& S, int64_t p, int32_t f)
+{
+    const int64_t b = (p % S.pdesc_cap) * S.dw_max + (f >> 6);
+    const uint64_t mk = S.pmask[b];
+    if (f < 0 || f >= S.d_max || !((mk >> (f & 63)) & 1ull)) {
+        atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 4ull);
+        return 0;
+    }
+    return S.prank[b] + __popcll(mk & ((1ull << (f & 63)) - 1ull));
+}
+
+// Words of a block over n transactions, and its row stride.
+__device__ __forceinline__ int64_t sg_block_words(int64_t n) { return (n + 63) >> 6; }
+__device__ __forceinline__ int64_t sg_block_stride(int64_t n) { return (sg_block_words(n) + 7) & ~7ll; }
+
+// How one taken record is counted this round. The record counts on its inherited block
+// while that block is usable, else on its window's rows; with projection on, a record
+// whose support is at most 1/proj_factor of its view's transactions gets a block of its
+// own, projected from that view this round, and counts on it.
+struct SgPlan {
+    int64_t view;      // block counted on (-1: the window's rows); a new block: its id
+    int64_t src;       // a new block's source view (-1: the window's rows)
+    int64_t words;     // words of the view's rows
+    int64_t src_words; // words of the source view (new blocks)
+    int64_t blk;       // arena words of a new block
+    int32_t pnew;      // a new block
+    int32_t nr;        // row-chunk ranges of the view
+};
+
+__device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgSegment& sg, const SgNodeMeta& m,
+                                                 int64_t pn0, int64_t an0)
+{
+    SgPlan P;
+    P.view = -1;
+    P.src = -1;
+    P.words = sg.words;
+    P.src_words = sg.words;
+    P.blk = 0;
+    P.pnew = 0;
+    int64_t n_view = sg.n;
+    if (S.proj_factor > 0) {
+        SgProj D;
+        if (sg_block_usable(S, m.proj, pn0, an0, D)) {
+            P.view = m.proj;
+            P.words = D.words;
+            n_view = D.n;
+        }
+        const int64_t blk = (int64_t)m.ncand * sg_block_stride(m.sup);
+        if (m.ncand > 0 && (int64_t)m.sup * S.proj_factor <= n_view && blk <= S.proj_max_block && m.sup <= S.ptid_cap) {
+            P.src = P.view;
+            P.src_words = P.words;
+            P.pnew = 1;
+            P.blk = blk;
+            P.words = sg_block_words(m.sup);
+            P.view = -2;   // its id is assigned from the scan
+        }
+    }
+    P.nr = sg_ranges(P.words, S.chunk_words, S.chunk_range);
+    return P;
+}
+
 // Take up to bmax records from the highest buckets, as many as fit the budgets: new
 // records (at most one per child) within the free slots and newborn table, count
-// cells within the cell table. Writes the U-list, child, cell and unit offsets. The
-// sizes are parked in the offset arrays between the passes (each thread rereads only
-// its own entries), so record metadata is loaded once.
-__global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
+// cells within the cell table, new blocks within the round's transaction-list and arena
+// budgets. Writes the U-list, child, cell and unit offsets and each record's view.
+// Counting units: records on window rows run range-major across the batch (units
+// [0, UNITS_A): all blocks of range 0, then range 1, ...), records on blocks record-major
+// after them (a record's blocks, range by range). Sizes are parked in the offset arrays
+// between the passes (each thread rereads only its own entries).
+__global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
+    constexpr int NQ = 11;
     __shared__ int32_t tops[SG_NBUCKETS], ends[SG_NBUCKETS];
-    __shared__ int64_t wsum[5][SEL_THREADS / 64];
-    __shared__ int64_t s_budget, tot[5];
+    __shared__ int64_t wsum[NQ][SEL_THREADS / 64];
+    __shared__ int64_t tot[NQ];
+    __shared__ int64_t s_budget;
     __shared__ int32_t s_total, s_cut, s_maxr;
     const int tid = threadIdx.x;
     static_assert(SG_NBUCKETS == 64, "one lane per bucket");
@@ -460,6 +545,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         S.ctl[CTL_T0] = 0x7fffffffffffffffll;
         S.ctl[CTL_T1] = 0;
     }
+    const int64_t pn0 = S.ctl[CTL_PROJ_NEXT], an0 = S.ctl[CTL_ARENA_NEXT];
     __syncthreads();
     const int total = s_total;
     constexpr int PER = SG_BMAX / SEL_THREADS;
@@ -483,28 +569,40 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             nid[q] = S.bucket[(int64_t)b * S.cap + (tops[b] - 1 - pos)];
         }
     }
-    int64_t su = 0, sc = 0, scell = 0, sunit = 0, ssib = 0;
-    int32_t mr = 0;   // longest window (row-chunk ranges) among the taken records
+    // sums: U items, children, cells, window-row blocks, sibling words, block units,
+    // transactions, arena words, new blocks, transaction-list units, gather units
+    int64_t sm[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) sm[n] = 0;
+    int32_t mr = 0;   // longest window (row-chunk ranges) among the taken records on window rows
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = tid * pe + q;
         if (nid[q] >= 0) {
             const SgNodeMeta m = S.meta[nid[q]];
-            mr = max(mr, m.nrange);
-            su += m.ncand;
-            sc += m.ntail;
-            scell += (int64_t)m.ncand * m.ntail;
-            sunit += m.nblk;
-            ssib += (int64_t)(1 + m.ntail) * S.dw_max;
+            const SgPlan P = sg_plan_record(S, segs[m.seg], m, pn0, an0);
+            const bool onb = P.view != -1;
+            if (!onb) mr = max(mr, P.nr);
+            sm[0] += m.ncand;
+            sm[1] += m.ntail;
+            sm[2] += (int64_t)m.ncand * m.ntail;
+            sm[3] += onb ? 0 : m.nblk;
+            sm[4] += (int64_t)(1 + m.ntail) * S.dw_max;
+            sm[5] += onb ? (int64_t)m.nblk * P.nr : 0;
+            if (P.pnew) {
+                sm[6] += m.sup;
+                sm[7] += P.blk;
+                sm[8] += 1;
+                sm[9] += (P.src_words + SG_PT_WORDS - 1) / SG_PT_WORDS;
+                sm[10] += ((int64_t)m.sup + SG_PG_TIDS - 1) / SG_PG_TIDS * ((m.ncand + SG_PG_ROWS - 1) / SG_PG_ROWS);
+            }
             S.batch[i] = nid[q];
-            S.coff[i] = m.ncand;
-            S.choff[i] = m.ntail;
-            S.unit_off[i] = m.nblk;
         }
     }
-    int64_t ex[5] = {su, sc, scell, sunit, ssib};
-    sg_block_excl_scan<5>(ex, wsum, tot);
-    int64_t eu = ex[0], ec = ex[1], ecell = ex[2], eunit = ex[3], esib = ex[4];
+    int64_t ex[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; n++) ex[n] = sm[n];
+    sg_block_excl_scan<NQ>(ex, wsum, tot);
     const int64_t sib_base = S.ctl[CTL_SIB_NEXT];
     // pass 2: offsets; the cut is the first record past a budget (offsets only grow)
     const int64_t budget = s_budget;
@@ -512,26 +610,54 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
     for (int q = 0; q < PER; q++) {
         const int i = tid * pe + q;
         if (nid[q] >= 0) {
-            const int32_t nu = S.coff[i], nt = S.choff[i];
-            const int64_t nun = S.unit_off[i];
+            const SgNodeMeta m = S.meta[nid[q]];
+            const SgSegment sg = segs[m.seg];
+            SgPlan P = sg_plan_record(S, sg, m, pn0, an0);
+            const int32_t nu = m.ncand, nt = m.ntail;
             const int64_t cells = (int64_t)nu * nt;
-            if (ec + nt > budget || ecell + cells > S.cell_cap) atomicMin(&s_cut, i);
-            S.coff[i] = (int32_t)eu;
-            S.choff[i] = (int32_t)ec;
-            S.celloff[i] = ecell;
-            S.unit_off[i] = eunit;
+            if (ex[1] + nt > budget || ex[2] + cells > S.cell_cap ||
+                (P.pnew && (ex[6] + m.sup > S.ptid_cap || ex[7] + P.blk > S.proj_round_words)))
+                atomicMin(&s_cut, i);
+            S.coff[i] = (int32_t)ex[0];
+            S.choff[i] = (int32_t)ex[1];
+            S.celloff[i] = ex[2];
+            S.unit_off[i] = ex[3];
             // a block that would wrap onto another of this round gets none
             const int64_t sz = (int64_t)(1 + nt) * S.dw_max;
-            S.sib_off[i] = esib + sz <= S.sib_cap ? sib_base + esib : -1;
-            esib += sz;
-            eu += nu;
-            ec += nt;
-            ecell += cells;
-            eunit += nun;
+            S.sib_off[i] = ex[4] + sz <= S.sib_cap ? sib_base + ex[4] : -1;
+            S.bunitB[i] = ex[5];
+            S.bunitT[i] = ex[9];
+            S.bunitG[i] = ex[10];
+            // ring positions before this record (the allocation at the cut); a new block
+            // gets descriptor pn0 + ex[8] and arena words an0 + ex[7] onwards, and its
+            // descriptor is written by k_candlist for the records before the cut only
+            S.bpd[i] = pn0 + ex[8];
+            S.bpa[i] = an0 + ex[7];
+            if (P.pnew) {
+                P.view = pn0 + ex[8];
+                S.ptoff[i] = ex[6];
+            }
+            S.bview[i] = P.view;
+            S.bsrc[i] = P.src;
+            S.bpnew[i] = P.pnew;
+            S.bnr[i] = P.nr;
+            ex[0] += nu;
+            ex[1] += nt;
+            ex[2] += cells;
+            ex[3] += P.view == -1 ? m.nblk : 0;
+            ex[4] += sz;
+            ex[5] += P.view == -1 ? 0 : (int64_t)m.nblk * P.nr;
+            if (P.pnew) {
+                ex[6] += m.sup;
+                ex[7] += P.blk;
+                ex[8] += 1;
+                ex[9] += (P.src_words + SG_PT_WORDS - 1) / SG_PT_WORDS;
+                ex[10] += ((int64_t)m.sup + SG_PG_TIDS - 1) / SG_PG_TIDS * ((nu + SG_PG_ROWS - 1) / SG_PG_ROWS);
+            }
         }
     }
-    // units = blocks x the longest window's ranges (taken over every taken record: a
-    // longer record past the cut only adds units that every workgroup skips)
+    // window-row units = their blocks x the longest window's ranges (taken over every taken
+    // record: a longer record past the cut only adds units that every workgroup skips)
     if (mr) atomicMax(&s_maxr, mr);
     __syncthreads();
     if (tid < SG_NBUCKETS) {   // release the taken records from their buckets
@@ -548,6 +674,9 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
             S.choff[cut] = (int32_t)tot[1];
             S.celloff[cut] = tot[2];
             S.unit_off[cut] = tot[3];
+            S.bunitB[cut] = tot[5];
+            S.bunitT[cut] = tot[9];
+            S.bunitG[cut] = tot[10];
         }
         if (cut == 0 && total > 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
         const int64_t nch = S.choff[cut];
@@ -555,17 +684,27 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S)
         S.ctl[CTL_CHILDREN] = nch;
         S.ctl[CTL_CELLS] = S.celloff[cut];
         S.ctl[CTL_BLOCKS] = S.unit_off[cut];
-        S.ctl[CTL_UNITS] = S.unit_off[cut] * (int64_t)s_maxr;
+        S.ctl[CTL_UNITS_A] = S.unit_off[cut] * (int64_t)s_maxr;
+        S.ctl[CTL_UNITS] = S.unit_off[cut] * (int64_t)s_maxr + S.bunitB[cut];
+        S.ctl[CTL_PROJ_TUNITS] = S.bunitT[cut];
+        S.ctl[CTL_PROJ_GUNITS] = S.bunitG[cut];
         S.ctl[CTL_NODES] += nch;
         S.ctl[CTL_SIB_NEXT] = S.ctl[CTL_SIB_NEXT] + tot[4];   // blocks of records past the cut stay unused
+        // the rings advance by the allocations of the records before the cut
+        S.ctl[CTL_PROJ_NEXT] = cut == total ? pn0 + tot[8] : S.bpd[cut];
+        S.ctl[CTL_ARENA_NEXT] = cut == total ? an0 + tot[7] : S.bpa[cut];
     }
 }
 
 // ------------------------------------------------------------------ U lists
 // One wave per record: expand U into an ascending id list; the children are its
-// first ntail entries (U is ascending and children are the items below `last`). The
-// wave also maps its children and its child blocks back to the record (chrec, blkrec),
-// so the counting and expansion kernels find a unit's record with one load.
+// first ntail entries (U is ascending and children are the items below `last`). Each
+// entry also gets its row in the record's view (crow): the id itself on window rows, its
+// rank in U of the block's itemset on a block (for a block projected this round that is
+// the entry's own position). The wave maps its children and, on window rows, its child
+// blocks back to the record (chrec, blkrec), so the counting and expansion kernels find
+// a unit's record with one load. A record that gets a new block writes the block's
+// descriptor, its U mask and rank prefix, and clears its transaction-list fill.
 // The grid also clears the batch's count cells.
 __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
@@ -587,20 +726,263 @@ __global__ __launch_bounds__(256) void k_candlist(SgSearchDev S, const SgSegment
         const int y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
+    const int64_t view = S.bview[b];
+    const int pnew = S.bpnew[b];
+    if (pnew) {   // the record's new block (descriptor slot and arena words allocated by k_select)
+        const int64_t ds = view % S.pdesc_cap;
+        if (lane < S.dw_max) {
+            S.pmask[ds * S.dw_max + lane] = v;
+            S.prank[ds * S.dw_max + lane] = x - c;
+        }
+        if (lane == 0) {
+            SgProj D;
+            D.off = S.bpa[b];
+            D.words = sg_block_words(m.sup);
+            D.stride = sg_block_stride(m.sup);
+            D.n = m.sup;
+            D.nrows = m.ncand;
+            S.pdesc[ds] = D;
+            S.pfill[b] = 0;
+            // words the block costs: X's unapplied rows and the U rows' words touched at T(X)
+            // in the source view (read), the block's rows (written)
+            const int64_t src = S.bsrc[b];
+            const int64_t sw = src >= 0 ? S.pdesc[src % S.pdesc_cap].words : segs[m.seg].words;
+            const int kk = m.k - (src >= 0 ? m.pk : 0);
+            const int64_t w = (int64_t)kk * sw + (int64_t)m.ncand * (min(sw, (int64_t)m.sup) + D.words);
+            atomicAdd((unsigned long long*)&S.ctl[CTL_PROJS], 1ull);
+            atomicAdd((unsigned long long*)&S.ctl[CTL_PROJ_WORDS], (unsigned long long)w);
+            if (src < 0) atomicAdd((unsigned long long*)&S.ctl[CTL_PROJ_ROWS], 1ull);
+            atomicAdd((unsigned long long*)&S.ctl[CTL_PROJ_ARENA], (unsigned long long)((int64_t)m.ncand * D.stride));
+        }
+    }
+    // rank of this lane's first item in U of the view's block (inherited blocks)
+    int32_t rk = 0;
+    uint64_t bm = 0ull;
+    if (view >= 0 && !pnew && lane < S.dw_max) {
+        const int64_t q = (view % S.pdesc_cap) * S.dw_max + lane;
+        rk = S.prank[q];
+        bm = S.pmask[q];
+    }
     int pos = S.coff[b] + x - c;
     uint64_t t = v;
     while (t) {
         const int bit = (int)__builtin_ctzll(t);
         t &= t - 1ull;
-        S.clist[pos++] = lane * 64 + bit;
+        const int32_t f = lane * 64 + bit;
+        S.clist[pos] = f;
+        int32_t r = f;
+        if (pnew) {
+            r = pos - S.coff[b];
+        } else if (view >= 0) {
+            r = rk + __popcll(bm & ((1ull << bit) - 1ull));
+            if (!((bm >> bit) & 1ull)) {   // not in the block's U (see sg_block_row)
+                atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 4ull);
+                r = 0;
+            }
+        }
+        S.crow[pos] = r;
+        pos++;
     }
     const int32_t c0 = S.choff[b];
     for (int j = lane; j < m.ntail; j += 64) {
         S.ccount[c0 + j] = 0;
         S.chrec[c0 + j] = b;
     }
-    const int64_t u0 = S.unit_off[b];
-    for (int i = lane; i < m.nblk; i += 64) S.blkrec[u0 + i] = b;
+    if (view == -1) {
+        const int64_t u0 = S.unit_off[b];
+        for (int i = lane; i < m.nblk; i += 64) S.blkrec[u0 + i] = b;
+    }
+}
+
+// Largest batch position g < n with off[g] <= v, for a non-decreasing prefix off (the
+// record whose range of units holds v; records without units share their successor's
+// offset and are never returned for a v inside a range).
+__device__ __forceinline__ int sg_find_record(const int64_t* __restrict__ off, int n, int64_t v)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------ projection
+// Transaction lists of the round's new blocks: T(X) of each such record over its source
+// view, as positions in that view. Units are (record, SG_PT_WORDS source words); each wave
+// takes 1024 of them as 16 steps of 64 coalesced words, ANDs X's rows the source has not
+// applied, and appends the set bits' positions (ascending within the wave) to the
+// record's list at a position reserved with one atomic. Every row of the block is then
+// gathered in this list's order, so the order across waves is free.
+__global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t* __restrict__ rows,
+                                                  const SgSegment* __restrict__ segs)
+{
+    __shared__ int32_t s_row[SG_MAX_DEPTH];
+    __shared__ int s_g;
+    __shared__ int64_t s_local;
+    const int64_t nunits = S.ctl[CTL_PROJ_TUNITS];
+    const int nb = (int)S.ctl[CTL_BATCH_N];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {   // workgroup-uniform
+        __syncthreads();   // the previous unit is done with the LDS
+        if (threadIdx.x == 0) {
+            const int g = sg_find_record(S.bunitT, nb, u);
+            s_g = g;
+            s_local = u - S.bunitT[g];
+        }
+        __syncthreads();
+        const int g = s_g;
+        const int32_t slot = S.batch[g];
+        const SgNodeMeta m = S.meta[slot];
+        const int64_t src = S.bsrc[g];
+        const uint64_t* vrows;
+        int64_t vstride, vwords;
+        int k0;
+        if (src >= 0) {
+            const SgProj D = S.pdesc[src % S.pdesc_cap];
+            vrows = S.parena + D.off % S.parena_cap;
+            vstride = D.stride;
+            vwords = D.words;
+            k0 = m.pk;
+        } else {
+            const SgSegment sg = segs[m.seg];
+            vrows = rows + sg.row_base;
+            vstride = sg.stride;
+            vwords = sg.words;
+            k0 = 0;
+        }
+        const int kk = m.k - k0;
+        if (kk <= 0) {   // a record always has items its view has not applied
+            if (threadIdx.x == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 4ull);
+            continue;
+        }
+        if (threadIdx.x < kk) {
+            const int32_t f = S.items[(int64_t)slot * SG_MAX_DEPTH + k0 + threadIdx.x];
+            s_row[threadIdx.x] = src >= 0 ? sg_block_row(S, src, f) : f;
+        }
+        __syncthreads();
+        const int64_t wb = s_local * SG_PT_WORDS + (int64_t)wv * (SG_PT_WORDS / 4);
+        uint64_t mv[SG_PT_WORDS / 256];
+        int32_t ex[SG_PT_WORDS / 256];
+        int32_t tot = 0;
+#pragma unroll
+        for (int i = 0; i < SG_PT_WORDS / 256; i++) {
+            const int64_t w = wb + i * 64 + lane;
+            uint64_t a = 0ull;
+            if (w < vwords) {
+                a = vrows[(int64_t)s_row[0] * vstride + w];
+                for (int j = 1; j < kk; j++) a &= vrows[(int64_t)s_row[j] * vstride + w];
+            }
+            mv[i] = a;
+            const int32_t cnt = __popcll(a);
+            int32_t y = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t z = __shfl_up(y, o, 64);
+                if (lane >= o) y += z;
+            }
+            ex[i] = tot + y - cnt;
+            tot += __shfl(y, 63, 64);
+        }
+        int64_t base = 0;
+        if (lane == 0 && tot) base = (int64_t)atomicAdd((unsigned long long*)&S.pfill[g], (unsigned long long)tot);
+        base = __shfl(base, 0, 64);
+        int32_t* out = S.ptid + S.ptoff[g];
+        const int64_t lim = m.sup;
+#pragma unroll
+        for (int i = 0; i < SG_PT_WORDS / 256; i++) {
+            const int64_t w = wb + i * 64 + lane;
+            uint64_t a = mv[i];
+            int64_t pos = base + ex[i];
+            while (a) {
+                const int bit = (int)__builtin_ctzll(a);
+                a &= a - 1ull;
+                if (pos < lim) out[pos] = (int32_t)(w * 64 + bit);
+                pos++;
+            }
+        }
+    }
+}
+
+// Rows of the round's new blocks: row r of record X's block holds bit j = transaction
+// T(X)[j] of the source view in the row of X's r-th item of U. One wave per unit
+// (record, SG_PG_TIDS list positions = SG_PG_TIDS / 64 block words, up to SG_PG_ROWS
+// rows): the wave loads the unit's list positions once (SG_PG_TIDS / 64 per lane), then
+// per row gathers one bit per position and packs 64 of them per ballot; lane j keeps word
+// j of the unit, so each row is one contiguous store. A list whose length differs from
+// the record's support flags an error (the block is then not trusted: the search stops
+// with SGFIM_E_DEVICE).
+__global__ __launch_bounds__(256) void k_proj_gather(SgSearchDev S, const uint64_t* __restrict__ rows,
+                                                    const SgSegment* __restrict__ segs)
+{
+    constexpr int NJ = SG_PG_TIDS / 64;   // block words per unit row
+    static_assert(NJ <= 64, "one lane per block word of a unit row");
+    const int64_t nunits = S.ctl[CTL_PROJ_GUNITS];
+    const int nb = (int)S.ctl[CTL_BATCH_N];
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t u = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); u < nunits; u += nwaves) {   // wave-uniform
+        const int g = sg_find_record(S.bunitG, nb, u);
+        const int64_t local = u - S.bunitG[g];
+        const int32_t slot = S.batch[g];
+        const SgNodeMeta m = S.meta[slot];
+        const int nrg = (m.ncand + SG_PG_ROWS - 1) / SG_PG_ROWS;
+        const int64_t tb = local / nrg;
+        const int rg = (int)(local - tb * nrg);
+        if (S.pfill[g] != (int64_t)m.sup) {
+            if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 4ull);
+            continue;
+        }
+        const int64_t src = S.bsrc[g];
+        const uint64_t* vrows;
+        int64_t vstride;
+        if (src >= 0) {
+            const SgProj D = S.pdesc[src % S.pdesc_cap];
+            vrows = S.parena + D.off % S.parena_cap;
+            vstride = D.stride;
+        } else {
+            const SgSegment sg = segs[m.seg];
+            vrows = rows + sg.row_base;
+            vstride = sg.stride;
+        }
+        const SgProj N = S.pdesc[S.bview[g] % S.pdesc_cap];
+        uint64_t* dst = S.parena + N.off % S.parena_cap;
+        const int32_t* tl = S.ptid + S.ptoff[g];
+        int32_t t[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int64_t p = tb * SG_PG_TIDS + j * 64 + lane;
+            t[j] = p < m.sup ? tl[p] : -1;
+        }
+        const int32_t* ul = S.clist + S.coff[g];
+        const int r1 = min(m.ncand, (rg + 1) * SG_PG_ROWS);
+        const int64_t wo = tb * NJ + lane;   // the word of the block row this lane stores
+        // four rows per memory round trip: their NJ bit loads per lane are all in flight
+        // before the first ballot (32-bit loads: the word half that holds the bit)
+        constexpr int RB = 4;
+        for (int r = rg * SG_PG_ROWS; r < r1; r += RB) {
+            const uint32_t* rp[RB];
+#pragma unroll
+            for (int q = 0; q < RB; q++) {
+                const int32_t f = ul[min(r + q, r1 - 1)];
+                rp[q] = (const uint32_t*)(vrows + (int64_t)(src >= 0 ? sg_block_row(S, src, f) : f) * vstride);
+            }
+            uint32_t v[RB][NJ];
+#pragma unroll
+            for (int q = 0; q < RB; q++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++) v[q][j] = t[j] >= 0 ? rp[q][t[j] >> 5] : 0u;
+#pragma unroll
+            for (int q = 0; q < RB; q++) {
+                uint64_t word = 0ull;
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const uint64_t bal = __ballot((v[q][j] >> (t[j] & 31)) & 1u);
+                    if (lane == j) word = bal;
+                }
+                if (r + q < r1 && lane < NJ && wo < N.words) dst[(int64_t)(r + q) * N.stride + wo] = word;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ need rows
@@ -750,7 +1132,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     __shared__ uint64_t pre[CW];             // T(X) over the chunk
     __shared__ uint64_t cp[SG_CB][CW];       // child prefixes; the children's need rows during set-up
     __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
-    __shared__ int64_t s_unit;
+    __shared__ int32_t s_g, s_cr, s_blk;
     __shared__ int32_t s_items[SG_MAX_DEPTH];
     __shared__ int32_t s_child[SG_CB];
     __shared__ int32_t s_nc;
@@ -759,39 +1141,81 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     uint64_t* okr = &cp[0][0];
     const int64_t n_units = S.ctl[CTL_UNITS];
     const int64_t n_blocks = S.ctl[CTL_BLOCKS];
+    const int64_t units_a = S.ctl[CTL_UNITS_A];
+    const int nb = (int)S.ctl[CTL_BATCH_N];
     if (n_units == 0) return;   // a round queued after the frontier emptied: nothing to count
     if (threadIdx.x == 0)
         atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
     for (;;) {
         if (threadIdx.x == 0) {
-            s_unit = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_UNIT_NEXT], 1ull);
+            const int64_t u = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_UNIT_NEXT], 1ull);
             s_nc = 0;
+            if (u >= n_units) {
+                s_g = -1;
+            } else if (u < units_a) {
+                // records on window rows, range-major: all their child blocks work on
+                // row-chunk range cr together, so the rows of one range stay in L2 while
+                // every record uses them
+                const int cr = (int)(u / n_blocks);
+                const int64_t bq = u - (int64_t)cr * n_blocks;
+                const int g = S.blkrec[bq];
+                s_g = g;
+                s_cr = cr;
+                s_blk = (int)(bq - S.unit_off[g]);
+            } else {
+                // records on projected blocks, record-major: a record's blocks range by range
+                const int64_t v = u - units_a;
+                const int g = sg_find_record(S.bunitB, nb, v);
+                const int64_t local = v - S.bunitB[g];
+                const int nblk = S.meta[S.batch[g]].nblk;
+                const int cr = (int)(local / nblk);
+                s_g = g;
+                s_cr = cr;
+                s_blk = (int)(local - (int64_t)cr * nblk);
+            }
         }
         __syncthreads();
-        const int64_t u = s_unit;
-        if (u >= n_units) break;
-        // range-major order: all child blocks of the batch work on row-chunk range cr
-        // together, so the rows of one range stay in L2 while every record uses them
-        const int cr = (int)(u / n_blocks);
-        const int64_t bq = u - (int64_t)cr * n_blocks;
-        const int g = S.blkrec[bq];
+        const int g = s_g;
+        if (g < 0) break;
+        const int cr = s_cr, blk = s_blk;
         const int32_t slot = S.batch[g];
         const SgNodeMeta m = S.meta[slot];
-        if (cr >= m.nrange) {   // past this record's (shorter) window
+        if (cr >= S.bnr[g]) {   // past this record's (shorter) window
             __syncthreads();
             continue;
         }
         const SgSegment sg = segs[m.seg];
-        const int blk = (int)(bq - S.unit_off[g]);
-        const int64_t nchunk = (sg.words + CW - 1) / CW;
+        // the record's view: its projected block (T(X) of the block's itemset is every
+        // transaction of it; X's later items are ANDed) or the window's rows (all of X)
+        const int64_t view = S.bview[g];
+        const uint64_t* seg_rows;
+        int64_t vstride, vwords;
+        int k0;
+        if (view >= 0) {
+            const SgProj D = S.pdesc[view % S.pdesc_cap];
+            seg_rows = S.parena + D.off % S.parena_cap;
+            vstride = D.stride;
+            vwords = D.words;
+            k0 = S.bpnew[g] ? m.k : m.pk;
+        } else {
+            seg_rows = rows + sg.row_base;
+            vstride = sg.stride;
+            vwords = sg.words;
+            k0 = 0;
+        }
+        const int kk = m.k - k0;
+        const int64_t nchunk = (vwords + CW - 1) / CW;
         const int c0 = cr * S.chunk_range;
         const int c1 = (int)min((int64_t)(c0 + S.chunk_range), nchunk);
         const int j0 = blk * SG_CB;
         const int ncb = min(SG_CB, m.ntail - j0);
         const int32_t u0 = S.coff[g];
         const int nu = m.ncand;
-        if (threadIdx.x < m.k) s_items[threadIdx.x] = S.items[(int64_t)slot * SG_MAX_DEPTH + threadIdx.x];
-        if (threadIdx.x < ncb) s_child[threadIdx.x] = S.clist[u0 + j0 + threadIdx.x];
+        if (threadIdx.x < kk) {
+            const int32_t f = S.items[(int64_t)slot * SG_MAX_DEPTH + k0 + threadIdx.x];
+            s_items[threadIdx.x] = view >= 0 ? sg_block_row(S, view, f) : f;
+        }
+        if (threadIdx.x < ncb) s_child[threadIdx.x] = S.crow[u0 + j0 + threadIdx.x];
         __syncthreads();
         // set-up: the children's need rows (k_need_rows), then the list of needed candidates
         {
@@ -816,18 +1240,17 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
         }
         __syncthreads();
         const int nc = s_nc;
-        const uint64_t* seg_rows = rows + sg.row_base;
         int32_t* cells = S.cells + S.celloff[g];
-        const int32_t* clist = S.clist + u0;
+        const int32_t* crow = S.crow + u0;   // rows of U's items in the view
         // the wave's first step takes the same candidates in every chunk of the unit
         const int i0f = wave * SG_RC;
         uint64_t nmf;
         int32_t frf[SG_RC];
         int ilf;
-        sg_step_plan(s_cand, clist, nc, i0f, nmf, frf, ilf);
+        sg_step_plan(s_cand, crow, nc, i0f, nmf, frf, ilf);
         for (int c = c0; c < c1; c++) {
             const int64_t w0 = (int64_t)c * CW;
-            const int wn = (int)min((int64_t)CW, sg.words - w0);
+            const int wn = (int)min((int64_t)CW, vwords - w0);
             const int qn = (wn + 63) >> 6;   // 64-word columns holding row words
             __syncthreads();   // every wave is done with the previous chunk's prefixes
             // child prefixes T(X) & T(e) (zero past the last word) and the children's
@@ -839,7 +1262,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
                 for (int jj = 0; jj < 2; jj++) {
                     const int j = wave + NW * jj;
-                    const uint64_t* row = seg_rows + (int64_t)(j < ncb ? s_child[j] : 0) * sg.stride + w0;
+                    const uint64_t* row = seg_rows + (int64_t)(j < ncb ? s_child[j] : 0) * vstride + w0;
 #pragma unroll
                     for (int q = 0; q < PER; q++) {
                         const int w = lane + 64 * q;
@@ -850,8 +1273,8 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
                 if (lane < SW) {
                     const int w = wave * SW + lane;
                     uint64_t v = w < wn ? ~0ull : 0ull;
-                    for (int i = 0; i < m.k; i++)
-                        if (w < wn) v &= seg_rows[(int64_t)s_items[i] * sg.stride + w0 + w];
+                    for (int i = 0; i < kk; i++)
+                        if (w < wn) v &= seg_rows[(int64_t)s_items[i] * vstride + w0 + w];
                     pre[w] = v;
                 }
                 __syncthreads();
@@ -873,7 +1296,7 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
             }
             // the first step's candidates load while the other waves finish their prefixes
             uint64_t xf[SG_RC][PER];
-            sg_step_load<PER>(xf, seg_rows, sg.stride, w0, wn, i0f < nc ? nmf : 0ull, frf);
+            sg_step_load<PER>(xf, seg_rows, vstride, w0, wn, i0f < nc ? nmf : 0ull, frf);
             __syncthreads();
             if (i0f < nc) {
                 if (qn == PER) sg_step_count<PER, true>(xf, qn, nmf, ilf, cp, ncb, cells, j0, nu);
@@ -883,9 +1306,9 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
                 uint64_t nm;
                 int32_t fr[SG_RC];
                 int il;
-                sg_step_plan(s_cand, clist, nc, i0, nm, fr, il);
+                sg_step_plan(s_cand, crow, nc, i0, nm, fr, il);
                 uint64_t x[SG_RC][PER];
-                sg_step_load<PER>(x, seg_rows, sg.stride, w0, wn, nm, fr);
+                sg_step_load<PER>(x, seg_rows, vstride, w0, wn, nm, fr);
                 if (qn == PER) sg_step_count<PER, true>(x, qn, nm, il, cp, ncb, cells, j0, nu);
                 else sg_step_count<PER, false>(x, qn, nm, il, cp, ncb, cells, j0, nu);
             }
@@ -960,8 +1383,11 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             }
             for (int o = 32; o > 0; o >>= 1) needed += __shfl_xor(needed, o, 64);
             if (lane == 0) {
+                // words per count: the row length of the view the record counted on
+                const int64_t view = S.bview[g];
+                const int64_t vw = view >= 0 ? S.pdesc[view % S.pdesc_cap].words : sg.words;
                 atomicAdd(&s_counts, (unsigned long long)needed);
-                atomicAdd(&s_words, (unsigned long long)(needed + 1) * (unsigned long long)sg.words);
+                atomicAdd(&s_words, (unsigned long long)(needed + 1) * (unsigned long long)vw);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1004,9 +1430,14 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
                 nslot = __shfl(nslot, 0, 64);
                 if (nslot >= 0) {
                     float* fo = S.feat_on ? S.feat + nbi * SG_FEAT : nullptr;
+                    // the child counts on the block its parent counted on; that block has
+                    // applied the parent's first pk items (all of them for a block
+                    // projected at the parent this round)
+                    const int64_t view = S.bview[g];
+                    const int32_t pk = view >= 0 ? (S.bpnew[g] ? m.k : m.pk) : 0;
                     sg_make_record(S, sg, sup_all + seg_item_base[m.seg], pair, nslot, sX[wv], k, sup_e, e, m.seg,
                                    sF[wv], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot],
-                                   S.sib_off[g]);
+                                   S.sib_off[g], view, pk);
                     if (lane == 0) S.newborn[nbi] = nslot;
                 }
             }
@@ -1149,8 +1580,12 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
 {
     (void)uniform;
     hipError_t e;
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S, segs);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
+    if (S.proj_factor > 0) {
+        hipLaunchKernelGGL(k_proj_tids, dim3(2048), dim3(256), 0, s, S, rows, segs);
+        hipLaunchKernelGGL(k_proj_gather, dim3(2048), dim3(256), 0, s, S, rows, segs);
+    }
     hipLaunchKernelGGL(k_need_rows, dim3(1024), dim3(256), 0, s, S, segs, ok);
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
     switch (S.chunk_words) {

@@ -52,6 +52,24 @@ struct SgNodeMeta {
     int32_t nrange;     // row-chunk ranges of the record's window
     float score;        // priority score
     int64_t sib;        // the parent's sibling block in the sibling ring (-1: none, a root)
+    int64_t proj;       // projected block the record's subtree counts on (-1: none; see SgProj)
+    int32_t pk;         // leading items of X that the projected block has already applied
+    int32_t pad_;
+};
+
+// A projected block: the rows of U(Y) restricted to the transactions of T(Y) for an
+// itemset Y of the search, bit-compacted (bit j of a row = the j-th transaction of T(Y)).
+// Every descendant X of Y counts on it: T(X) = T(Y) & AND of X's later items, and a
+// candidate's row is |T(Y)| / 64 words instead of the window's. Blocks live in a ring of
+// words (the arena) and a ring of descriptors, with absolute positions that only grow; a
+// block is used while neither ring has wrapped onto it, otherwise its records fall back to
+// the window's rows. Row r of a block is the r-th item of U(Y) (ascending).
+struct SgProj {
+    int64_t off;        // absolute word position of row 0 in the arena ring
+    int64_t words;      // words per row: ceil(n / 64)
+    int64_t stride;     // words between rows (words rounded up to 8)
+    int32_t n;          // transactions: |T(Y)|
+    int32_t nrows;      // rows: |U(Y)|
 };
 
 // Launchers (kernels_*.hip); all launch on the given stream and return hipError_t.
@@ -153,6 +171,33 @@ struct SgSearchDev {
     int32_t chunk_range;  // chunks per counting unit
     int32_t count_grid;
     int32_t feat_on;     // compute the 24 features of newborn nodes (learned order or tracing)
+    // projected blocks (SgProj); proj_factor 0 turns them off
+    int32_t proj_factor;  // project a record when its support is at most 1/proj_factor of its view's transactions
+    int32_t pad_;
+    uint64_t* parena;     // [parena_cap + proj_max_block] words
+    int64_t parena_cap;
+    int64_t proj_max_block;  // largest block (words)
+    int64_t proj_round_words;  // arena words one round may allocate (blocks used in a round are
+                               // those the ring cannot reach within it)
+    SgProj* pdesc;        // [pdesc_cap]
+    uint64_t* pmask;      // [pdesc_cap][dw_max] U(Y) of each block
+    int32_t* prank;       // [pdesc_cap][dw_max] items of U(Y) in the words before each word
+    int64_t pdesc_cap;
+    int32_t* ptid;        // [ptid_cap] transaction lists T(Y) of the round's new blocks
+    int64_t ptid_cap;
+    // per batch record (bmax + 1 entries)
+    int64_t* bview;       // block the record counts on this round (-1: the window's rows)
+    int64_t* bsrc;        // a new block's source: the block it is projected from (-1: the window)
+    int32_t* bpnew;       // 1: the record's block is projected this round
+    int32_t* bnr;         // row-chunk ranges of the record's view
+    int64_t* bunitB;      // counting units of records on blocks (prefix; those records run record-major)
+    int64_t* bunitT;      // transaction-list units of new blocks (prefix)
+    int64_t* bunitG;      // gather units of new blocks (prefix)
+    int64_t* bpa;         // arena position of the record's new block (absolute; the prefix at the cut)
+    int64_t* bpd;         // descriptor id of the record's new block (absolute; the prefix at the cut)
+    int64_t* ptoff;       // a new block's transaction list in ptid
+    int64_t* pfill;       // transactions written into that list
+    int32_t* crow;        // [bmax * d_max] row of each U-list item in its record's view
 };
 
 // Control block layout (int64 slots of SgSearchDev::ctl).
@@ -178,7 +223,16 @@ enum SgCtl {
     CTL_CHILDREN,         // children to expand in the current batch
     CTL_CELLS,            // count-matrix cells of the current batch
     CTL_SIB_NEXT,         // sibling-ring words written so far (end of the current round's blocks)
-    CTL_SLOTS = 20 + SG_NBUCKETS
+    CTL_PROJ_NEXT,        // projected-block descriptors allocated so far (absolute)
+    CTL_ARENA_NEXT,       // projected-block arena words allocated so far (absolute)
+    CTL_UNITS_A,          // counting units of the records on window rows (range-major, first)
+    CTL_PROJ_TUNITS,      // transaction-list units of the round's new blocks
+    CTL_PROJ_GUNITS,      // gather units of the round's new blocks
+    CTL_PROJS,            // projected blocks built (cumulative)
+    CTL_PROJ_WORDS,       // words read and written building them (cumulative)
+    CTL_PROJ_ROWS,        // of them, projected from window rows (cumulative)
+    CTL_PROJ_ARENA,       // arena words of the blocks built (cumulative)
+    CTL_SLOTS = 32 + SG_NBUCKETS
 };
 
 struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)
@@ -199,6 +253,10 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
                           const uint64_t* ok, const int32_t* roots, int32_t n_roots,
                           const int32_t* sup_all, const int64_t* seg_item_base, int32_t order_mode,
                           const SgScorer* scorer, hipStream_t s);
+// Words per transaction-list unit and transactions per gather unit of the projection kernels.
+#define SG_PT_WORDS 1024
+#define SG_PG_TIDS 1024
+#define SG_PG_ROWS 64
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
                            const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,

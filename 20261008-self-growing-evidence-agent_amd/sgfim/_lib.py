@@ -43,7 +43,8 @@ class RoundStats(C.Structure):
                 ("frontier", C.c_int64), ("words", C.c_int64), ("pair_counts", C.c_int64),
                 ("count_launches", C.c_int64), ("count_ms", C.c_double), ("event_ms", C.c_double),
                 ("elapsed_s", C.c_double),
-                ("done", C.c_int32), ("status", C.c_int32)]
+                ("done", C.c_int32), ("status", C.c_int32), ("projections", C.c_int64), ("proj_words", C.c_int64),
+                ("proj_from_rows", C.c_int64), ("proj_block_words", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -69,6 +70,8 @@ _SIGS = {
     "sgfim_set_tile": (C.c_int, [P, C.c_int32, C.c_int32]),
     "sgfim_set_sib_ring": (C.c_int, [P, C.c_int64]),
     "sgfim_get_tile": (C.c_int, [P, P, P]),
+    "sgfim_set_projection": (C.c_int, [P, C.c_int32, C.c_int64]),
+    "sgfim_get_projection": (C.c_int, [P, P, P]),
     "sgfim_device_info": (C.c_int, [P, C.c_char_p, C.c_int32, P]),
     "sgfim_result_windows": (C.c_int, [P, PP]),
     "sgfim_load_zipf": (C.c_int, [P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,

@@ -313,6 +313,18 @@ class Context:
         on it, a larger ring prunes more counts on deep searches."""
         self._check(self._L.sgfim_set_sib_ring(self._h, int(words)), "set_sib_ring")
 
+    def set_projection(self, factor: int = 4, arena_words: int = 0):
+        """Projected blocks of later searches: a record whose support is at most 1/factor of
+        its rows' transactions counts its subtree on rows compacted to its transactions
+        (factor 0 = off; arena_words 0 = default ring size). Results do not depend on it."""
+        self._check(self._L.sgfim_set_projection(self._h, int(factor), int(arena_words)), "set_projection")
+
+    def projection(self) -> tuple:
+        """(factor, arena_words) the next search uses (arena_words 0 = default)."""
+        f, a = C.c_int32(), C.c_int64()
+        self._check(self._L.sgfim_get_projection(self._h, C.byref(f), C.byref(a)), "get_projection")
+        return f.value, a.value
+
     def tile(self) -> tuple:
         """(chunk_words, chunk_range) the next search uses."""
         cw, cr = C.c_int32(), C.c_int32()

@@ -105,6 +105,13 @@ typedef struct sgfim_round_stats {
     double elapsed_s;        /* wall time since sgfim_anytime_begin                        */
     int32_t done;            /* 1 when the frontier is empty: every MFI is confirmed       */
     int32_t status;          /* SGFIM_OK or the error that ended the run                   */
+    int64_t projections;     /* projected blocks built so far (sgfim_set_projection)       */
+    int64_t proj_words;      /* 64-bit words the projections read and wrote: each new
+                                block's unapplied prefix rows and the U rows' words at its
+                                transactions in its source view, plus its own rows         */
+    int64_t proj_from_rows;  /* of the projected blocks, those built from window rows (the
+                                others from an ancestor's block)                           */
+    int64_t proj_block_words;/* 64-bit words of the blocks built (their rows x strides)    */
 } sgfim_round_stats;
 
 /* ---- context (replaces the engine process: kernel_bitmap.cu:796 main) ---- */
@@ -180,6 +187,20 @@ int sgfim_set_tile(sgfim_ctx* ctx, int32_t chunk_words, int32_t chunk_range);
  * a larger ring prunes more counts on deep searches (1 = none). No reference counterpart. */
 int sgfim_set_sib_ring(sgfim_ctx* ctx, int64_t words);
 int sgfim_get_tile(const sgfim_ctx* ctx, int32_t* chunk_words, int32_t* chunk_range);
+
+/* Projected blocks of later searches. With factor >= 2, a frontier record X whose support
+ * is at most 1/factor of the transactions its rows span gets the rows of its frequent
+ * extensions U(X) restricted to T(X) and bit-compacted (|T(X)| / 64 words per row instead
+ * of the window's), built on the device in the round that counts X; X and its whole
+ * subtree then count on them, and a deeper record can be projected again from them. The
+ * reference has no counterpart: its counter always ANDs the full-length rows of every item
+ * of X for every node (kernel_bitmap.cu:390-442), which is what a record without a block
+ * does here. Blocks live in a ring of arena_words 64-bit words (0 = default: 32x the
+ * loaded rows and at least 2^28 words, at most a quarter of the device memory); a block that later blocks have overwritten is not
+ * used, its records count on the window's rows. factor 0 turns projection off. Results do
+ * not depend on either value. */
+int sgfim_set_projection(sgfim_ctx* ctx, int32_t factor, int64_t arena_words);
+int sgfim_get_projection(const sgfim_ctx* ctx, int32_t* factor, int64_t* arena_words);
 /* The context's device: its gcnArchName (NUL-terminated, at most len - 1 characters) and
  * compute units; the key of the tile autotuner's cache. No counterpart in the reference
  * (its Results header prints the device, kernel_bitmap.cu:608-690). */

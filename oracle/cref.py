@@ -58,6 +58,9 @@ def lib():
         L.kat_csr.argtypes = [P, C.c_int64, P, P]
         L.kat_levelwise.argtypes = [P, C.c_int64, C.c_int64, P, P, C.c_int64]
         L.kat_levelwise.restype = C.c_int64
+        L.zipf_bitmap_ref.argtypes = [C.c_int64, C.c_int32, P, C.c_int32, P, P, P, C.c_uint64, C.c_int64, P]
+        L.ref_antichain_violations.argtypes = [P, P, C.c_int64, C.c_int32]
+        L.ref_antichain_violations.restype = C.c_int64
         L.ref_simd_path.restype = C.c_int
         L.ref_set_simd.argtypes = [C.c_int]; L.ref_set_simd.restype = C.c_int
         _lib = L
@@ -209,3 +212,25 @@ def zipf_csr(n, d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, s
     items = np.zeros(max(nnz, 1), dtype=np.int32)
     lib().zipf_csr_ref(n, d, _p(q), n_planted, _p(poff), _p(pit_c), _p(pr_c), seed, _p(off), _p(items))
     return off, items[:nnz]
+
+
+def zipf_bitmap(n, d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed):
+    """Vertical bitmap rows [d, ceil(n / 64)] of the synthetic Zipf set (the draws of
+    zipf_csr), built with all host threads."""
+    q, poff, pit, pr = zipf_spec(d, s, mean_len, n_planted, size_lo, size_hi, rate_lo, rate_hi, seed)
+    pit_c = np.ascontiguousarray(pit if pit.size else np.zeros(1, np.int32))
+    pr_c = np.ascontiguousarray(pr if pr.size else np.zeros(1))
+    W = (n + 63) // 64
+    rows = np.empty((d, W), dtype=np.uint64)
+    lib().zipf_bitmap_ref(n, d, _p(q), n_planted, _p(poff), _p(pit_c), _p(pr_c), seed, W, _p(rows))
+    return rows
+
+
+def antichain_violations(items: np.ndarray, off: np.ndarray, d: int) -> int:
+    """Members of the family (items[off[i]:off[i+1]], ids 0..d-1) that are proper subsets of
+    another member; 0 for an antichain."""
+    it = np.ascontiguousarray(items, dtype=np.int32)
+    of = np.ascontiguousarray(off, dtype=np.int64)
+    if it.size == 0:
+        it = np.zeros(1, dtype=np.int32)
+    return int(lib().ref_antichain_violations(_p(it), _p(of), int(of.shape[0]) - 1, int(d)))

@@ -411,3 +411,62 @@ void ref_result_free(ref_result* r)
     if (!r) return;
     free(r->items); free(r->off); free(r->sup); free(r);
 }
+
+/* ---------------------------------------------------------------- antichain check
+ * Number of itemsets of a family that are proper subsets of another member (0 for an
+ * antichain, as every family of maximal itemsets is). Itemsets are items[off[i] ..
+ * off[i+1]) over ids 0..d-1 in any order. A superset Y of X contains X's item that lies
+ * in the fewest members, so only that item's members are compared with X: first by a
+ * 64-bit signature of item ids mod 64, then exactly on sorted copies. */
+static int cmp_i32(const void* a, const void* b)
+{
+    const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+int64_t ref_antichain_violations(const int32_t* items, const int64_t* off, int64_t n, int32_t d)
+{
+    const int64_t m = off[n];
+    int32_t* srt = (int32_t*)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    uint64_t* sig = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t* cnt = (int64_t*)calloc((size_t)d + 1, sizeof(int64_t));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t a = off[i], k = off[i + 1] - off[i];
+        memcpy(srt + a, items + a, sizeof(int32_t) * (size_t)k);
+        qsort(srt + a, (size_t)k, sizeof(int32_t), cmp_i32);
+        uint64_t s = 0;
+        for (int64_t j = 0; j < k; j++) s |= 1ULL << (srt[a + j] & 63);
+        sig[i] = s;
+    }
+    for (int64_t j = 0; j < m; j++) cnt[srt[j] + 1]++;
+    for (int32_t x = 0; x < d; x++) cnt[x + 1] += cnt[x];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(d > 0 ? d : 1));
+    int64_t* lst = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+    for (int32_t x = 0; x < d; x++) fill[x] = cnt[x];
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = off[i]; j < off[i + 1]; j++) lst[fill[srt[j]]++] = i;
+    int64_t bad = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : bad)
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t a = off[i], k = off[i + 1] - off[i];
+        if (k == 0) { bad += n > 1; continue; }
+        int32_t best = srt[a];
+        for (int64_t j = a + 1; j < a + k; j++)
+            if (cnt[srt[j] + 1] - cnt[srt[j]] < cnt[best + 1] - cnt[best]) best = srt[j];
+        for (int64_t p = cnt[best]; p < cnt[best + 1]; p++) {
+            const int64_t y = lst[p];
+            const int64_t b = off[y], ky = off[y + 1] - off[y];
+            if (y == i || ky <= k || (sig[i] & ~sig[y])) continue;
+            int64_t u = 0, v = 0;
+            while (u < k && v < ky) {
+                if (srt[a + u] == srt[b + v]) { u++; v++; }
+                else if (srt[a + u] > srt[b + v]) v++;
+                else break;
+            }
+            if (u == k) { bad++; break; }
+        }
+    }
+    free(srt); free(sig); free(cnt); free(fill); free(lst);
+    return bad;
+}

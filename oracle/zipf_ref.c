@@ -13,6 +13,9 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static uint64_t zmix(uint64_t z)
 {
@@ -84,4 +87,43 @@ int64_t zipf_csr_ref(int64_t n, int32_t d, const double* q, int32_t np, const in
     }
     free(q53); free(r53); free(mark);
     return nnz;
+}
+
+/* Vertical bitmap rows[(i-1) * W + (t >> 6)] of the same set (W = ceil(n / 64) words per
+ * row), drawn straight into the row layout: each 64-transaction word is owned by one
+ * OpenMP thread, so the draws need no synchronisation. The draws are those of
+ * zipf_csr_ref, transaction by transaction. */
+void zipf_bitmap_ref(int64_t n, int32_t d, const double* q, int32_t np, const int32_t* poff, const int32_t* pit,
+                     const double* prate, uint64_t seed, int64_t W, uint64_t* rows)
+{
+    const double two53 = 9007199254740992.0;
+    uint64_t* q53 = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)d);
+    uint64_t* r53 = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(np > 0 ? np : 1));
+    for (int32_t i = 0; i < d; i++) q53[i] = (uint64_t)(q[i] * two53);
+    for (int32_t j = 0; j < np; j++) r53[j] = (uint64_t)(prate[j] * two53);
+    const uint64_t sp = seed ^ 0x5851F42D4C957F2DULL;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t w = 0; w < W; w++) {
+        for (int32_t i = 0; i < d; i++) {
+            uint64_t v = 0;
+            for (int b = 0; b < 64; b++) {
+                const int64_t t = w * 64 + b;
+                if (t >= n) break;
+                const uint64_t tk = (uint64_t)t * 0x9E3779B97F4A7C15ULL;
+                v |= (uint64_t)((zmix(seed ^ tk ^ ((uint64_t)(i + 1) * 0xC2B2AE3D27D4EB4FULL)) >> 11) < q53[i]) << b;
+            }
+            rows[(size_t)i * (size_t)W + (size_t)w] = v;
+        }
+        for (int32_t j = 0; j < np; j++) {
+            uint64_t v = 0;
+            for (int b = 0; b < 64; b++) {
+                const int64_t t = w * 64 + b;
+                if (t >= n) break;
+                const uint64_t tk = (uint64_t)t * 0x9E3779B97F4A7C15ULL;
+                v |= (uint64_t)((zmix(sp ^ tk ^ ((uint64_t)(j + 1) * 0xC2B2AE3D27D4EB4FULL)) >> 11) < r53[j]) << b;
+            }
+            for (int32_t a = poff[j]; a < poff[j + 1]; a++) rows[(size_t)(pit[a] - 1) * (size_t)W + (size_t)w] |= v;
+        }
+    }
+    free(q53); free(r53);
 }

@@ -205,3 +205,26 @@ def test_zipf_spec_library_matches_oracle():
     q, poff, pit, pr = cref.zipf_spec(300, 1.1, 20.0, 7, 3, 6, 0.01, 0.1, 9)
     extra = float((np.diff(poff) * pr).sum())
     assert 20.0 - 0.5 < np.diff(off).mean() < 20.0 + extra + 0.5
+
+
+def test_zipf_bitmap_equals_csr_draws():
+    """The multithreaded bitmap regenerator draws exactly the CSR generator's set."""
+    for n, d, mean, npl, seed in ((1000, 70, 9.0, 3, 1), (4097, 300, 30.0, 8, 5)):
+        args = (n, d, 1.1, mean, npl, 3, 8, 0.01, 0.08, seed)
+        off, items = cref.zipf_csr(*args)
+        assert np.array_equal(cref.zipf_bitmap(*args), cref.bitmap(off, items, d))
+
+
+def test_antichain_check_against_brute_force():
+    rng = np.random.default_rng(3)
+    for case in range(40):
+        d = int(rng.integers(3, 9))
+        fam = {tuple(sorted(rng.choice(d, size=int(rng.integers(1, d + 1)), replace=False).tolist()))
+               for _ in range(int(rng.integers(1, 12)))}
+        fam = [list(f) for f in fam]
+        for f in fam:
+            rng.shuffle(f)
+        it = np.array([x for f in fam for x in f], np.int32)
+        off = np.cumsum([0] + [len(f) for f in fam]).astype(np.int64)
+        want = sum(1 for a in fam if any(set(a) < set(b) for b in fam))
+        assert cref.antichain_violations(it, off, d) == want, fam
