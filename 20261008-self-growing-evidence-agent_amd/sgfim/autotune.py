@@ -8,10 +8,15 @@ candidate rows through registers. Two knobs shape it and neither changes a resul
   chunk_range  chunks per work unit: the unit's set-up (need masks, the sorted candidate
                list -- the candidates a workgroup takes) is paid once per range, and a
                range of every record is counted before the next (its rows stay in L2).
-tune() times a caller's search for every candidate shape on the loaded data, checks that
-every shape returns the same MFIs, keeps the fastest, and records it in a JSON cache keyed
-by device and input shape; apply() sets a cached shape. `python -m sgfim.autotune` tunes
-the benchmark workloads and writes tile_cache.json next to this file.
+A third knob is the projection factor (sgfim_set_projection): a record whose support is at
+most 1/factor of its rows' transactions gets its U rows compacted to its transactions, and
+its subtree counts on them. Deep lattices over long rows gain (workload 5), shallow ones
+pay the projection without reusing it (workload 2), so it is tuned per input shape too.
+tune() times a caller's search for every projection factor on the loaded data, then for every
+candidate shape with the chosen factor, checks that every setting returns the same MFIs,
+keeps the fastest, and records it in a JSON cache keyed by device and input shape; apply()
+sets a cached setting. `python -m sgfim.autotune` tunes the benchmark workloads and writes
+tile_cache.json next to this file.
 """
 from __future__ import annotations
 
@@ -24,6 +29,8 @@ import numpy as np
 
 CANDIDATES = ((128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (512, 1), (512, 2), (512, 4))
 DEFAULT = (256, 4)
+PROJ_CANDIDATES = (0, 4, 8, 16, 32)
+PROJ_DEFAULT = 4   # the library's default factor
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_cache.json")
 
 
@@ -64,10 +71,20 @@ def lookup(key: str, path: str = CACHE):
     return (int(e["chunk_words"]), int(e["chunk_range"])) if e else None
 
 
+def lookup_projection(key: str, path: str = CACHE):
+    """Projection factor cached for key, or None."""
+    e = load_cache(path).get(key)
+    return int(e["proj_factor"]) if e and "proj_factor" in e else None
+
+
 def apply(ctx, n_sets: int = 1, path: str = CACHE):
-    """Set the cached shape for this context's device and data; returns it (or the default)."""
-    t = lookup(context_key(ctx, n_sets), path)
+    """Set the cached tile shape and projection factor for this context's device and data;
+    returns the shape (or the default). Without a cached factor the library default stays."""
+    key = context_key(ctx, n_sets)
+    t = lookup(key, path)
     ctx.set_tile(*(t or (0, 0)))
+    f = lookup_projection(key, path)
+    ctx.set_projection(PROJ_DEFAULT if f is None else f)
     return t or DEFAULT
 
 
@@ -110,35 +127,60 @@ def signature(res) -> tuple:
         return (n, int(np.sum(h, dtype=np.uint64)))
 
 
+def _time(run, reps: int) -> float:
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    return 1e3 * min(ts)
+
+
 def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None, path: str | None = CACHE,
-         verify: bool = True, log=None) -> dict:
-    """Time run() (a search on ctx returning a MineResult) under every candidate shape and
-    keep the fastest (best of reps wall times after one warm-up run, which also captures
-    the round graphs). With verify, every shape must give the same MFIs. The chosen shape
-    is left set on ctx and, when key and path are given, stored in the cache."""
-    timings, ref = {}, None
-    for cw, cr in candidates:
-        ctx.set_tile(cw, cr)
+         verify: bool = True, log=None, proj_candidates=PROJ_CANDIDATES) -> dict:
+    """Time run() (a search on ctx returning a MineResult) under every projection factor with
+    the default tile shape, keep the fastest factor, then time every candidate tile shape
+    with that factor and keep the fastest (each time the best of reps wall times after one
+    warm-up run, which also captures the round graphs). With verify, every setting must give
+    the same MFIs. The chosen setting is left on ctx and, when key and path are given, stored
+    in the cache."""
+    ref = None
+
+    def warm_check(what):
+        nonlocal ref
         r = run()
         if verify:
             sig = signature(r)
             if ref is None:
                 ref = sig
             elif sig != ref:
-                raise RuntimeError(f"tile {cw}x{cr} changed the result")
-        ts = []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            run()
-            ts.append(time.perf_counter() - t0)
-        timings[(cw, cr)] = 1e3 * min(ts)
+                raise RuntimeError(f"{what} changed the result")
+
+    ctx.set_tile(*DEFAULT)
+    ptimes = {}
+    for f in proj_candidates:
+        ctx.set_projection(f)
+        warm_check(f"projection factor {f}")
+        ptimes[f] = _time(run, reps)
+        if log:
+            log(f"projection factor {f}: {ptimes[f]:.3f} ms")
+    pf = min(ptimes, key=lambda f: (ptimes[f], f != PROJ_DEFAULT))
+    ctx.set_projection(pf)
+    timings = {DEFAULT: ptimes[pf]}
+    for cw, cr in candidates:
+        if (cw, cr) == DEFAULT:
+            continue
+        ctx.set_tile(cw, cr)
+        warm_check(f"tile {cw}x{cr}")
+        timings[(cw, cr)] = _time(run, reps)
         if log:
             log(f"tile chunk_words={cw} chunk_range={cr}: {timings[(cw, cr)]:.3f} ms")
-    best = choose(timings)
+    best = choose({sh: timings[sh] for sh in candidates if sh in timings})
     ctx.set_tile(*best)
-    entry = {"chunk_words": best[0], "chunk_range": best[1], "ms": round(timings[best], 4),
-             "default_ms": round(timings.get(DEFAULT, float("nan")), 4),
-             "all": {f"{cw}x{cr}": round(ms, 4) for (cw, cr), ms in timings.items()}}
+    entry = {"chunk_words": best[0], "chunk_range": best[1], "proj_factor": pf, "ms": round(timings[best], 4),
+             "default_ms": round(ptimes.get(PROJ_DEFAULT, float("nan")), 4),
+             "all": {f"{cw}x{cr}": round(ms, 4) for (cw, cr), ms in timings.items()},
+             "proj_all": {str(f): round(ms, 4) for f, ms in sorted(ptimes.items())}}
     if key and path:
         save_entry(key, entry, path)
     return entry
@@ -146,7 +188,8 @@ def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None,
 
 def main(argv=None):
     """Tune the benchmark's workload 2 (three supports in one search), workload 3 (512
-    windows) and workload 4 (rounds of up to 4096 records) on device 0 and write the cache."""
+    windows), workload 4 (rounds of up to 4096 records) and workload 5 (the whole search to
+    the 0.5% floor in the learned order) on device 0 and write the cache."""
     import argparse
     from . import gen
     from .core import Context
@@ -157,7 +200,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     say = lambda m: print(m, flush=True)
     for w in a.workloads.split(","):
-        ctx = Context(device=0, batch_nodes=4096 if w == "4" else 0)
+        ctx = Context(device=0, batch_nodes=4096 if w == "4" else 0, frontier_cap=(1 << 23) if w == "5" else 0)
         if w == "2":
             T = gen.workload2(0)
             ctx.load_csr(T.offsets, T.items)
@@ -175,6 +218,14 @@ def main(argv=None):
             ctx.load_csr(T.offsets, T.items)
             thr = gen.min_count(0.02, T.n_txn)
             run = lambda: ctx.mine(thr, convert=False)
+            n_sets = 1
+        elif w == "5":   # the dense-sparse Zipf set to its floor, learned frontier order
+            from . import learned
+            from ._lib import ORDER_LEARNED
+            ctx.load_zipf(seed=0, **gen.WORKLOAD5)
+            ctx.set_scorer(*learned.load_weights())
+            thr = gen.min_count(0.005, ctx.n_txn)
+            run = lambda: ctx.mine(thr, flags=ORDER_LEARNED, convert=False)
             n_sets = 1
         else:
             raise SystemExit(f"unknown workload {w}")

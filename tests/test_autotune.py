@@ -32,6 +32,19 @@ def test_committed_cache_entries_are_valid_shapes():
     for key, e in A.load_cache().items():
         assert (e["chunk_words"], e["chunk_range"]) in A.CANDIDATES, key
         assert e["ms"] <= e["default_ms"] + 1e-9, key      # the choice is the fastest measured
+        if "proj_factor" in e:
+            assert e["proj_factor"] in A.PROJ_CANDIDATES, key
+            assert e["ms"] <= min(e["proj_all"].values()) + 1e-9, key   # tile tuned with the best factor
+            assert e["all"][f"{A.DEFAULT[0]}x{A.DEFAULT[1]}"] == e["proj_all"][str(e["proj_factor"])], key
+
+
+def test_projection_lookup(tmp_path):
+    p = str(tmp_path / "tiles.json")
+    A.save_entry("k", {"chunk_words": 256, "chunk_range": 4, "proj_factor": 16, "ms": 1.0}, p)
+    A.save_entry("old", {"chunk_words": 256, "chunk_range": 4, "ms": 1.0}, p)
+    assert A.lookup_projection("k", p) == 16
+    assert A.lookup_projection("old", p) is None
+    assert A.lookup_projection("absent", p) is None
 
 
 def test_signature_is_order_free_and_sensitive():
