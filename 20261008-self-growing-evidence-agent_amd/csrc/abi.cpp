@@ -48,7 +48,7 @@ struct SearchBufs {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int32_t graph_feat[2] = {0, 0};                 // the D.feat_on each round was captured with
     uint64_t graph_gen[2] = {0, 0};                 // the search universe each round was captured on
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;        // around the counting kernel (direct launches)
+    hipEvent_t ev[SG_STAGES + 1] = {};             // around the round's stages (direct launches)
     int64_t* h_ctl = nullptr;                       // pinned: processed copy of the control block, then
                                                     // two snapshots of rounds in flight
     hipEvent_t snap_ev[2] = {nullptr, nullptr};     // a round and its snapshot have completed
@@ -94,6 +94,7 @@ struct SearchState {
     int32_t n_roots = 0;
     double count_ms = 0.0;           // device wall-clock duration of the counting kernel
     double event_ms = 0.0;           // HIP-event duration (direct launches only)
+    double stage_ms[SG_STAGES] = {};   // HIP-event duration of each round stage (direct launches only)
     int64_t count_launches = 0;
     SearchBufs* B = nullptr;         // the context's buffers used by this run
     Universe U;                      // the items it searches
@@ -283,8 +284,8 @@ void bufs_free(sgfim_ctx* c)
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->h_ring_items) (void)hipHostFree(B->h_ring_items);
     if (B->h_ring_meta) (void)hipHostFree(B->h_ring_meta);
-    if (B->ev0) (void)hipEventDestroy(B->ev0);
-    if (B->ev1) (void)hipEventDestroy(B->ev1);
+    for (int i = 0; i <= SG_STAGES; i++)
+        if (B->ev[i]) (void)hipEventDestroy(B->ev[i]);
     for (int i = 0; i < 2; i++)
         if (B->snap_ev[i]) (void)hipEventDestroy(B->snap_ev[i]);
     dev_free_all(c, B->bufs);
@@ -651,7 +652,7 @@ int launch_round(sgfim_ctx* c, SearchState* S)
     (void)hipGetLastError();
     if (!S->use_graph) {
         HIPCHK(c, sg_search_round(B->D, S->U.rows, c->d_vseg, S->U.pair, S->U.ok, S->U.sup, c->d_vseg_ibase,
-                                  uniform, S->order_mode, &c->scorer, B->ev0, B->ev1, c->stream));
+                                  uniform, S->order_mode, &c->scorer, B->ev, c->stream));
         return SGFIM_OK;
     }
     const int g = S->order_mode;
@@ -661,7 +662,7 @@ int launch_round(sgfim_ctx* c, SearchState* S)
         B->graph_gen[g] = S->U.gen;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = sg_search_round(B->D, S->U.rows, c->d_vseg, S->U.pair, S->U.ok, S->U.sup,
-                                       c->d_vseg_ibase, uniform, S->order_mode, &c->scorer, nullptr, nullptr,
+                                       c->d_vseg_ibase, uniform, S->order_mode, &c->scorer, nullptr,
                                        c->stream);
         hipError_t e2 = hipStreamEndCapture(c->stream, &B->graph[g]);
         if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round capture failed: %s", hipGetErrorString(e));
@@ -778,8 +779,7 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     }
     HIPCHK(c, hipHostGetDevicePointer((void**)&D.ring_items, B->h_ring_items, 0));
     HIPCHK(c, hipHostGetDevicePointer((void**)&D.ring_meta, B->h_ring_meta, 0));
-    HIPCHK(c, hipEventCreate(&B->ev0));
-    HIPCHK(c, hipEventCreate(&B->ev1));
+    for (int i = 0; i <= SG_STAGES; i++) HIPCHK(c, hipEventCreate(&B->ev[i]));
     for (int i = 0; i < 2; i++) HIPCHK(c, hipEventCreateWithFlags(&B->snap_ev[i], hipEventDisableTiming));
     return SGFIM_OK;
 }
@@ -1853,10 +1853,13 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
             // workgroup end, from the constant-rate wall clock
             const int64_t t0 = B->h_ctl[CTL_T0], t1 = B->h_ctl[CTL_T1];
             if (t1 > t0 && c->wall_khz > 0) S->count_ms += (double)(t1 - t0) / (double)c->wall_khz;
-            if (!S->use_graph) {
-                float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, B->ev0, B->ev1) == hipSuccess) S->event_ms += ms;
-                else (void)hipGetLastError();
+            if (!S->use_graph) {   // one round in flight: its stage events are this round's
+                for (int i = 0; i < SG_STAGES; i++) {
+                    float ms = 0.0f;
+                    if (hipEventElapsedTime(&ms, B->ev[i], B->ev[i + 1]) == hipSuccess) S->stage_ms[i] += ms;
+                    else (void)hipGetLastError();
+                    if (i == 3) S->event_ms += ms;
+                }
             }
         }
         if (rc != SGFIM_OK) continue;   // an earlier failure: only wait for the rounds in flight
@@ -1983,6 +1986,13 @@ int sgfim_set_thresholds(sgfim_ctx* c, const int64_t* min_counts, const int32_t*
     }
     c->multi_thr.assign(min_counts, min_counts + n);
     if (root_topk) c->multi_topk.assign(root_topk, root_topk + n);
+    return SGFIM_OK;
+}
+
+int sgfim_stage_times(const sgfim_ctx* c, double* ms, int32_t n)
+{
+    if (!c || !ms || n < 0) return SGFIM_E_INVAL;
+    for (int32_t i = 0; i < n; i++) ms[i] = (c->S && i < SG_STAGES) ? c->S->stage_ms[i] : 0.0;
     return SGFIM_OK;
 }
 

@@ -1576,27 +1576,34 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
                            const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,
-                           const SgScorer* scorer, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s)
+                           const SgScorer* scorer, const hipEvent_t* ev, hipStream_t s)
 {
     (void)uniform;
     hipError_t e;
+    // ev (direct launches only): SG_STAGES + 1 events bracketing the round's stages
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], s) : hipSuccess; };
+    if ((e = mark(0)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, s, S, segs);
     hipLaunchKernelGGL(k_candlist, dim3((unsigned)((S.bmax + 3) / 4)), dim3(256), 0, s, S, segs);
+    if ((e = mark(1)) != hipSuccess) return e;
     if (S.proj_factor > 0) {
         hipLaunchKernelGGL(k_proj_tids, dim3(2048), dim3(256), 0, s, S, rows, segs);
         hipLaunchKernelGGL(k_proj_gather, dim3(2048), dim3(256), 0, s, S, rows, segs);
     }
+    if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_need_rows, dim3(1024), dim3(256), 0, s, S, segs, ok);
-    if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
+    if ((e = mark(3)) != hipSuccess) return e;
     switch (S.chunk_words) {
     case 128: hipLaunchKernelGGL(k_count_group<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
     case 512: hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
     default: hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
     }
-    if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
+    if ((e = mark(4)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,
                        seg_item_base);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(5)) != hipSuccess) return e;
     if ((e = sg_commit(S, segs, order_mode, scorer, 1, s)) != hipSuccess) return e;
+    if ((e = mark(6)) != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -257,10 +257,13 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 #define SG_PT_WORDS 1024
 #define SG_PG_TIDS 1024
 #define SG_PG_ROWS 64
+// Stages of a search round, timed with events on direct launches: selection + U lists,
+// projected blocks, need rows, counting, expansion, scoring + commit.
+#define SG_STAGES 6
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
                            const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,
-                           const SgScorer* scorer, hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);
+                           const SgScorer* scorer, const hipEvent_t* ev, hipStream_t s);
 
 // Proposal network (kernels_mlp.hip).
 #define SG_FEAT 24

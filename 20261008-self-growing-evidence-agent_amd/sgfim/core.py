@@ -344,6 +344,14 @@ class Context:
         self._check(self._L.sgfim_copy_probe(self._h, int(nbytes), int(iters), C.byref(g)), "copy_probe")
         return g.value
 
+    STAGES = ("select", "project", "need_rows", "count", "expand", "commit")
+
+    def stage_times(self) -> dict:
+        """Device ms of each round stage of the latest search run with NO_GRAPH (zeros otherwise)."""
+        out = np.zeros(len(self.STAGES), dtype=np.float64)
+        self._check(self._L.sgfim_stage_times(self._h, _p(out), len(out)), "stage_times")
+        return dict(zip(self.STAGES, out.tolist()))
+
     def last_stats(self) -> dict:
         st = L.RoundStats()
         self._L.sgfim_last_stats(self._h, C.byref(st))

@@ -307,6 +307,14 @@ int sgfim_copy_probe(sgfim_ctx* ctx, int64_t bytes, int32_t iters, double* gbps)
 /* Cumulative search statistics of the last sgfim_mine / anytime run. */
 int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
 
+/* Device time (ms, HIP events) of each stage of the current or latest search, summed over
+ * its rounds, for searches run with SGFIM_NO_GRAPH (one round at a time, every launch
+ * bracketed); 0 otherwise. ms receives min(n, 6) values: selection and U lists, projected
+ * blocks, need rows, counting, expansion, scoring and commit. A profile of the round's
+ * kernels without an external profiler; no reference counterpart (the engines print phase
+ * totals, kernel_bmma.cu:985-986). */
+int sgfim_stage_times(const sgfim_ctx* ctx, double* ms, int32_t n);
+
 /* Proposal network (24 -> 256 ReLU -> 1): load BF16 weights used by
  * SGFIM_ORDER_LEARNED. w1 is [24][256] row-major, b1 [256], w2 [256], b2 [1], all
  * float32 here and rounded to BF16 on the device. A later call overwrites the weights in
