@@ -115,6 +115,7 @@ struct sgfim_ctx {
     std::string dev_name;               // gcnArchName of the device
     int32_t tile_cw = 0, tile_cr = 0;   // counting-tile shape of the search (0: default)
     int64_t sib_words = 0;              // sibling-ring words of the search (0: default)
+    void* scratch = nullptr;            // this device's static count-batch scratch (sg_count_scratch)
     int32_t proj_factor = 4;            // projected blocks: factor (0: off)
     int64_t proj_arena = 0;             // projected-block arena words (0: default)
     sgfim_opts opts{};
@@ -834,6 +835,15 @@ int sgfim_create(const sgfim_opts* opts, sgfim_ctx** out)
             (void)hipGetLastError();
             c->pair_ev[i] = nullptr;
         }
+    // the device address of this device's allocation-free scratch, looked up once per device
+    if (sg_count_scratch(c->device, &c->scratch) != hipSuccess || !c->scratch) {
+        (void)hipGetLastError();
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        for (int i = 0; i < 2; i++)
+            if (c->pair_ev[i]) (void)hipEventDestroy(c->pair_ev[i]);
+        delete c;
+        return SGFIM_E_DEVICE;
+    }
     *out = c;
     return SGFIM_OK;
 }
@@ -1491,8 +1501,11 @@ int sgfim_pair_time(sgfim_ctx* c, double* ms)
 }
 int sgfim_pair_matrix_valu(sgfim_ctx* c, int32_t* out) { return pair_out(c, out, true); }
 
-int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
-                      const int32_t* cand, const int64_t* cand_off, int32_t* out)
+// The counting behind sgfim_count_batch and sgfim_count_batch_noalloc: the device scratch
+// (node descriptors, compacted candidates, counts, work counter) is allocated per call, or
+// taken from the device's static scratch (noalloc: no allocation at all, sizes bounded).
+static int count_batch_impl(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
+                            const int32_t* cand, const int64_t* cand_off, int32_t* out, bool noalloc)
 {
     (void)hipGetLastError();
     if (!c || n_prefix < 0 || (n_prefix > 0 && (!prefix_off || !cand_off))) return SGFIM_E_INVAL;
@@ -1500,6 +1513,8 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
     if (c->nseg != 1) return fail(c, SGFIM_E_STATE, "count_batch needs a single loaded transaction set");
     if (n_prefix == 0) return SGFIM_OK;
     if (prefix_off[0] != 0 || cand_off[0] != 0) return fail(c, SGFIM_E_INVAL, "offsets must start at 0");
+    if (noalloc && n_prefix > SG_NOALLOC_NODES)
+        return fail(c, SGFIM_E_INVAL, "count_batch_noalloc: %d prefixes, at most %d", n_prefix, SG_NOALLOC_NODES);
     // original id -> dense id
     auto dense = [&](int32_t id) -> int32_t {
         auto it = std::lower_bound(c->ids.begin(), c->ids.end(), id);
@@ -1547,6 +1562,9 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
         std::fill(out, out + ncand_total, 0);
         return SGFIM_OK;
     }
+    if (noalloc && (int64_t)dc.size() > SG_NOALLOC_CANDS)
+        return fail(c, SGFIM_E_INVAL, "count_batch_noalloc: %lld candidates present in the set, at most %d",
+                    (long long)dc.size(), SG_NOALLOC_CANDS);
     (void)hipSetDevice(c->device);
     SgSegment seg{};
     seg.row_base = 0;
@@ -1561,8 +1579,14 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
     SgCountNode* d_nodes = nullptr;
     int32_t *d_cand = nullptr, *d_out = nullptr, *d_ctr = nullptr;
     int rc;
-    if ((rc = dev_alloc_t(c, tmp, &d_nodes, nodes.size())) || (rc = dev_alloc_t(c, tmp, &d_cand, dc.size())) ||
-        (rc = dev_alloc_t(c, tmp, &d_out, dc.size())) || (rc = dev_alloc_t(c, tmp, &d_ctr, 1))) {
+    if (noalloc) {   // the device's static scratch: [nodes | candidate ids | counts | counter]
+        unsigned char* b = (unsigned char*)c->scratch;
+        d_nodes = (SgCountNode*)b;
+        d_cand = (int32_t*)(b + SG_NOALLOC_NODE_BYTES);
+        d_out = d_cand + SG_NOALLOC_CANDS;
+        d_ctr = d_out + SG_NOALLOC_CANDS;
+    } else if ((rc = dev_alloc_t(c, tmp, &d_nodes, nodes.size())) || (rc = dev_alloc_t(c, tmp, &d_cand, dc.size())) ||
+               (rc = dev_alloc_t(c, tmp, &d_out, dc.size())) || (rc = dev_alloc_t(c, tmp, &d_ctr, 1))) {
         dev_free_all(c, tmp);
         return rc;
     }
@@ -1580,6 +1604,18 @@ int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* pr
     std::fill(out, out + ncand_total, 0);
     for (size_t j = 0; j < src.size(); j++) out[src[j]] = res[j];
     return SGFIM_OK;
+}
+
+int sgfim_count_batch(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
+                      const int32_t* cand, const int64_t* cand_off, int32_t* out)
+{
+    return count_batch_impl(c, prefix_ids, prefix_off, n_prefix, cand, cand_off, out, false);
+}
+
+int sgfim_count_batch_noalloc(sgfim_ctx* c, const int32_t* prefix_ids, const int64_t* prefix_off, int32_t n_prefix,
+                              const int32_t* cand, const int64_t* cand_off, int32_t* out)
+{
+    return count_batch_impl(c, prefix_ids, prefix_off, n_prefix, cand, cand_off, out, true);
 }
 
 // ------------------------------------------------------------------ search

@@ -6,6 +6,8 @@
 // persistent grid drains from one device work counter.
 #include "count_core.h"
 
+#include <mutex>
+
 template <int CW>
 __global__ __launch_bounds__(256) void k_count_nodes(const uint64_t* __restrict__ rows,
                                                      const SgSegment* __restrict__ segs,
@@ -74,4 +76,34 @@ hipError_t sg_launch_count_nodes(const uint64_t* rows, const SgSegment* segs,
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ allocation-free scratch
+// Device scratch of sgfim_count_batch_noalloc: node descriptors, compacted candidate ids,
+// their counts and the work counter, in one static array per device (each GPU the library
+// runs on holds its own copy until the process ends). Host code must not pass `scratch`
+// itself to a kernel or copy (its name is a host-side stand-in); sg_count_scratch() looks
+// up each device's address once and every caller uses that.
+static __device__ __align__(256) unsigned char scratch[SG_NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SG_NOALLOC_MAX_DEVICES] = {};
+
+hipError_t sg_count_scratch(int device, void** ptr)
+{
+    if (device < 0 || device >= SG_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
 }

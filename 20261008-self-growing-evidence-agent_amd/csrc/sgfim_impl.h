@@ -109,6 +109,16 @@ struct SgCountNode {
     int64_t unit_off;   // first work unit of this node
     int32_t items[SG_MAX_DEPTH];
 };
+// Allocation-free count batches: at most SG_NOALLOC_NODES prefixes and SG_NOALLOC_CANDS
+// candidates per call, in a static device scratch of SG_NOALLOC_BYTES per device laid out as
+// [nodes | candidate ids | counts | work counter].
+#define SG_NOALLOC_NODES 16384
+#define SG_NOALLOC_CANDS (1 << 20)
+#define SG_NOALLOC_NODE_BYTES ((size_t)SG_NOALLOC_NODES * sizeof(SgCountNode))
+#define SG_NOALLOC_BYTES (SG_NOALLOC_NODE_BYTES + (size_t)SG_NOALLOC_CANDS * 8 + 256)
+#define SG_NOALLOC_MAX_DEVICES 64
+// The device address of `device`'s scratch (looked up once per device, then cached).
+hipError_t sg_count_scratch(int device, void** ptr);
 hipError_t sg_launch_count_nodes(const uint64_t* rows, const SgSegment* segs,
                                  const SgCountNode* nodes, int32_t n_nodes, int64_t n_units,
                                  const int32_t* cand, int32_t* out, int32_t chunk_words,

@@ -72,6 +72,7 @@ _SIGS = {
     "sgfim_get_tile": (C.c_int, [P, P, P]),
     "sgfim_set_projection": (C.c_int, [P, C.c_int32, C.c_int64]),
     "sgfim_stage_times": (C.c_int, [P, P, C.c_int32]),
+    "sgfim_count_batch_noalloc": (C.c_int, [P, P, P, C.c_int32, P, P, P]),
     "sgfim_get_projection": (C.c_int, [P, P, P]),
     "sgfim_device_info": (C.c_int, [P, C.c_char_p, C.c_int32, P]),
     "sgfim_result_windows": (C.c_int, [P, PP]),

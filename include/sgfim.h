@@ -264,6 +264,18 @@ int sgfim_count_batch(sgfim_ctx* ctx, const int32_t* prefix_ids, const int64_t* 
                       int32_t n_prefix, const int32_t* cand, const int64_t* cand_off,
                       int32_t* out);
 
+/* sgfim_count_batch without any allocation: no counterpart in the reference. It is for
+ * callers that cannot allocate device memory (code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, results and errors as
+ * sgfim_count_batch; the device scratch (node descriptors, candidate ids, counts, work
+ * counter) is a static array of each device, 10.5 MB, set up when the context is created.
+ * At most 16384 prefixes and 2^20 candidates present in the set per call (more:
+ * SGFIM_E_INVAL). Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several contexts or threads) race and must be serialised by the caller. */
+int sgfim_count_batch_noalloc(sgfim_ctx* ctx, const int32_t* prefix_ids, const int64_t* prefix_off,
+                              int32_t n_prefix, const int32_t* cand, const int64_t* cand_off,
+                              int32_t* out);
+
 /* Exact MFI mining at one threshold; blocks until the search ends.
  * Replaces the engine run CoParaCG_<baseline|bmma>.exe <fimi> <rel_thr>
  * (kernel_bitmap.cu:796-910, kernel_bmma.cu:1179-1295) and one AnyFIM stage
