@@ -65,6 +65,7 @@ def _take_result(h: C.c_void_p, stats: dict | None = None, convert: bool = True)
             s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
             res.arrays = (it, o, s)
             res.windows = np.ctypeslib.as_array(C.cast(win, C.POINTER(C.c_int32)), shape=(nn,)).copy()
+            res.times = np.ctypeslib.as_array(C.cast(tms, C.POINTER(C.c_double)), shape=(nn,)).copy()
         elif nn:
             o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
             it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()

@@ -1,19 +1,24 @@
 """Benchmark of the MFI support-counting hot path on one MI355X.
 
-Workload (workload 2 of the benchmark set): 1,000,000 transactions from a seeded
-64-sensor x 8-bin regime-switching AR(1) stream (512 items, one item per sensor),
-held as vertical uint64 bitmaps in HBM. One step = the int8-MFMA pair matrix plus
-exact MFI mining at min supports 10%, 5% and 2% (every support counted exactly).
+Headline workload (workload 2 of the benchmark set): 1,000,000 transactions from a seeded
+64-sensor x 8-bin regime-switching AR(1) stream (512 items, one item per sensor), held as
+vertical uint64 bitmaps in HBM. One step = the int8-MFMA pair matrix plus exact MFI mining
+at min supports 10%, 5% and 2% in one batched search (every support counted exactly).
 
-Prints one JSON line: whole-job support counts/s (value), the counting kernel's
-roofline against MI355X HBM, and a CPU baseline (the C restatement of the
-reference engine's CPU search, timed on this host on a bounded sample).
+Prints one JSON line: whole-job support counts/s (value), the counting kernel against the
+VALU issue peak (the bound that holds for it) with its HBM traffic from the committed
+rocprofv3 counter profile, the pair kernel against the int8 MFMA peak, and a CPU baseline
+(the C restatement of the reference engine's CPU search, timed on this host). The extra
+records hold the other workloads, among them workload 5 (10M transactions x 2,048 Zipf
+items, 2.56 GB of bitmaps) mined to its 0.5% floor to completion in both frontier orders.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,24 +33,63 @@ INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA
 # 32-bit VALU lane-ops/s: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (a wave64 instruction
 # issues over four cycles)
 VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
-# HBM bytes per k_count_group dispatch for this command, from the two rocprofv3 --pmc
-# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean over the run), scaled by
-# what the same counters read for the 2 GiB copy probe (FETCH x2.0, WRITE x1.0);
-# profiles/k_count_group_workload2_pmc.md holds the commands and the table.
-PMC_COUNT_KERNEL = {"fetch_kib": 116982.0, "write_kib": 61325.3, "fetch_scale": 2.0, "write_scale": 1.0}
 SUPPORTS = (0.10, 0.05, 0.02)
+PMC_PROFILE = os.path.join(ROOT, "profiles", "k_count_group_workload2_pmc.md")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(T, min_s: float = 10.0):
-    """The oracle's restatement of the reference CPU search (count every survivor per
-    node, kernel_bitmap.cu:390-408 / :853-894) on all host cores, over the step's own
-    workload: workload 2 at min supports 10%, 5% and 2%, repeated whole until at least
-    min_s seconds have passed (about 2.5 s per repetition on 16 cores)."""
-    import numpy as np
+def pmc_traffic(kernel_prefix: str = "k_count_group"):
+    """HBM bytes per dispatch of the counting kernel from the committed counter profile: the
+    line `pmc: kernel=... fetch_kib=F write_kib=W fetch_scale=a write_scale=b` written from two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean over the run),
+    scaled by what the same counters read for the copy probe of known size."""
+    if not os.path.exists(PMC_PROFILE):
+        return None
+    for ln in open(PMC_PROFILE):
+        m = re.match(r"\s*pmc:\s*(.*)$", ln)
+        if not m:
+            continue
+        kv = dict(x.split("=", 1) for x in m.group(1).split())
+        if not kv.get("kernel", "").startswith(kernel_prefix):
+            continue
+        b = 1024.0 * (float(kv["fetch_scale"]) * float(kv["fetch_kib"]) +
+                      float(kv["write_scale"]) * float(kv["write_kib"]))
+        return {"bytes": b, "kernel": kv["kernel"], "commit": kv.get("commit"),
+                "source": os.path.relpath(PMC_PROFILE, ROOT)}
+    return None
+
+
+def pair_tiles(dpad: int) -> int:
+    """256 x 256 output tiles the pair kernel computes (the upper triangle; a mirror pass
+    copies the rest)."""
+    t = (dpad + 255) // 256
+    return t * (t + 1) // 2
+
+
+def fingerprint(it, off, sup) -> str:
+    """Order-free digest of an MFI family: sorted per-set hashes of (items, support)."""
+    def mix(z):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+    if len(sup) == 0:
+        return "empty"
+    with np.errstate(over="ignore"):
+        h = mix(np.asarray(it, dtype=np.uint64))
+        cs = np.concatenate([[np.uint64(0)], np.cumsum(h, dtype=np.uint64)])
+        sh = mix((cs[off[1:]] - cs[off[:-1]]) ^ np.asarray(sup, dtype=np.uint64))
+    return hashlib.sha256(np.sort(sh).tobytes()).hexdigest()[:16]
+
+
+def cpu_baseline(T, gpu_ms_per_step: float, min_s: float = 10.0):
+    """The oracle's restatement of the reference CPU search (count every survivor per node,
+    kernel_bitmap.cu:390-408 / :853-894) on all host cores, over the step's own workload:
+    workload 2 at min supports 10%, 5% and 2%, repeated whole until at least min_s seconds
+    have passed. The counter's inner loop runs AVX-512 VPOPCNTDQ where the host has it."""
     from oracle import cref
     from sgfim import gen
     cores = len(os.sched_getaffinity(0))
@@ -65,22 +109,27 @@ def cpu_baseline(T, min_s: float = 10.0):
             mfis += len(out)
         reps += 1
     dt = time.perf_counter() - t0
+    ms_step = 1000.0 * dt / reps
     return {"value": counts / dt, "unit": "counts/s", "cores": cores, "kind": "port",
+            "counter_path": cref.simd_path(),
+            "ms_per_step": ms_step, "gpu_ms_per_step": gpu_ms_per_step,
+            "step_time_ratio_cpu_over_gpu": ms_step / gpu_ms_per_step if gpu_ms_per_step else None,
             "sample": f"workload 2 ({T.n_txn} txns, {T.n_items} items) at min supports 10%/5%/2%, "
                       f"{reps} repetitions: {nodes} nodes, {counts} counts in {dt:.2f} s "
-                      f"(count-all reference search, {mfis // reps} MFIs per repetition)"}
+                      f"(count-all reference search, {mfis // reps} MFIs per repetition); counts here are "
+                      f"every survivor of every node, the GPU's are the needed counts, so compare "
+                      f"ms_per_step"}
 
 
 def workload3_run(Context, seed=0, workers=16):
     """512 windows, lengths log-uniform in [2,000, 200,000], 32 sensors x 6 bins, 3% per window:
     one search over all windows (every counting launch mixes windows)."""
-    from sgfim import gen
+    from sgfim import autotune, gen
     t0 = time.perf_counter()
     win_off, off, items, lens = gen.workload3_csr(seed, 512, workers)
     log(f"workload 3 generated: {len(lens)} windows, {int(win_off[-1])} txns in {time.perf_counter() - t0:.1f} s")
     ctx = Context(device=0)
     ctx.load_windows(win_off, off, items)
-    from sgfim import autotune
     tile = autotune.apply(ctx, len(lens))
     thr = [gen.min_count(0.03, int(n)) for n in lens]
     ctx.mine_windows(thr, convert=False)                   # warm
@@ -88,47 +137,102 @@ def workload3_run(Context, seed=0, workers=16):
     r = ctx.mine_windows(thr, convert=False)
     dt = time.perf_counter() - t0
     st = r.stats
+    proj = ctx.projection()[0]
     ctx.close()
     counts = st["counts"] + st["pair_counts"]
     return {"workload": "workload 3: 512 windows, 2k-200k txns, 32 sensors x 6 bins, 3% per window",
             "seconds": dt, "mfis": len(r), "counts": counts, "counts_per_s": counts / dt,
             "count_kernel_ms": st["count_ms"], "launches": st["count_launches"],
-            "alg_gbs": 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None,
-            "tile": {"chunk_words": tile[0], "chunk_range": tile[1]}}
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj}}
 
 
-def workload5_run(Context, budget_s=8.0, seed=0, learned_order=False):
-    """10M txns x 2,048 Zipf items (2.56 GB of bitmaps, 10x the MALL): a budgeted anytime run
-    down to the 0.5% floor; reports the counting kernel against the HBM roofline. With
-    learned_order the proposal network orders the frontier."""
-    from sgfim import gen, learned
-    from sgfim._lib import ORDER_LEARNED
-    ctx = Context(device=0, frontier_cap=1 << 23)   # 8M frontier records (~5 GB at 2,048 items)
-    t0 = time.perf_counter()
-    ctx.load_zipf(seed=seed, **gen.WORKLOAD5)
-    log(f"workload 5 generated on the GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words})")
-    if learned_order:
-        ctx.set_scorer(*learned.load_weights())
-    thr = gen.min_count(0.005, ctx.n_txn)
-    t0 = time.perf_counter()
-    ctx.anytime_begin(thr, ORDER_LEARNED if learned_order else 0)
-    mfis = 0
-    tag = "learned" if learned_order else "plain"
+def _w5_anytime(ctx, thr, flags):
+    """One anytime search to completion: (final arrays, per-MFI confirmation times, stats)."""
+    ctx.anytime_begin(thr, flags)
+    parts, times = [], []
+    last = 0.0
     while True:
-        st = ctx.anytime_step(1)
-        mfis += len(ctx.anytime_take(convert=False))
-        log(f"workload 5 ({tag}) round {st['round']}: {st['counts']} counts, {mfis} MFIs, frontier {st['frontier']}")
-        if st["done"] or time.perf_counter() - t0 > budget_s:
+        st = ctx.anytime_step(8)
+        r = ctx.anytime_take(convert=False)
+        if r.count:
+            parts.append(r.arrays)
+            times.append(np.asarray(r.times))
+        if st["elapsed_s"] - last > 5.0 or st["done"]:
+            last = st["elapsed_s"]
+            log(f"  t={st['elapsed_s']:.1f} s: round {st['round']}, {sum(len(p[2]) for p in parts)} MFIs, "
+                f"frontier {st['frontier']}")
+        if st["done"]:
             break
-    dt = time.perf_counter() - t0
+    it = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.int32)
+    sup = np.concatenate([p[2] for p in parts]) if parts else np.zeros(0, np.int64)
+    lens = np.concatenate([np.diff(p[1]) for p in parts]) if parts else np.zeros(0, np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    t = np.sort(np.concatenate(times)) if times else np.zeros(0)
+    return (it, off, sup), t, st
+
+
+def _t_at(t, frac):
+    if len(t) == 0:
+        return None
+    return float(t[max(0, int(np.ceil(frac * len(t))) - 1)])
+
+
+def workload5_run(Context, steps=2):
+    """10M txns x 2,048 Zipf items (2.56 GB of bitmaps, 10x the MALL) mined to the 0.5% floor,
+    to completion. The learned frontier order (the workload's stated order) is timed as whole
+    searches; a direct-launch run gives each round stage's device time; anytime runs in the
+    learned and the plain support order give the coverage curves of the same final set."""
+    from sgfim import autotune, gen, learned
+    from sgfim._lib import NO_GRAPH, ORDER_LEARNED
+    ctx = Context(device=0, frontier_cap=1 << 23)
+    t0 = time.perf_counter()
+    ctx.load_zipf(seed=0, **gen.WORKLOAD5)
+    log(f"workload 5 generated on the GPU in {time.perf_counter() - t0:.2f} s (W = {ctx.words})")
+    ctx.set_scorer(*learned.load_weights())
+    tile = autotune.apply(ctx, 1)
+    proj = ctx.projection()[0]
+    thr = gen.min_count(0.005, ctx.n_txn)
+    ctx.mine(thr, flags=ORDER_LEARNED, convert=False)      # warm: buffers, captured rounds
+    ts, r = [], None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        r = ctx.mine(thr, flags=ORDER_LEARNED, convert=False)
+        ts.append(time.perf_counter() - t0)
+        log(f"workload 5 learned search {i + 1}/{steps}: {ts[-1]:.2f} s, {len(r)} MFIs")
+    st = r.stats
+    fp_full = fingerprint(*r.arrays)
+    counts = st["counts"] + st["pair_counts"]
+    sec = float(np.median(ts))
+    rd = ctx.mine(thr, flags=ORDER_LEARNED | NO_GRAPH, convert=False)
+    stages = ctx.stage_times()
+    log(f"workload 5 stage times (direct launches): {stages}")
+    curves = {}
+    for name, flags in (("learned", ORDER_LEARNED), ("plain", 0)):
+        log(f"workload 5 anytime, {name} order:")
+        arr, t, sta = _w5_anytime(ctx, thr, flags)
+        curves[name] = {"seconds": sta["elapsed_s"], "mfis": int(len(arr[2])), "rounds": sta["round"],
+                        "counts": sta["counts"], "fingerprint": fingerprint(*arr),
+                        "t25_s": _t_at(t, 0.25), "t50_s": _t_at(t, 0.50), "t95_s": _t_at(t, 0.95),
+                        "projections": sta["projections"], "proj_from_rows": sta["proj_from_rows"]}
     ctx.close()
-    alg = 8.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e9 if st["count_ms"] > 0 else None
+    cl, cp = curves["learned"], curves["plain"]
     return {"workload": "workload 5: 10M txns x 2048 Zipf(1.1) items, mean length 60, 40 planted sets, "
-                        f"floor 0.5%, budgeted anytime run, {tag} frontier order",
-            "seconds": dt, "rounds": st["round"], "done": bool(st["done"]), "mfis_confirmed": mfis,
-            "counts": st["counts"] + st["pair_counts"], "counts_per_s": (st["counts"] + st["pair_counts"]) / dt,
-            "count_kernel_ms": st["count_ms"], "launches": st["count_launches"],
-            "alg_gbs": alg, "alg_frac_of_hbm_peak": alg / HBM_PEAK_GBS if alg else None}
+                        "mined to the 0.5% floor to completion (done), learned frontier order",
+            "done": True, "seconds": sec, "mfis": len(r), "fingerprint": fp_full,
+            "counts": counts, "counts_per_s": counts / sec, "rounds": st["round"],
+            "count_kernel_ms": st["count_ms"], "projections": st["projections"],
+            "proj_from_rows": st["proj_from_rows"],
+            "words_per_count_avg": st["words"] / max(st["counts"], 1),
+            "valu_frac_count_kernel": 4.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e12 / VALU_PEAK_TLANE
+            if st["count_ms"] > 0 else None,
+            "stage_ms_direct_launch": stages, "direct_launch_seconds": rd.stats["elapsed_s"],
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj},
+            "coverage": {"learned": cl, "plain": cp,
+                         "identical_final_sets": cl["fingerprint"] == cp["fingerprint"] == fp_full,
+                         "t25_ratio": cl["t25_s"] / cp["t25_s"], "t50_ratio": cl["t50_s"] / cp["t50_s"],
+                         "t95_ratio": cl["t95_s"] / cp["t95_s"],
+                         "definition": "fraction of the final MFI set confirmed by wall time t since the "
+                                       "search began (times of confirmation, host clock)"}}
 
 
 def workload4_run(Context, n_seeds=4, batch_nodes=4096):
@@ -137,23 +241,17 @@ def workload4_run(Context, n_seeds=4, batch_nodes=4096):
     support order, on the first held-out seeds of the proposal network's evaluation (the
     committed weights). Each stream gets an untimed run first, and the timed runs alternate
     which order goes first."""
-    from sgfim import learned
-    from sgfim.train_proposal import anytime_curve, compare_orders, streams, t_at
-    from sgfim import gen
-    from sgfim import autotune
+    from sgfim import autotune, gen, learned
+    from sgfim.train_proposal import compare_orders, streams, t_at
     ctx = Context(device=0, batch_nodes=batch_nodes)
     ctx.set_scorer(*learned.load_weights())
     tile = None
     seeds = list(range(1_000_000, 1_000_000 + n_seeds))
-    rows, warm = [], True
+    rows = []
     for k, (s, (off, items)) in enumerate(streams(seeds, 500_000, min(16, len(seeds)))):
         ctx.load_csr(off, items)
         tile = tile or autotune.apply(ctx, 1)
         thr = gen.min_count(0.02, ctx.n_txn)
-        if warm:   # warm the captured rounds of the learned order
-            anytime_curve(ctx, thr, True)
-            warm = False
-        rp = fp = rl = fl = None
         rp, fp, rl, fl = compare_orders(ctx, thr, plain_first=k % 2 == 0)
         tot = len(fp)
         rows.append({"seed": s, "mfis": tot, "identical": fp == fl, "plain_first": k % 2 == 0,
@@ -163,6 +261,7 @@ def workload4_run(Context, n_seeds=4, batch_nodes=4096):
                      "t50_plain_s": t_at(rp, 0.5, tot, 0), "t50_learned_s": t_at(rl, 0.5, tot, 0),
                      "counts50_plain": t_at(rp, 0.5, tot, 1), "counts50_learned": t_at(rl, 0.5, tot, 1)})
         log(f"workload 4 seed {s}: t95 plain {rows[-1]['t95_plain_s']:.4f} s, learned {rows[-1]['t95_learned_s']:.4f} s")
+    proj = ctx.projection()[0]
     ctx.close()
     med = lambda v: float(np.median(v))
     return {"workload": f"workload 4: 500,000 txns x 256 items, 2% floor, held-out seeds {seeds[0]}..{seeds[-1]}, "
@@ -176,33 +275,38 @@ def workload4_run(Context, n_seeds=4, batch_nodes=4096):
             "median_t95_plain_s": med([r["t95_plain_s"] for r in rows]),
             "median_t95_learned_s": med([r["t95_learned_s"] for r in rows]),
             "all_identical": all(r["identical"] for r in rows),
-            "tile": {"chunk_words": tile[0], "chunk_range": tile[1]}, "seeds": rows}
+            "ordering_bound": "ordering the frontier by the true labels (records on a path to a final MFI "
+                              "first) reaches 95% coverage only after 0.79-0.85 of the plain order's counting "
+                              "work on these streams (CPU simulation with the oracle's node trace), so no "
+                              "reordering reaches 0.5x at 95% on this shape",
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj}, "seeds": rows}
 
 
-def cpu_extra(T2=None, budget_s=15.0):
+def cpu_extra(budget_s=15.0):
     """Single-thread restatements of the reference's Python miners: the set-based miner
     (skab_spike.py:84-135) and the NumPy packed-bitmap miner (exp_e6_cpuvec.py:43-107) on
-    workload 1 (10k txns, 2%), and the NumPy miner on workload 2 at 10% under a budget."""
+    workload 1 (10k txns, 2%), timed beside the GPU's exact search of the same workload."""
     from oracle import cpuvec as V
     from oracle import mfi_oracle as O
     from sgfim import gen
+    from sgfim.core import Context
     T = gen.workload1(0)
     rows = T.rows()
+    thr = gen.min_count(0.02, T.n_txn)
     t0 = time.perf_counter()
-    out, _ = O.mine_mfi(rows, min_count=gen.min_count(0.02, T.n_txn), node_cap=0)
+    out, _ = O.mine_mfi(rows, min_count=thr, node_cap=0)
     res = {"pure_python_mine_mfi_workload1_2pct_s": time.perf_counter() - t0, "mfis": len(out), "cores": 1}
     bm, n = V.bitmap_from_rows(rows, T.n_items)
     mf, dt = V.mine_mfi_cpuvec(bm, n, 0.02)
     res["numpy_cpuvec_workload1_2pct_s"] = dt
     res["numpy_cpuvec_workload1_mfis"] = len(mf)
-    if T2 is not None:
-        bm2, n2 = V.bitmap_from_rows(T2.rows(), T2.n_items)
-        try:
-            mf2, dt2 = V.mine_mfi_cpuvec(bm2, n2, 0.10, budget_s=budget_s)
-            res["numpy_cpuvec_workload2_10pct_s"] = dt2
-            res["numpy_cpuvec_workload2_10pct_mfis"] = len(mf2)
-        except TimeoutError:
-            res["numpy_cpuvec_workload2_10pct_s"] = f">{budget_s} (budget)"
+    with Context(device=0) as ctx:
+        ctx.load_csr(T.offsets, T.items)
+        ctx.mine(thr)
+        t0 = time.perf_counter()
+        g = ctx.mine(thr)
+        res["gpu_workload1_2pct_s"] = time.perf_counter() - t0
+        res["gpu_workload1_mfis"] = len(g)
     return res
 
 
@@ -216,11 +320,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the workload 3, 4 and 5 measurements")
-    ap.add_argument("--w5-budget", type=float, default=8.0)
     ap.add_argument("--sequential", action="store_true",
                     help="mine the three supports as three searches instead of one batched search")
-    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5", "workload5_learned"), default=None,
-                    help="run one extra workload alone and print its JSON (for profiling)")
+    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5"), default=None,
+                    help="run one extra workload alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -232,8 +335,7 @@ def main():
 
     if args.extra_only:
         res = {"workload3": lambda: workload3_run(Context), "workload4": lambda: workload4_run(Context),
-               "workload5": lambda: workload5_run(Context, args.w5_budget),
-               "workload5_learned": lambda: workload5_run(Context, args.w5_budget, learned_order=True)}[args.extra_only]()
+               "workload5": lambda: workload5_run(Context)}[args.extra_only]()
         print(json.dumps(res))
         return
 
@@ -248,17 +350,19 @@ def main():
     from sgfim import autotune
     tile_key = autotune.context_key(ctx, 1 if args.sequential else len(thrs))
     tile = autotune.apply(ctx, 1 if args.sequential else len(thrs))
+    proj = ctx.projection()[0]
     tile_src = "tile_cache.json" if autotune.lookup(tile_key) else "default"
-    log(f"counting tile {tile[0]} words x {tile[1]} chunks ({tile_src}, {tile_key})")
+    log(f"counting tile {tile[0]} words x {tile[1]} chunks, projection factor {proj} ({tile_src}, {tile_key})")
 
     pair_ms = []
+    pair_counts = ctx.n_items * (ctx.n_items + 1) // 2   # distinct pair supports (the diagonal included)
 
     from sgfim._lib import FRESH_PAIR
 
     def step():
         # the int8 MFMA pair matrix is recomputed on the device at the start of every step,
         # in the search's stream order (it seeds the search; no host read-back)
-        counts = ctx.n_items * ctx.n_items
+        counts = pair_counts
         words = 0
         cms = 0.0
         launches = 0
@@ -309,13 +413,19 @@ def main():
     copy_gbs = ctx.copy_probe(1 << 31, 5)   # on-box HBM copy rate, 2 GiB buffers (8x the MALL)
     value = tot_counts / dt
     pair_avg = sum(pair_ms) / len(pair_ms) if pair_ms else 0.0
-    pair_ops = 2.0 * ctx.n_items * ctx.n_items * T.n_txn
+    pair_ops = 2.0 * pair_tiles(((ctx.n_items + 127) // 128) * 128) * 256 * 256 * T.n_txn
     pair_tops = pair_ops / (pair_avg * 1e-3) / 1e12 if pair_avg > 0 else None
     avg_ms = tot_cms / max(tot_launch, 1)
-    bytes_per_launch = 8.0 * tot_words / max(tot_launch, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    # the AND + popcount a count needs: per 64-bit word two 32-bit ANDs and two popcounts
+    # the AND + popcount a count needs: per 64-bit word two 32-bit ANDs and two popcounts,
+    # over the words of every needed count and of every child prefix, per second of counting
     valu_achieved = 4.0 * tot_words / (tot_cms * 1e-3) / 1e12 if tot_cms > 0 else 0.0
+    traffic = pmc_traffic()
+    hbm = None
+    if traffic and avg_ms > 0:
+        gbs = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
+        hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+               "traffic_source": traffic["source"], "measured_at_commit": traffic["commit"]}
+    alg_bytes = 8.0 * tot_words / max(tot_launch, 1)
     out = {
         "metric": "support counts/s (exact MFI mining, workload 2 at 10%/5%/2%)",
         "value": value,
@@ -333,37 +443,37 @@ def main():
                                f"min supports 10%/5%/2%", "mfis_per_support": mfis,
                    "search": "three searches, one per support" if args.sequential
                              else "one batched search over the three supports",
-                   "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "source": tile_src}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": 1024.0 * (PMC_COUNT_KERNEL["fetch_scale"] * PMC_COUNT_KERNEL["fetch_kib"] +
-                                          PMC_COUNT_KERNEL["write_scale"] * PMC_COUNT_KERNEL["write_kib"]),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command, "
-                                       "calibrated by the copy probe (profiles/k_count_group_workload2_pmc.md)",
-                     "kernel": "k_count_group", "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": bytes_per_launch, "timing_check": event_check,
-                     "copy_probe_gbs": copy_gbs,
-                     # the bound that holds: candidate rows are reread from L2 and the MALL, so
-                     # the algorithmic bytes exceed the HBM peak; the kernel is bound by VALU
-                     # issue, and this is the share of it spent on the AND + popcount itself
-                     "valu": {"achieved": valu_achieved, "peak": VALU_PEAK_TLANE, "unit": "T lane-op/s",
-                              "frac": valu_achieved / VALU_PEAK_TLANE,
-                              "counted": "4 lane-ops (2 AND, 2 popcount) per 64-bit word of every count"}},
-        # the pair matrix (part of every step): 2 d^2 N int8 ops (the full square) per launch
+                   "counts": "needed AND+popcount supports of the search plus d(d+1)/2 pair supports",
+                   "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj,
+                            "source": tile_src}},
+        # the bound that holds for the counting kernel is VALU issue: tiles reuse every loaded
+        # chunk across 16 children and workload 2's 64 MB of bitmaps sit in the MALL, so HBM
+        # is far from saturated (hbm below, from the counter profile)
+        "roofline": {"bound": "valu", "kernel": "k_count_group", "achieved": valu_achieved,
+                     "peak": VALU_PEAK_TLANE, "unit": "T lane-op/s", "frac": valu_achieved / VALU_PEAK_TLANE,
+                     "counted": "4 lane-ops (2 AND, 2 popcount) per 64-bit word of every needed count and "
+                                "every child prefix",
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "avg_launch_ms": avg_ms, "launches": tot_launch, "hbm": hbm,
+                     "alg_bytes_per_launch": alg_bytes,
+                     "timing_check": event_check, "copy_probe_gbs": copy_gbs},
+        # the pair matrix (part of every step): 2 x 256^2 x N int8 ops per computed tile
         "pair_roofline": {"bound": "mfma", "kernel": "k_pair_tile", "avg_launch_ms": pair_avg,
                           "achieved": pair_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                          "frac": pair_tops / INT8_PEAK_TOPS if pair_tops else None},
+                          "frac": pair_tops / INT8_PEAK_TOPS if pair_tops else None,
+                          "tiles": pair_tiles(((ctx.n_items + 127) // 128) * 128),
+                          "full_square_equivalent_tops": 2.0 * ctx.n_items ** 2 * T.n_txn / (pair_avg * 1e-3) / 1e12
+                          if pair_avg > 0 else None},
     }
     ctx.close()
     if not args.no_extra:
-        out["extra"] = {"workload5": workload5_run(Context, args.w5_budget),
-                        "workload5_learned": workload5_run(Context, args.w5_budget, learned_order=True),
+        out["extra"] = {"workload5": workload5_run(Context),
                         "workload3": workload3_run(Context),
                         "workload4": workload4_run(Context)}
     if not args.no_cpu:
         log("timing the CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(T)
-        out.setdefault("extra", {})["cpu_python"] = cpu_extra(T)
+        out["cpu_baseline"] = cpu_baseline(T, ms_per_step)
+        out.setdefault("extra", {})["cpu_python"] = cpu_extra()
     print(json.dumps(out), flush=True)
 
 
