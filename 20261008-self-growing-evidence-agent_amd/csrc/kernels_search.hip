@@ -485,8 +485,10 @@ __device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgS
             P.words = D.words;
             n_view = D.n;
         }
+        // A record with one child gets no block of its own: the child can project itself
+        // at no more cost when it is counted, and needs none if it is an MFI.
         const int64_t blk = (int64_t)m.ncand * sg_block_stride(m.sup);
-        if (m.ncand > 0 && (int64_t)m.sup * S.proj_factor <= n_view && blk <= S.proj_max_block && m.sup <= S.ptid_cap) {
+        if (m.ntail >= 2 && (int64_t)m.sup * S.proj_factor <= n_view && blk <= S.proj_max_block && m.sup <= S.ptid_cap) {
             P.src = P.view;
             P.src_words = P.words;
             P.pnew = 1;
