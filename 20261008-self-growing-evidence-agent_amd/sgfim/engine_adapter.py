@@ -115,13 +115,21 @@ def read_fimi(path):
 
 
 def _csr_without_zero(off, items):
-    """Id 0 is counted in masks but never a candidate in the engines; the ABI takes ids >= 1,
-    so it is dropped here (it can then neither extend nor be reported)."""
+    """The engines count id 0 in prefix masks but never as a candidate (kernel_bitmap.cu:344-366,
+    AnyTime kernel.cu:355-358: survivors are ids >= 1), so for every root other than 0 their
+    output is the MFI family of the transactions without item 0; the ABI takes ids >= 1, and
+    id 0 is dropped here. Returns (off, items, support of id 0)."""
     if items.size and (items == 0).any():
         keep = items != 0
         cs = np.concatenate([[0], np.cumsum(keep)])
-        return cs[off], items[keep]
-    return off, items
+        return cs[off], items[keep], int((~keep).sum())
+    return off, items, 0
+
+
+def _csr_shift(off, items):
+    """Ids + 1, so id 0 is an ordinary item (the exact family: the Python miner treats 0 like
+    any other id, skab_spike.py:84-135); results are shifted back."""
+    return off, items + 1
 
 
 # ---------------------------------------------------------------------------- Results files
@@ -249,7 +257,12 @@ def run_tensorfim(engine, fimi_path, rel_thr, reps=1, timeout=1800, exact=False)
     ctx = _ctx()   # no usable device or library: raised, as a missing engine executable is
     t_pre0 = time.perf_counter()
     try:
-        off, items = _csr_without_zero(*read_fimi(fimi_path))
+        off, items = read_fimi(fimi_path)
+        zero = 0
+        if exact:
+            off, items = _csr_shift(off, items)
+        else:
+            off, items, zero = _csr_without_zero(off, items)
         ctx.load_csr(off, items)
     except (OSError, L.SgfimError) as e:
         # the engine cannot read its input and writes no Results file (kernel_bitmap.cu:801)
@@ -259,6 +272,14 @@ def run_tensorfim(engine, fimi_path, rel_thr, reps=1, timeout=1800, exact=False)
     pre = time.perf_counter() - t_pre0
     n = ctx.n_txn
     thr = max(1, math.ceil(n * rel_thr)) if n else 1
+    if zero and n and zero / n > rel_thr:
+        # id 0 passes the root rule (kernel_bitmap.cu:816): the engine also mines a subtree
+        # under root 0 whose output (kernel_bitmap.cu:866-889 with id 0 never a candidate)
+        # is not reproduced here
+        if os.path.exists(rp):
+            os.remove(rp)
+        return {"ok": False, "mfis": {}, "comp_times": [], "tot_times": [],
+                "log": "item id 0 is a root of the engine's search; its subtree is not reproduced"}
     for _ in range(reps):
         if os.path.exists(rp):
             os.remove(rp)
@@ -272,8 +293,9 @@ def run_tensorfim(engine, fimi_path, rel_thr, reps=1, timeout=1800, exact=False)
         dt = time.perf_counter() - t0
         comp.append(dt)
         tot.append(dt + pre)
-        mfis = res.as_dict()
-        write_tensorfim_results(rp, fimi_path, list(zip(res.itemsets, res.supports)), n, rel_thr, dt, pre,
+        sets = [tuple(x - 1 for x in it) for it in res.itemsets] if exact else res.itemsets
+        mfis = {frozenset(x): s for x, s in zip(sets, res.supports)}
+        write_tensorfim_results(rp, fimi_path, list(zip(sets, res.supports)), n, rel_thr, dt, pre,
                                 {"transNum": n, "dimension": ctx.n_items})
     out = {"ok": ok, "mfis": mfis, "comp_times": comp, "tot_times": tot}
     if log:
@@ -312,11 +334,24 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
     ctx = _ctx()   # no usable device or library: raised, as a missing engine executable is
     t_pre0 = time.perf_counter()
     try:
-        off, items = _csr_without_zero(*read_fimi(fimi_path))
+        off, items = read_fimi(fimi_path)
+        zero = 0
+        if exact:
+            off, items = _csr_shift(off, items)
+        else:
+            off, items, zero = _csr_without_zero(off, items)
         ctx.load_csr(off, items)
     except (OSError, L.SgfimError) as e:
         return _anyfim_failed(rp, str(e))
     pre = time.perf_counter() - t_pre0
+    if zero:
+        # id 0 ranks in the engine's stable frequency order (kernel.cu:269-300): with it among
+        # the first upto_stage items its stage thresholds and root sets differ from the ones
+        # computed without it, and the engine's root-0 output is not reproduced
+        s0 = ctx.item_supports()
+        rank0 = int((s0 > zero).sum())          # items ranked before id 0 (ties: lower id first)
+        if rank0 < upto_stage:
+            return _anyfim_failed(rp, "item id 0 is among the engine's first stage roots; not reproduced")
     if upto_stage > ctx.n_items:
         # the engine refuses (kernel.cu:804-806) and writes no file
         out = _anyfim_failed(rp, "Up to the stage number being greater than the dimension !")
@@ -339,7 +374,7 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
             stages[k] = []
         done_at = [0.0] * upto_stage
         for x, sp, w, t in zip(res.itemsets, res.supports, res.windows, res.times):
-            stages[w + 1].append((x, sp))
+            stages[w + 1].append((tuple(i - 1 for i in x) if exact else x, sp))
             done_at[w] = max(done_at[w], t)
         runtimes = done_at
         write_anyfim_results(rp, fimi_path, stages, thrs, runtimes, ctx.n_txn, pre)
@@ -355,7 +390,8 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
         except L.SgfimError as e:
             return _anyfim_failed(rp, str(e))
         runtimes.append(time.perf_counter() - t0)
-        stages[k] = list(zip(res.itemsets, res.supports))
+        sets = [tuple(i - 1 for i in x) for x in res.itemsets] if exact else res.itemsets
+        stages[k] = list(zip(sets, res.supports))
     write_anyfim_results(rp, fimi_path, stages, thrs, runtimes, ctx.n_txn, pre)
     st, meta = parse_anyfim_results(rp)
     return {"ok": True, "stages": st, "meta": meta, "results_path": rp}

@@ -219,3 +219,31 @@ def subprocess_run_shim(name, *args):
     bindir = os.path.join(os.path.dirname(EA.__file__), "bin")
     return subprocess.run([sys.executable, os.path.join(bindir, name), *args], capture_output=True, text=True,
                           timeout=300)
+
+
+def test_item_id_zero(tmp_path):
+    """Id 0: an ordinary item of the exact family (the Python miner's semantics), and in
+    engine mode refused (ok=False) only when it would be a root of the engine's search --
+    otherwise the engines' output without item 0 is reproduced (id 0 is never a candidate)."""
+    print("case: item id 0", flush=True)
+    rng = np.random.default_rng(8)
+    # items 1..11 at random, and id 0 in most transactions (the most frequent item)
+    rows = [tuple(sorted(set(int(x) for x in rng.choice(np.arange(1, 12), size=4, replace=False)) |
+                         ({0} if rng.random() < 0.8 else set()))) for _ in range(300)]
+    p = tmp_path / "zero.txt"
+    p.write_text("\n".join(" ".join(map(str, r)) for r in rows) + "\n")
+    n = len(rows)
+    ex = EA.run_tensorfim("bmma", str(p), 0.1, exact=True)
+    assert ex["ok"]
+    assert ex["mfis"] == {frozenset(q): c for q, c in O.mine_mfi(rows, min_count=int(np.ceil(0.1 * n)), node_cap=0)[0]}
+    assert any(0 in k for k in ex["mfis"])
+    eng = EA.run_tensorfim("bmma", str(p), 0.1)           # id 0 is frequent: a root of the engine
+    assert eng["ok"] is False and "id 0" in eng["log"]
+    rare = [r if 0 not in r or i % 20 == 0 else tuple(x for x in r if x) for i, r in enumerate(rows)]
+    q = tmp_path / "rare0.txt"
+    q.write_text("\n".join(" ".join(map(str, r)) for r in rare) + "\n")
+    eng = EA.run_tensorfim("bmma", str(q), 0.1)           # id 0 infrequent: reproduced
+    without = [tuple(x for x in r if x) for r in rare]
+    assert eng["ok"] and eng["mfis"] == O.tensorfim_family(without, 0.1)
+    af = EA.run_anyfim(str(p), 3)
+    assert af["ok"] is False and "id 0" in af["log"]
