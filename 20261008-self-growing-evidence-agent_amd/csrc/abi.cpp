@@ -118,6 +118,8 @@ struct sgfim_ctx {
     void* scratch = nullptr;            // this device's static count-batch scratch (sg_count_scratch)
     int32_t proj_factor = 4;            // projected blocks: factor (0: off)
     int64_t proj_arena = 0;             // projected-block arena words (0: default)
+    int32_t proj_dense = 32;            // blocks at least 1/proj_dense dense are compacted word by word (0: never)
+    int32_t proj_mintail = 16;          // children a record needs for such a block
     sgfim_opts opts{};
     // loaded data
     bool loaded = false;
@@ -676,8 +678,8 @@ int launch_round(sgfim_ctx* c, SearchState* S)
 
 // (Re)use the context's search buffers for a run needing the given sizes.
 struct ProjSizes {
-    int32_t factor = 0;
-    int64_t arena = 0, pdesc = 0, ptid = 0;
+    int32_t factor = 0, dense = 0, mintail = 0;
+    int64_t arena = 0, pdesc = 0, ptid = 0, pent = 0;
 };
 
 int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap, int32_t d, int32_t dw,
@@ -693,8 +695,9 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         const bool fits = D.sib_cap == sib_cap && D.cap >= cap && D.newborn_cap >= newborn_cap && D.ring_cap >= ring_cap && D.d_max >= d &&
                           D.dw_max >= dw && D.bmax == bmax && D.chunk_words == chunk && D.chunk_range == range && D.trace_cap == trace_cap &&
                           B->roots_cap >= n_roots && D.cell_cap >= cell_cap && D.ring_cap >= n_roots + 2 * D.newborn_cap &&
-                          D.proj_factor == ps.factor && D.parena_cap == ps.arena && D.pdesc_cap == ps.pdesc &&
-                          D.ptid_cap == ps.ptid;
+                          D.proj_factor == ps.factor && D.proj_dense == ps.dense && D.proj_mintail == ps.mintail &&
+                          D.parena_cap == ps.arena && D.pdesc_cap == ps.pdesc &&
+                          D.ptid_cap == ps.ptid && D.pent_cap == ps.pent;
         if (fits) return SGFIM_OK;
         bufs_free(c);
     }
@@ -714,11 +717,14 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     D.cell_cap = cell_cap;
     D.sib_cap = sib_cap;
     D.proj_factor = ps.factor;
+    D.proj_dense = ps.dense;
+    D.proj_mintail = ps.mintail;
     D.parena_cap = ps.arena;
     D.proj_round_words = ps.arena / 4;
     D.proj_max_block = D.proj_round_words;
     D.pdesc_cap = ps.pdesc;
     D.ptid_cap = ps.ptid;
+    D.pent_cap = ps.pent;
     B->roots_cap = std::max<int64_t>(n_roots, 1);
     auto& V = B->bufs;
     int rc;
@@ -745,7 +751,8 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         (rc = dev_alloc_t(c, V, &D.bunitB, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.bunitT, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &D.bunitG, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.bpa, (size_t)bmax + 1)) ||
         (rc = dev_alloc_t(c, V, &D.bpd, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.ptoff, (size_t)bmax + 1)) ||
-        (rc = dev_alloc_t(c, V, &D.pfill, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.crow, (size_t)bmax * d))) {
+        (rc = dev_alloc_t(c, V, &D.pfill, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.crow, (size_t)bmax * d)) ||
+        (rc = dev_alloc_t(c, V, &D.bunitC, (size_t)bmax + 1)) || (rc = dev_alloc_t(c, V, &D.peoff, (size_t)bmax + 1))) {
         bufs_free(c);
         return rc;
     }
@@ -753,7 +760,8 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
         if ((rc = dev_alloc_t(c, V, &D.parena, (size_t)(ps.arena + D.proj_max_block))) ||
             (rc = dev_alloc_t(c, V, &D.pdesc, (size_t)ps.pdesc)) ||
             (rc = dev_alloc_t(c, V, &D.pmask, (size_t)ps.pdesc * dw)) ||
-            (rc = dev_alloc_t(c, V, &D.prank, (size_t)ps.pdesc * dw)) || (rc = dev_alloc_t(c, V, &D.ptid, (size_t)ps.ptid))) {
+            (rc = dev_alloc_t(c, V, &D.prank, (size_t)ps.pdesc * dw)) || (rc = dev_alloc_t(c, V, &D.ptid, (size_t)ps.ptid)) ||
+            (rc = dev_alloc_t(c, V, &D.pent, (size_t)ps.pent))) {
             bufs_free(c);
             return rc;
         }
@@ -1229,6 +1237,25 @@ int sgfim_get_projection(const sgfim_ctx* c, int32_t* factor, int64_t* arena_wor
     if (!c || !factor || !arena_words) return SGFIM_E_INVAL;
     *factor = c->proj_factor;
     *arena_words = c->proj_arena;
+    return SGFIM_OK;
+}
+
+int sgfim_set_block_rule(sgfim_ctx* c, int32_t dense, int32_t min_children)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (dense < 0 || dense == 1 || dense > 1024 || min_children < 2 || min_children > SG_DMAX)
+        return fail(c, SGFIM_E_INVAL, "block rule: dense must be 0 or 2..1024, min_children 2..%d", SG_DMAX);
+    if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "block rule: an anytime search is in progress");
+    c->proj_dense = dense;
+    c->proj_mintail = min_children;
+    return SGFIM_OK;
+}
+
+int sgfim_get_block_rule(const sgfim_ctx* c, int32_t* dense, int32_t* min_children)
+{
+    if (!c || !dense || !min_children) return SGFIM_E_INVAL;
+    *dense = c->proj_dense;
+    *min_children = c->proj_mintail;
     return SGFIM_OK;
 }
 
@@ -1781,6 +1808,8 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     ProjSizes ps;
     if (c->proj_factor > 0) {
         ps.factor = c->proj_factor;
+        ps.dense = c->proj_dense;
+        ps.mintail = c->proj_mintail;
         int64_t a = c->proj_arena;
         if (a <= 0) {
             a = std::max<int64_t>((int64_t)1 << 28, 32 * c->rows_words);
@@ -1795,6 +1824,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
         int64_t nmax = 0;
         for (int32_t v = 0; v < nv; v++) nmax = std::max<int64_t>(nmax, vs[v].n);
         ps.ptid = std::max<int64_t>((int64_t)1 << 27, 2 * nmax / ps.factor + 1);
+        ps.pent = ps.ptid / 4;   // 16-byte word entries of compacted blocks
     }
     if ((rc = ensure_bufs(c, cap, newborn_cap, ring_cap, d, dw, bmax, search_chunk(c), search_range(c),
                           c->trace ? c->trace_cap : 0, n_roots, cell_cap, ps)))
@@ -1894,7 +1924,7 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
                     float ms = 0.0f;
                     if (hipEventElapsedTime(&ms, B->ev[i], B->ev[i + 1]) == hipSuccess) S->stage_ms[i] += ms;
                     else (void)hipGetLastError();
-                    if (i == 3) S->event_ms += ms;
+                    if (i == 5) S->event_ms += ms;
                 }
             }
         }

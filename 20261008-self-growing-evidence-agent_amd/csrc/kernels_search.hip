@@ -463,7 +463,10 @@ struct SgPlan {
     int64_t words;     // words of the view's rows
     int64_t src_words; // words of the source view (new blocks)
     int64_t blk;       // arena words of a new block
-    int32_t pnew;      // a new block
+    int64_t list;      // a new block's ptid entries: its transactions (gathered) or one per compress unit
+    int64_t ents;      // a compacted new block's word entries (at most one per source word)
+    int64_t units;     // a new block's gather or compress units
+    int32_t pnew;      // a new block: 1 gathered bit by bit, 2 compacted word by word
     int32_t nr;        // row-chunk ranges of the view
 };
 
@@ -476,6 +479,9 @@ __device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgS
     P.words = sg.words;
     P.src_words = sg.words;
     P.blk = 0;
+    P.list = 0;
+    P.ents = 0;
+    P.units = 0;
     P.pnew = 0;
     int64_t n_view = sg.n;
     if (S.proj_factor > 0) {
@@ -486,15 +492,30 @@ __device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgS
             n_view = D.n;
         }
         // A record with one child gets no block of its own: the child can project itself
-        // at no more cost when it is counted, and needs none if it is an MFI.
+        // at no more cost when it is counted, and needs none if it is an MFI. A dense block
+        // (built word by word: ~50 lane operations per source word and row, against 4 for
+        // one count over that word) needs proj_mintail children to repay its build.
         const int64_t blk = (int64_t)m.ncand * sg_block_stride(m.sup);
-        if (m.ntail >= 2 && (int64_t)m.sup * S.proj_factor <= n_view && blk <= S.proj_max_block && m.sup <= S.ptid_cap) {
+        const bool dense = S.proj_dense > 0 && (int64_t)m.sup * S.proj_dense >= n_view;
+        if (m.ntail >= (dense ? max(S.proj_mintail, 2) : 2) && (int64_t)m.sup * S.proj_factor <= n_view &&
+            blk <= S.proj_max_block && m.sup <= S.ptid_cap) {
             P.src = P.view;
             P.src_words = P.words;
-            P.pnew = 1;
             P.blk = blk;
             P.words = sg_block_words(m.sup);
             P.view = -2;   // its id is assigned from the scan
+            const int64_t ents = min((int64_t)m.sup, P.src_words);
+            const int64_t cu = (P.words + SG_PC_WORDS - 1) / SG_PC_WORDS;
+            if (dense && ents <= S.pent_cap && P.src_words < ((int64_t)1 << 24)) {
+                P.pnew = 2;
+                P.list = cu;
+                P.ents = ents;
+                P.units = cu * ((m.ncand + SG_PC_ROWS - 1) / SG_PC_ROWS);
+            } else {
+                P.pnew = 1;
+                P.list = m.sup;
+                P.units = ((int64_t)m.sup + SG_PG_TIDS - 1) / SG_PG_TIDS * ((m.ncand + SG_PG_ROWS - 1) / SG_PG_ROWS);
+            }
         }
     }
     P.nr = sg_ranges(P.words, S.chunk_words, S.chunk_range);
@@ -511,7 +532,7 @@ __device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgS
 // between the passes (each thread rereads only its own entries).
 __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgSegment* __restrict__ segs)
 {
-    constexpr int NQ = 11;
+    constexpr int NQ = 13;
     __shared__ int32_t tops[SG_NBUCKETS], ends[SG_NBUCKETS];
     __shared__ int64_t wsum[NQ][SEL_THREADS / 64];
     __shared__ int64_t tot[NQ];
@@ -572,7 +593,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
         }
     }
     // sums: U items, children, cells, window-row blocks, sibling words, block units,
-    // transactions, arena words, new blocks, transaction-list units, gather units
+    // list entries, arena words, new blocks, transaction-list units, gather units, word
+    // entries, compress units
     int64_t sm[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; n++) sm[n] = 0;
@@ -592,11 +614,12 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
             sm[4] += (int64_t)(1 + m.ntail) * S.dw_max;
             sm[5] += onb ? (int64_t)m.nblk * P.nr : 0;
             if (P.pnew) {
-                sm[6] += m.sup;
+                sm[6] += P.list;
                 sm[7] += P.blk;
                 sm[8] += 1;
                 sm[9] += (P.src_words + SG_PT_WORDS - 1) / SG_PT_WORDS;
-                sm[10] += ((int64_t)m.sup + SG_PG_TIDS - 1) / SG_PG_TIDS * ((m.ncand + SG_PG_ROWS - 1) / SG_PG_ROWS);
+                sm[P.pnew == 2 ? 12 : 10] += P.units;
+                sm[11] += P.ents;
             }
             S.batch[i] = nid[q];
         }
@@ -618,7 +641,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
             const int32_t nu = m.ncand, nt = m.ntail;
             const int64_t cells = (int64_t)nu * nt;
             if (ex[1] + nt > budget || ex[2] + cells > S.cell_cap ||
-                (P.pnew && (ex[6] + m.sup > S.ptid_cap || ex[7] + P.blk > S.proj_round_words)))
+                (P.pnew && (ex[6] + P.list > S.ptid_cap || ex[7] + P.blk > S.proj_round_words ||
+                            ex[11] + P.ents > S.pent_cap)))
                 atomicMin(&s_cut, i);
             S.coff[i] = (int32_t)ex[0];
             S.choff[i] = (int32_t)ex[1];
@@ -630,6 +654,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
             S.bunitB[i] = ex[5];
             S.bunitT[i] = ex[9];
             S.bunitG[i] = ex[10];
+            S.bunitC[i] = ex[12];
             // ring positions before this record (the allocation at the cut); a new block
             // gets descriptor pn0 + ex[8] and arena words an0 + ex[7] onwards, and its
             // descriptor is written by k_candlist for the records before the cut only
@@ -638,6 +663,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
             if (P.pnew) {
                 P.view = pn0 + ex[8];
                 S.ptoff[i] = ex[6];
+                S.peoff[i] = ex[11];
             }
             S.bview[i] = P.view;
             S.bsrc[i] = P.src;
@@ -650,11 +676,12 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
             ex[4] += sz;
             ex[5] += P.view == -1 ? 0 : (int64_t)m.nblk * P.nr;
             if (P.pnew) {
-                ex[6] += m.sup;
+                ex[6] += P.list;
                 ex[7] += P.blk;
                 ex[8] += 1;
                 ex[9] += (P.src_words + SG_PT_WORDS - 1) / SG_PT_WORDS;
-                ex[10] += ((int64_t)m.sup + SG_PG_TIDS - 1) / SG_PG_TIDS * ((nu + SG_PG_ROWS - 1) / SG_PG_ROWS);
+                ex[P.pnew == 2 ? 12 : 10] += P.units;
+                ex[11] += P.ents;
             }
         }
     }
@@ -679,6 +706,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
             S.bunitB[cut] = tot[5];
             S.bunitT[cut] = tot[9];
             S.bunitG[cut] = tot[10];
+            S.bunitC[cut] = tot[12];
         }
         if (cut == 0 && total > 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 1ull);
         const int64_t nch = S.choff[cut];
@@ -690,6 +718,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
         S.ctl[CTL_UNITS] = S.unit_off[cut] * (int64_t)s_maxr + S.bunitB[cut];
         S.ctl[CTL_PROJ_TUNITS] = S.bunitT[cut];
         S.ctl[CTL_PROJ_GUNITS] = S.bunitG[cut];
+        S.ctl[CTL_PROJ_CUNITS] = S.bunitC[cut];
         S.ctl[CTL_NODES] += nch;
         S.ctl[CTL_SIB_NEXT] = S.ctl[CTL_SIB_NEXT] + tot[4];   // blocks of records past the cut stay unused
         // the rings advance by the allocations of the records before the cut
@@ -811,11 +840,15 @@ __device__ __forceinline__ int sg_find_record(const int64_t* __restrict__ off, i
 
 // ------------------------------------------------------------------ projection
 // Transaction lists of the round's new blocks: T(X) of each such record over its source
-// view, as positions in that view. Units are (record, SG_PT_WORDS source words); each wave
-// takes 1024 of them as 16 steps of 64 coalesced words, ANDs X's rows the source has not
-// applied, and appends the set bits' positions (ascending within the wave) to the
-// record's list at a position reserved with one atomic. Every row of the block is then
-// gathered in this list's order, so the order across waves is free.
+// view. Units are (record, SG_PT_WORDS source words); each wave takes 256 of them as 4
+// steps of 64 coalesced words and ANDs X's rows the source has not applied. A gathered
+// block's list holds the set bits' positions (ascending within the wave), appended at a
+// position reserved with one atomic. A compacted block's list holds one entry per nonzero
+// word (SgPWord: the word, its mask and the block bit of its first transaction); one
+// atomic on the packed counter (entries << 40 | transactions) reserves the wave's entries
+// and bits together, so entries are in block-bit order. The entry holding the first bit of
+// each compress unit is recorded in ptid. Every row of the block is then built in this
+// list's order, so the order across waves is free.
 __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t* __restrict__ rows,
                                                   const SgSegment* __restrict__ segs)
 {
@@ -884,6 +917,46 @@ __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t
             }
             ex[i] = tot + y - cnt;
             tot += __shfl(y, 63, 64);
+        }
+        if (S.bpnew[g] == 2) {
+            int32_t ne[SG_PT_WORDS / 256], el[SG_PT_WORDS / 256];
+            int32_t etot = 0;
+#pragma unroll
+            for (int i = 0; i < SG_PT_WORDS / 256; i++) {
+                const uint64_t nz = __ballot(mv[i] != 0ull);
+                ne[i] = __popcll(nz);
+                el[i] = __popcll(nz & ((1ull << lane) - 1ull));
+                etot += ne[i];
+            }
+            uint64_t base = 0ull;
+            if (lane == 0 && tot)
+                base = atomicAdd((unsigned long long*)&S.pfill[g], ((unsigned long long)etot << 40) | (unsigned long long)tot);
+            base = __shfl(base, 0, 64);
+            const int64_t e0 = (int64_t)(base >> 40), b0 = (int64_t)(base & ((1ull << 40) - 1ull));
+            const int64_t ecap = min((int64_t)m.sup, vwords);
+            SgPWord* ent = S.pent + S.peoff[g];
+            int32_t* first = S.ptid + S.ptoff[g];
+            int64_t eacc = 0;
+#pragma unroll
+            for (int i = 0; i < SG_PT_WORDS / 256; i++) {
+                if (mv[i]) {
+                    const int64_t e = e0 + eacc + el[i];
+                    const int64_t p = b0 + ex[i];
+                    const int64_t c = __popcll(mv[i]);
+                    if (e < ecap && p + c <= m.sup) {
+                        SgPWord E;
+                        E.w = (int32_t)(wb + i * 64 + lane);
+                        E.p = (int32_t)p;
+                        E.m = mv[i];
+                        ent[e] = E;
+                        // the compress unit whose first bit is among this entry's bits
+                        const int64_t u = (p + c - 1) / (64 * SG_PC_WORDS);
+                        if (u * (64 * SG_PC_WORDS) >= p) first[u] = (int32_t)e;
+                    }
+                }
+                eacc += ne[i];
+            }
+            continue;
         }
         int64_t base = 0;
         if (lane == 0 && tot) base = (int64_t)atomicAdd((unsigned long long*)&S.pfill[g], (unsigned long long)tot);
@@ -984,6 +1057,166 @@ __global__ __launch_bounds__(256) void k_proj_gather(SgSearchDev S, const uint64
                 if (r + q < r1 && lane < NJ && wo < N.words) dst[(int64_t)(r + q) * N.stride + wo] = word;
             }
         }
+    }
+}
+
+// Bit compress of x under mask m (the bits of x where m is set, packed to the low end), in
+// the form with the six move masks of m precomputed (Hacker's Delight, 2nd ed., sec. 7-4,
+// "compress" with the masks hoisted out of the loop): masks cost ~70 operations once per
+// mask, each compress 24 per word.
+__device__ __forceinline__ void sg_compress_masks(uint64_t m, uint64_t (&mv)[6])
+{
+    uint64_t mk = ~m << 1;   // zeros of m, counted from the right
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        uint64_t mp = mk ^ (mk << 1);   // parallel suffix of mk
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        mp ^= mp << 32;
+        const uint64_t v = mp & m;   // bits of m that move by 2^i
+        mv[i] = v;
+        m = (m ^ v) | (v >> (1 << i));
+        mk &= ~mp;
+    }
+}
+
+__device__ __forceinline__ uint64_t sg_compress(uint64_t x, uint64_t m, const uint64_t (&mv)[6])
+{
+    x &= m;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const uint64_t t = x & mv[i];
+        x = (x ^ t) | (t >> (1 << i));
+    }
+    return x;
+}
+
+// Rows of the round's compacted blocks. One wave per unit (record, SG_PC_WORDS block words
+// = bits [start, end), SG_PC_ROWS rows): starting at the word entry that holds bit start,
+// each lane takes one entry (a nonzero source word w with mask m and first bit p), and for
+// every row loads the source row's word w, compresses it under m and ORs the packed bits
+// into the unit's words in LDS at bit p - start (an entry that straddles the unit's start
+// or end keeps only its bits inside). Entries are in block-bit order, so the unit's entries
+// are consecutive; the wave takes 64 at a time until they pass end. The unit's words are
+// then stored whole, row by row.
+__global__ __launch_bounds__(64) void k_proj_compress(SgSearchDev S, const uint64_t* __restrict__ rows,
+                                                     const SgSegment* __restrict__ segs)
+{
+    __shared__ unsigned long long s_out[SG_PC_ROWS][SG_PC_WORDS];
+    constexpr int64_t BITS = 64 * SG_PC_WORDS;
+    const int64_t nunits = S.ctl[CTL_PROJ_CUNITS];
+    const int nb = (int)S.ctl[CTL_BATCH_N];
+    const int lane = threadIdx.x;
+    for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {   // workgroup (= wave) uniform
+        const int g = sg_find_record(S.bunitC, nb, u);
+        const int64_t local = u - S.bunitC[g];
+        const int32_t slot = S.batch[g];
+        const SgNodeMeta m = S.meta[slot];
+        const int nrg = (m.ncand + SG_PC_ROWS - 1) / SG_PC_ROWS;
+        const int32_t ub = (int32_t)local / nrg;   // a record's units fit 32 bits (its block words / 16)
+        const int rg = (int)local - ub * nrg;
+        const int64_t src = S.bsrc[g];
+        const uint64_t* vrows;
+        int64_t vstride, vwords;
+        if (src >= 0) {
+            const SgProj D = S.pdesc[src % S.pdesc_cap];
+            vrows = S.parena + D.off % S.parena_cap;
+            vstride = D.stride;
+            vwords = D.words;
+        } else {
+            const SgSegment sg = segs[m.seg];
+            vrows = rows + sg.row_base;
+            vstride = sg.stride;
+            vwords = sg.words;
+        }
+        const uint64_t fill = (uint64_t)S.pfill[g];
+        const int64_t nbits = (int64_t)(fill & ((1ull << 40) - 1ull)), nent = (int64_t)(fill >> 40);
+        if (nbits != (int64_t)m.sup || nent > min((int64_t)m.sup, vwords)) {
+            if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 4ull);
+            continue;
+        }
+        const SgProj N = S.pdesc[S.bview[g] % S.pdesc_cap];
+        uint64_t* dst = S.parena + N.off % S.parena_cap;
+        const SgPWord* ent = S.pent + S.peoff[g];
+        const int64_t efirst = S.ptid[S.ptoff[g] + ub];
+        if (efirst < 0 || efirst >= nent) {   // every unit's first bit has an entry
+            if (lane == 0) atomicOr((unsigned long long*)&S.ctl[CTL_ERR], 4ull);
+            continue;
+        }
+        const int64_t start = (int64_t)ub * BITS, end = min(start + BITS, (int64_t)m.sup);
+        const int r0 = rg * SG_PC_ROWS;
+        const int nr = min(m.ncand - r0, SG_PC_ROWS);
+        // lane r < nr holds the source row offset of block row r0 + r
+        int64_t roff = 0;
+        if (lane < nr) {
+            const int32_t f = S.clist[S.coff[g] + r0 + lane];
+            roff = (int64_t)(src >= 0 ? sg_block_row(S, src, f) : f) * vstride;
+        }
+        for (int i = lane; i < SG_PC_ROWS * SG_PC_WORDS; i += 64) (&s_out[0][0])[i] = 0ull;
+        __syncthreads();
+        for (int64_t e0 = efirst; e0 < nent; e0 += 64) {   // wave-uniform
+            const int64_t e = e0 + lane;
+            SgPWord E;
+            E.w = 0;
+            E.p = 0;
+            E.m = 0ull;
+            if (e < nent) E = ent[e];
+            const bool act = e < nent && E.p < end && E.m != 0ull;
+            if (!__ballot(act)) break;
+            // inactive lanes: no bits (mask 0), word 0 of the source row, LDS word 0
+            const uint64_t mk = act ? E.m : 0ull;
+            const int c = __popcll(mk);
+            const int64_t off = act ? (int64_t)E.p - start : 0;
+            const int hs = off < 0 ? (int)(-off) : 0;          // bits before the unit's start
+            const int64_t pos = off < 0 ? 0 : off;
+            const int64_t keep = act ? min((int64_t)(c - hs), end - start - pos) : 0;   // >= 1 on active lanes
+            const uint64_t km = keep >= 64 ? ~0ull : ((1ull << keep) - 1ull);
+            const int q = (int)(pos >> 6), s = (int)(pos & 63);
+            const bool spill = s != 0 && keep + s > 64;
+            const int64_t w = act ? E.w : 0;
+            uint64_t mvv[6];
+            sg_compress_masks(mk, mvv);
+            // source word w of block row rr (rows past the last reread the last row)
+            auto load = [&](int rr) -> uint64_t {
+                rr = min(rr, nr - 1);
+                const int64_t ro = ((int64_t)__builtin_amdgcn_readlane((int)(roff >> 32), rr) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)roff, rr);
+                return vrows[ro + w];
+            };
+            auto put = [&](int rr, uint64_t x) {
+                const uint64_t y = (sg_compress(x, mk, mvv) >> hs) & km;
+                atomicOr(&s_out[rr][q], (unsigned long long)(y << s));
+                if (spill) atomicOr(&s_out[rr][q + 1], (unsigned long long)(y >> (64 - s)));
+            };
+            // two batches of RB rows in registers: one batch's loads are in flight while
+            // the other's words are compressed
+            constexpr int RB = 4;
+            uint64_t xa[RB], xb[RB];
+#pragma unroll
+            for (int j = 0; j < RB; j++) xa[j] = load(j);
+            for (int r = 0; r < nr; r += 2 * RB) {   // wave-uniform
+#pragma unroll
+                for (int j = 0; j < RB; j++) xb[j] = load(r + RB + j);
+#pragma unroll
+                for (int j = 0; j < RB; j++)
+                    if (r + j < nr) put(r + j, xa[j]);
+                if (r + RB >= nr) break;
+#pragma unroll
+                for (int j = 0; j < RB; j++) xa[j] = load(r + 2 * RB + j);
+#pragma unroll
+                for (int j = 0; j < RB; j++)
+                    if (r + RB + j < nr) put(r + RB + j, xb[j]);
+            }
+        }
+        __syncthreads();
+        const int64_t nq = min((int64_t)SG_PC_WORDS, N.words - (int64_t)ub * SG_PC_WORDS);
+        for (int i = lane; i < nr * SG_PC_WORDS; i += 64) {
+            const int r = i / SG_PC_WORDS, qq = i - r * SG_PC_WORDS;
+            if (qq < nq) dst[(int64_t)(r0 + r) * N.stride + (int64_t)ub * SG_PC_WORDS + qq] = s_out[r][qq];
+        }
+        __syncthreads();   // the next unit clears the LDS
     }
 }
 
@@ -1590,22 +1823,27 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     if ((e = mark(1)) != hipSuccess) return e;
     if (S.proj_factor > 0) {
         hipLaunchKernelGGL(k_proj_tids, dim3(2048), dim3(256), 0, s, S, rows, segs);
+        if ((e = mark(2)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_proj_gather, dim3(2048), dim3(256), 0, s, S, rows, segs);
+        if ((e = mark(3)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_proj_compress, dim3(8192), dim3(64), 0, s, S, rows, segs);
+    } else if ((e = mark(2)) != hipSuccess || (e = mark(3)) != hipSuccess) {
+        return e;
     }
-    if ((e = mark(2)) != hipSuccess) return e;
+    if ((e = mark(4)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_need_rows, dim3(1024), dim3(256), 0, s, S, segs, ok);
-    if ((e = mark(3)) != hipSuccess) return e;
+    if ((e = mark(5)) != hipSuccess) return e;
     switch (S.chunk_words) {
     case 128: hipLaunchKernelGGL(k_count_group<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
     case 512: hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
     default: hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
     }
-    if ((e = mark(4)) != hipSuccess) return e;
+    if ((e = mark(6)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,
                        seg_item_base);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = mark(5)) != hipSuccess) return e;
+    if ((e = mark(7)) != hipSuccess) return e;
     if ((e = sg_commit(S, segs, order_mode, scorer, 1, s)) != hipSuccess) return e;
-    if ((e = mark(6)) != hipSuccess) return e;
+    if ((e = mark(8)) != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -72,6 +72,14 @@ struct SgProj {
     int32_t nrows;      // rows: |U(Y)|
 };
 
+// One nonzero source word of a new block's transaction set: T(Y) restricted to source word
+// w is the bit mask m, and its transactions are the block's bits p .. p + popcount(m) - 1.
+struct SgPWord {
+    int32_t w;
+    int32_t p;
+    uint64_t m;
+};
+
 // Launchers (kernels_*.hip); all launch on the given stream and return hipError_t.
 hipError_t sg_launch_transpose(const int64_t* off, const int32_t* items, const int32_t* idmap,
                                int64_t n_txn, int32_t id_max, uint64_t* rows, int32_t d,
@@ -183,6 +191,9 @@ struct SgSearchDev {
     int32_t feat_on;     // compute the 24 features of newborn nodes (learned order or tracing)
     // projected blocks (SgProj); proj_factor 0 turns them off
     int32_t proj_factor;  // project a record when its support is at most 1/proj_factor of its view's transactions
+    int32_t proj_dense;   // a new block whose support is at least 1/proj_dense of its source's is compacted
+                          // word by word (k_proj_compress), a sparser one gathered bit by bit
+    int32_t proj_mintail; // children a record needs for a compacted block (2 for a gathered one)
     int32_t pad_;
     uint64_t* parena;     // [parena_cap + proj_max_block] words
     int64_t parena_cap;
@@ -193,8 +204,11 @@ struct SgSearchDev {
     uint64_t* pmask;      // [pdesc_cap][dw_max] U(Y) of each block
     int32_t* prank;       // [pdesc_cap][dw_max] items of U(Y) in the words before each word
     int64_t pdesc_cap;
-    int32_t* ptid;        // [ptid_cap] transaction lists T(Y) of the round's new blocks
+    int32_t* ptid;        // [ptid_cap] transaction lists T(Y) of the round's gathered blocks, and
+                          // the first word entry of each compress unit of the compacted ones
     int64_t ptid_cap;
+    SgPWord* pent;        // [pent_cap] nonzero source words of the round's compacted blocks
+    int64_t pent_cap;
     // per batch record (bmax + 1 entries)
     int64_t* bview;       // block the record counts on this round (-1: the window's rows)
     int64_t* bsrc;        // a new block's source: the block it is projected from (-1: the window)
@@ -203,10 +217,13 @@ struct SgSearchDev {
     int64_t* bunitB;      // counting units of records on blocks (prefix; those records run record-major)
     int64_t* bunitT;      // transaction-list units of new blocks (prefix)
     int64_t* bunitG;      // gather units of new blocks (prefix)
+    int64_t* bunitC;      // compress units of new blocks (prefix)
+    int64_t* peoff;       // a compacted block's word entries in pent
     int64_t* bpa;         // arena position of the record's new block (absolute; the prefix at the cut)
     int64_t* bpd;         // descriptor id of the record's new block (absolute; the prefix at the cut)
     int64_t* ptoff;       // a new block's transaction list in ptid
-    int64_t* pfill;       // transactions written into that list
+    int64_t* pfill;       // transactions written into that list (compacted blocks: word entries
+                          // << 40 | transactions)
     int32_t* crow;        // [bmax * d_max] row of each U-list item in its record's view
 };
 
@@ -242,6 +259,7 @@ enum SgCtl {
     CTL_PROJ_WORDS,       // words read and written building them (cumulative)
     CTL_PROJ_ROWS,        // of them, projected from window rows (cumulative)
     CTL_PROJ_ARENA,       // arena words of the blocks built (cumulative)
+    CTL_PROJ_CUNITS,      // compress units of the round's new blocks
     CTL_SLOTS = 32 + SG_NBUCKETS
 };
 
@@ -267,9 +285,19 @@ hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int
 #define SG_PT_WORDS 1024
 #define SG_PG_TIDS 1024
 #define SG_PG_ROWS 64
+// Compacted blocks (dense enough, SgSearchDev::proj_dense) are built word by word
+// (k_proj_compress) instead of bit by bit (k_proj_gather); a compress unit is SG_PC_WORDS
+// block words of SG_PC_ROWS rows.
+#ifndef SG_PC_WORDS
+#define SG_PC_WORDS 32
+#endif
+#ifndef SG_PC_ROWS
+#define SG_PC_ROWS 16
+#endif
 // Stages of a search round, timed with events on direct launches: selection + U lists,
-// projected blocks, need rows, counting, expansion, scoring + commit.
-#define SG_STAGES 6
+// the new blocks' transaction lists, their gathered rows, their compacted rows, need rows,
+// counting, expansion, scoring + commit.
+#define SG_STAGES 8
 hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgSegment* segs,
                            const int32_t* pair, const uint64_t* ok, const int32_t* sup_all,
                            const int64_t* seg_item_base, int32_t uniform, int32_t order_mode,

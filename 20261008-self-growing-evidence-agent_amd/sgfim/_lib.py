@@ -74,6 +74,8 @@ _SIGS = {
     "sgfim_stage_times": (C.c_int, [P, P, C.c_int32]),
     "sgfim_count_batch_noalloc": (C.c_int, [P, P, P, C.c_int32, P, P, P]),
     "sgfim_get_projection": (C.c_int, [P, P, P]),
+    "sgfim_set_block_rule": (C.c_int, [P, C.c_int32, C.c_int32]),
+    "sgfim_get_block_rule": (C.c_int, [P, P, P]),
     "sgfim_device_info": (C.c_int, [P, C.c_char_p, C.c_int32, P]),
     "sgfim_result_windows": (C.c_int, [P, PP]),
     "sgfim_load_zipf": (C.c_int, [P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,

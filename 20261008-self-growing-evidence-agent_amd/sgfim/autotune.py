@@ -11,11 +11,14 @@ candidate rows through registers. Two knobs shape it and neither changes a resul
 A third knob is the projection factor (sgfim_set_projection): a record whose support is at
 most 1/factor of its rows' transactions gets its U rows compacted to its transactions, and
 its subtree counts on them. Deep lattices over long rows gain (workload 5), shallow ones
-pay the projection without reusing it (workload 2), so it is tuned per input shape too.
-tune() times a caller's search for every projection factor on the loaded data, then for every
-candidate shape with the chosen factor, checks that every setting returns the same MFIs,
-keeps the fastest, and records it in a JSON cache keyed by device and input shape; apply()
-sets a cached setting. `python -m sgfim.autotune` tunes the benchmark workloads and writes
+pay the projection without reusing it (workload 2), so it is tuned per input shape too, and
+so is the block rule (sgfim_set_block_rule): which blocks are compacted word by word and
+how many children such a block needs to repay its build (deep subtrees reuse a block
+many levels down, shallow ones only in the record's own round).
+tune() times a caller's search for every projection factor on the loaded data, then every
+block rule with the chosen factor, then every candidate shape, checks that every setting
+returns the same MFIs, keeps the fastest, and records it in a JSON cache keyed by device
+and input shape; apply() sets a cached setting. `python -m sgfim.autotune` tunes the benchmark workloads and writes
 tile_cache.json next to this file.
 """
 from __future__ import annotations
@@ -31,6 +34,8 @@ CANDIDATES = ((128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (512, 
 DEFAULT = (256, 4)
 PROJ_CANDIDATES = (0, 4, 8, 16, 32)
 PROJ_DEFAULT = 4   # the library's default factor
+RULE_CANDIDATES = ((32, 16), (32, 4), (32, 2), (8, 16), (8, 2), (0, 2))   # (dense, min_children)
+RULE_DEFAULT = (32, 16)   # the library's default block rule
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_cache.json")
 
 
@@ -77,14 +82,22 @@ def lookup_projection(key: str, path: str = CACHE):
     return int(e["proj_factor"]) if e and "proj_factor" in e else None
 
 
+def lookup_rule(key: str, path: str = CACHE):
+    """(dense, min_children) cached for key, or None."""
+    e = load_cache(path).get(key)
+    return (int(e["proj_dense"]), int(e["proj_mintail"])) if e and "proj_dense" in e else None
+
+
 def apply(ctx, n_sets: int = 1, path: str = CACHE):
-    """Set the cached tile shape and projection factor for this context's device and data;
-    returns the shape (or the default). Without a cached factor the library default stays."""
+    """Set the cached tile shape, projection factor and block rule for this context's
+    device and data; returns the shape (or the default). Without a cached value the
+    library default stays."""
     key = context_key(ctx, n_sets)
     t = lookup(key, path)
     ctx.set_tile(*(t or (0, 0)))
     f = lookup_projection(key, path)
     ctx.set_projection(PROJ_DEFAULT if f is None else f)
+    ctx.set_block_rule(*(lookup_rule(key, path) or RULE_DEFAULT))
     return t or DEFAULT
 
 
@@ -137,11 +150,12 @@ def _time(run, reps: int) -> float:
 
 
 def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None, path: str | None = CACHE,
-         verify: bool = True, log=None, proj_candidates=PROJ_CANDIDATES) -> dict:
+         verify: bool = True, log=None, proj_candidates=PROJ_CANDIDATES, rule_candidates=RULE_CANDIDATES) -> dict:
     """Time run() (a search on ctx returning a MineResult) under every projection factor with
-    the default tile shape, keep the fastest factor, then time every candidate tile shape
-    with that factor and keep the fastest (each time the best of reps wall times after one
-    warm-up run, which also captures the round graphs). With verify, every setting must give
+    the default tile shape and block rule, keep the fastest factor, then every block rule
+    with that factor (when blocks are on), then every candidate tile shape, keeping the
+    fastest each time (the best of reps wall times after one warm-up run, which also
+    captures the round graphs). With verify, every setting must give
     the same MFIs. The chosen setting is left on ctx and, when key and path are given, stored
     in the cache."""
     ref = None
@@ -157,6 +171,7 @@ def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None,
                 raise RuntimeError(f"{what} changed the result")
 
     ctx.set_tile(*DEFAULT)
+    ctx.set_block_rule(*RULE_DEFAULT)
     ptimes = {}
     for f in proj_candidates:
         ctx.set_projection(f)
@@ -166,7 +181,19 @@ def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None,
             log(f"projection factor {f}: {ptimes[f]:.3f} ms")
     pf = min(ptimes, key=lambda f: (ptimes[f], f != PROJ_DEFAULT))
     ctx.set_projection(pf)
-    timings = {DEFAULT: ptimes[pf]}
+    rtimes = {RULE_DEFAULT: ptimes[pf]}
+    if pf > 0:
+        for rule in rule_candidates:
+            if rule == RULE_DEFAULT:
+                continue
+            ctx.set_block_rule(*rule)
+            warm_check(f"block rule {rule}")
+            rtimes[rule] = _time(run, reps)
+            if log:
+                log(f"block rule dense={rule[0]} min_children={rule[1]}: {rtimes[rule]:.3f} ms")
+    rule = min(rtimes, key=lambda r: (rtimes[r], r != RULE_DEFAULT))
+    ctx.set_block_rule(*rule)
+    timings = {DEFAULT: rtimes[rule]}
     for cw, cr in candidates:
         if (cw, cr) == DEFAULT:
             continue
@@ -177,10 +204,12 @@ def tune(ctx, run, candidates=CANDIDATES, reps: int = 3, key: str | None = None,
             log(f"tile chunk_words={cw} chunk_range={cr}: {timings[(cw, cr)]:.3f} ms")
     best = choose({sh: timings[sh] for sh in candidates if sh in timings})
     ctx.set_tile(*best)
-    entry = {"chunk_words": best[0], "chunk_range": best[1], "proj_factor": pf, "ms": round(timings[best], 4),
+    entry = {"chunk_words": best[0], "chunk_range": best[1], "proj_factor": pf,
+             "proj_dense": rule[0], "proj_mintail": rule[1], "ms": round(timings[best], 4),
              "default_ms": round(ptimes.get(PROJ_DEFAULT, float("nan")), 4),
              "all": {f"{cw}x{cr}": round(ms, 4) for (cw, cr), ms in timings.items()},
-             "proj_all": {str(f): round(ms, 4) for f, ms in sorted(ptimes.items())}}
+             "proj_all": {str(f): round(ms, 4) for f, ms in sorted(ptimes.items())},
+             "rule_all": {f"{d}/{m}": round(ms, 4) for (d, m), ms in sorted(rtimes.items())}}
     if key and path:
         save_entry(key, entry, path)
     return entry

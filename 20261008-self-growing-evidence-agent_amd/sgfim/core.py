@@ -326,6 +326,18 @@ class Context:
         self._check(self._L.sgfim_get_projection(self._h, C.byref(f), C.byref(a)), "get_projection")
         return f.value, a.value
 
+    def set_block_rule(self, dense: int = 32, min_children: int = 16):
+        """How later searches build projected blocks: one at least 1/dense of its source's
+        transactions (dense 0: none) is compacted word by word and needs min_children
+        children; sparser ones are gathered bit by bit. Results do not depend on it."""
+        self._check(self._L.sgfim_set_block_rule(self._h, int(dense), int(min_children)), "set_block_rule")
+
+    def block_rule(self) -> tuple:
+        """(dense, min_children) the next search uses."""
+        d, m = C.c_int32(), C.c_int32()
+        self._check(self._L.sgfim_get_block_rule(self._h, C.byref(d), C.byref(m)), "get_block_rule")
+        return d.value, m.value
+
     def tile(self) -> tuple:
         """(chunk_words, chunk_range) the next search uses."""
         cw, cr = C.c_int32(), C.c_int32()
@@ -345,7 +357,7 @@ class Context:
         self._check(self._L.sgfim_copy_probe(self._h, int(nbytes), int(iters), C.byref(g)), "copy_probe")
         return g.value
 
-    STAGES = ("select", "project", "need_rows", "count", "expand", "commit")
+    STAGES = ("select", "proj_tids", "proj_gather", "proj_compress", "need_rows", "count", "expand", "commit")
 
     def stage_times(self) -> dict:
         """Device ms of each round stage of the latest search run with NO_GRAPH (zeros otherwise)."""

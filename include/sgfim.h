@@ -201,6 +201,16 @@ int sgfim_get_tile(const sgfim_ctx* ctx, int32_t* chunk_words, int32_t* chunk_ra
  * not depend on either value. */
 int sgfim_set_projection(sgfim_ctx* ctx, int32_t factor, int64_t arena_words);
 int sgfim_get_projection(const sgfim_ctx* ctx, int32_t* factor, int64_t* arena_words);
+
+/* How projected blocks of later searches are built. A block whose support is at least
+ * 1/dense of its source's transactions (dense = 0: none) is compacted word by word: each
+ * nonzero word of T(X) in the source view is bit-compressed in every row, about 50 lane
+ * operations per source word and row, which a record repays only with min_children or more
+ * children (2..4096, default 16); sparser blocks are gathered bit by bit and need 2
+ * children. Default dense = 32. No reference counterpart (see sgfim_set_projection).
+ * Results do not depend on either value. */
+int sgfim_set_block_rule(sgfim_ctx* ctx, int32_t dense, int32_t min_children);
+int sgfim_get_block_rule(const sgfim_ctx* ctx, int32_t* dense, int32_t* min_children);
 /* The context's device: its gcnArchName (NUL-terminated, at most len - 1 characters) and
  * compute units; the key of the tile autotuner's cache. No counterpart in the reference
  * (its Results header prints the device, kernel_bitmap.cu:608-690). */
@@ -321,8 +331,9 @@ int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
 
 /* Device time (ms, HIP events) of each stage of the current or latest search, summed over
  * its rounds, for searches run with SGFIM_NO_GRAPH (one round at a time, every launch
- * bracketed); 0 otherwise. ms receives min(n, 6) values: selection and U lists, projected
- * blocks, need rows, counting, expansion, scoring and commit. A profile of the round's
+ * bracketed); 0 otherwise. ms receives min(n, 8) values: selection and U lists, the
+ * projected blocks' transaction lists, their rows built bit by bit, their rows built word by
+ * word, need rows, counting, expansion, scoring and commit. A profile of the round's
  * kernels without an external profiler; no reference counterpart (the engines print phase
  * totals, kernel_bmma.cu:985-986). */
 int sgfim_stage_times(const sgfim_ctx* ctx, double* ms, int32_t n);

@@ -35,7 +35,14 @@ def test_committed_cache_entries_are_valid_shapes():
         if "proj_factor" in e:
             assert e["proj_factor"] in A.PROJ_CANDIDATES, key
             assert e["ms"] <= min(e["proj_all"].values()) + 1e-9, key   # tile tuned with the best factor
-            assert e["all"][f"{A.DEFAULT[0]}x{A.DEFAULT[1]}"] == e["proj_all"][str(e["proj_factor"])], key
+            base = e["proj_all"][str(e["proj_factor"])]
+            if "rule_all" in e:   # rule tuned with the best factor, tile with the best rule
+                rule = (e["proj_dense"], e["proj_mintail"])
+                assert rule in A.RULE_CANDIDATES, key
+                assert e["rule_all"][f"{A.RULE_DEFAULT[0]}/{A.RULE_DEFAULT[1]}"] == base, key
+                assert e["ms"] <= min(e["rule_all"].values()) + 1e-9, key
+                base = e["rule_all"][f"{rule[0]}/{rule[1]}"]
+            assert e["all"][f"{A.DEFAULT[0]}x{A.DEFAULT[1]}"] == base, key
 
 
 def test_projection_lookup(tmp_path):
@@ -45,6 +52,16 @@ def test_projection_lookup(tmp_path):
     assert A.lookup_projection("k", p) == 16
     assert A.lookup_projection("old", p) is None
     assert A.lookup_projection("absent", p) is None
+
+
+def test_rule_lookup(tmp_path):
+    p = str(tmp_path / "tiles.json")
+    A.save_entry("k", {"chunk_words": 256, "chunk_range": 4, "proj_factor": 16, "proj_dense": 8,
+                       "proj_mintail": 2, "ms": 1.0}, p)
+    A.save_entry("old", {"chunk_words": 256, "chunk_range": 4, "proj_factor": 16, "ms": 1.0}, p)
+    assert A.lookup_rule("k", p) == (8, 2)
+    assert A.lookup_rule("old", p) is None
+    assert A.RULE_DEFAULT in A.RULE_CANDIDATES
 
 
 def test_signature_is_order_free_and_sensitive():

@@ -87,6 +87,34 @@ def test_cancelled_anytime_run_reports_only_final_mfis():
         assert ctx.mine(thr).as_dict() == ref
 
 
+@pytest.mark.parametrize("flags", [0, L.ORDER_LEARNED])
+def test_direct_launch_anytime_rounds_of_64_match_oracle(flags):
+    """Anytime stepping with direct launches (no captured graphs, stage events on every
+    launch) in rounds of 64 records, the host path that drains the MFI ring and reads the
+    control snapshot and stage events every round; the stage times add up to device time."""
+    print(f"case: workload 4 seed 1000009 at 60,000 txns, 2%, NO_GRAPH anytime, rounds of 64, flags={flags}",
+          flush=True)
+    T = gen.workload4(1_000_009, 60_000)
+    thr = gen.min_count(0.02, T.n_txn)
+    ref = _oracle(T, thr)
+    with Context(device=0, batch_nodes=64) as ctx:
+        ctx.load_csr(T.offsets, T.items)
+        ctx.set_scorer(*learned.load_weights())
+        ctx.anytime_begin(thr, flags | L.NO_GRAPH)
+        got = {}
+        rounds = 0
+        while True:
+            st = ctx.anytime_step(1)
+            got.update(ctx.anytime_take().as_dict())
+            rounds += 1
+            if st["done"]:
+                break
+        stages = ctx.stage_times()
+    assert rounds > 20
+    assert got == ref
+    assert all(v >= 0.0 for v in stages.values()) and stages["count"] > 0.0
+
+
 def test_anytime_rounds_on_a_compact_universe_in_learned_order():
     print("case: 6,000 loaded items (zipf 0.3, n=20000), min_count 400, anytime learned order, rounds of 256",
           flush=True)

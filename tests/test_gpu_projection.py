@@ -3,8 +3,12 @@ counted (projection factor, arena size, round size, frontier order, windows, sev
 thresholds) gives the oracle's exact MFI family.
 
 A projected block holds the rows of U(Y) restricted and bit-compacted to T(Y); the
-records under Y count on it. The cases cover:
+records under Y count on it. Dense blocks are compacted word by word (bit compress of each
+nonzero source word), sparse ones gathered bit by bit; the block rule says which and how
+many children a dense block needs. The cases cover:
 - factor 0 (window rows only), 2, 4 and 16;
+- block rules that build every block word by word (dense 2), every block bit by bit
+  (dense 0), and mixes, with 2 to 16 children required;
 - arenas so small that blocks are refused or overwritten before their records are
   counted (the records then fall back to window rows or project again);
 - rounds of one record (each block built alone) up to whole batches;
@@ -38,32 +42,37 @@ def _by_tag(res, n):
 
 
 CASES = [
-    # kind, n, rel, factor, arena words, batch, order
-    ("w4", 30_000, 0.02, 4, 0, 0, 0),
-    ("w4", 30_000, 0.02, 2, 0, 64, L.ORDER_LEARNED),
-    ("w4", 30_000, 0.02, 16, 1 << 12, 256, 0),        # tiny arena: most blocks refused
-    ("w4", 30_000, 0.02, 4, 1 << 16, 7, 0),           # small arena: blocks overwritten
-    ("w4", 30_000, 0.02, 4, 1 << 16, 1, L.ORDER_LEARNED),
-    ("w1", 20_000, 0.005, 2, 0, 0, 0),
-    ("w1", 20_000, 0.005, 4, 1 << 14, 33, L.ORDER_LEARNED),
-    ("zipf", 8_000, 0.03, 4, 0, 0, 0),
-    ("zipf", 8_000, 0.03, 2, 1 << 15, 128, L.ORDER_LEARNED),
-    ("windows", 0, 0.03, 4, 0, 0, 0),
-    ("windows", 0, 0.02, 2, 1 << 15, 16, L.ORDER_LEARNED),
-    ("multi", 25_000, 0.02, 4, 0, 0, 0),
-    ("multi", 25_000, 0.02, 4, 1 << 16, 100, L.ORDER_LEARNED),
+    # kind, n, rel, factor, arena words, batch, order, block rule (dense, min children)
+    ("w4", 30_000, 0.02, 4, 0, 0, 0, (32, 2)),
+    ("w4", 30_000, 0.02, 4, 0, 0, 0, (32, 16)),
+    ("w4", 30_000, 0.02, 4, 0, 0, L.ORDER_LEARNED, (2, 2)),    # every block word by word
+    ("w4", 30_000, 0.02, 4, 0, 0, 0, (0, 2)),                  # every block bit by bit
+    ("w4", 30_000, 0.02, 2, 0, 64, L.ORDER_LEARNED, (8, 2)),
+    ("w4", 30_000, 0.02, 16, 1 << 12, 256, 0, (32, 2)),        # tiny arena: most blocks refused
+    ("w4", 30_000, 0.02, 4, 1 << 16, 7, 0, (32, 4)),           # small arena: blocks overwritten
+    ("w4", 30_000, 0.02, 4, 1 << 16, 1, L.ORDER_LEARNED, (2, 2)),
+    ("w1", 20_000, 0.005, 2, 0, 0, 0, (32, 2)),
+    ("w1", 20_000, 0.005, 4, 1 << 14, 33, L.ORDER_LEARNED, (2, 2)),
+    ("zipf", 8_000, 0.03, 4, 0, 0, 0, (2, 2)),
+    ("zipf", 8_000, 0.03, 2, 1 << 15, 128, L.ORDER_LEARNED, (8, 16)),
+    ("windows", 0, 0.03, 4, 0, 0, 0, (2, 2)),
+    ("windows", 0, 0.02, 2, 1 << 15, 16, L.ORDER_LEARNED, (32, 2)),
+    ("multi", 25_000, 0.02, 4, 0, 0, 0, (2, 2)),
+    ("multi", 25_000, 0.02, 4, 1 << 16, 100, L.ORDER_LEARNED, (32, 16)),
 ]
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_projected_search_matches_oracle(case):
-    kind, n, rel, factor, arena, batch, flags = CASES[case]
-    print(f"case {case}: kind={kind} n={n} rel={rel} factor={factor} arena={arena} batch={batch} flags={flags}",
-          flush=True)
+    kind, n, rel, factor, arena, batch, flags, rule = CASES[case]
+    print(f"case {case}: kind={kind} n={n} rel={rel} factor={factor} arena={arena} batch={batch} flags={flags} "
+          f"rule={rule}", flush=True)
     with Context(device=0, batch_nodes=batch) as ctx:
         ctx.set_scorer(*learned.load_weights())
         ctx.set_projection(factor, arena)
+        ctx.set_block_rule(*rule)
         assert ctx.projection() == (factor, arena)
+        assert ctx.block_rule() == rule
         if kind == "windows":
             wins = gen.workload3(case, n_windows=6, lo=500, hi=20_000)
             win_off, off, items, _ = gen.concat_windows(wins)
@@ -94,7 +103,7 @@ def test_projected_search_matches_oracle(case):
         st = ctx.last_stats()
         print(f"  {len(res)} MFIs, {st['projections']} blocks ({st['proj_from_rows']} from window rows)", flush=True)
         assert res.as_dict() == _oracle(off, items, d, thr)
-        if factor and not arena and kind == "w4":
+        if factor and not arena and kind == "w4" and rule[1] == 2:
             assert st["projections"] > 0   # the case exercises blocks, and blocks of blocks
             assert st["projections"] > st["proj_from_rows"]
 
@@ -107,3 +116,9 @@ def test_projection_settings_are_validated():
             assert e.value.code == L.E_INVAL
         ctx.set_projection(0)
         assert ctx.projection() == (0, 0)
+        for d, m in ((1, 2), (-1, 2), (2000, 2), (32, 1), (32, 0), (32, 5000)):
+            with pytest.raises(L.SgfimError) as e:
+                ctx.set_block_rule(d, m)
+            assert e.value.code == L.E_INVAL
+        ctx.set_block_rule(0, 2)
+        assert ctx.block_rule() == (0, 2)
