@@ -41,11 +41,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(kernel_prefix: str = "k_count_group"):
+def pmc_traffic(setting: str, kernel_prefix: str = "k_count_group"):
     """HBM bytes per dispatch of the counting kernel from the committed counter profile: the
-    line `pmc: kernel=... fetch_kib=F write_kib=W fetch_scale=a write_scale=b` written from two
-    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean over the run),
-    scaled by what the same counters read for the copy probe of known size."""
+    line `pmc: kernel=... setting=... fetch_kib=F write_kib=W fetch_scale=a write_scale=b`
+    written from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean
+    over the run) of the bench at that search setting, scaled by what the same counters read
+    for the copy probe of known size. None when no line matches the setting."""
     if not os.path.exists(PMC_PROFILE):
         return None
     for ln in open(PMC_PROFILE):
@@ -53,13 +54,22 @@ def pmc_traffic(kernel_prefix: str = "k_count_group"):
         if not m:
             continue
         kv = dict(x.split("=", 1) for x in m.group(1).split())
-        if not kv.get("kernel", "").startswith(kernel_prefix):
+        if not kv.get("kernel", "").startswith(kernel_prefix) or kv.get("setting") != setting:
             continue
         b = 1024.0 * (float(kv["fetch_scale"]) * float(kv["fetch_kib"]) +
                       float(kv["write_scale"]) * float(kv["write_kib"]))
-        return {"bytes": b, "kernel": kv["kernel"], "commit": kv.get("commit"),
+        return {"bytes": b, "kernel": kv["kernel"], "setting": setting, "commit": kv.get("commit"),
                 "source": os.path.relpath(PMC_PROFILE, ROOT)}
     return None
+
+
+def setting_name(ctx) -> str:
+    """The search setting a counter profile line belongs to: tile, projection factor and
+    block rule."""
+    cw, cr = ctx.tile()
+    f = ctx.projection()[0]
+    d, m = ctx.block_rule()
+    return f"tile{cw}x{cr}-f{f}" + (f"-d{d}m{m}" if f else "")
 
 
 def pair_tiles(dpad: int) -> int:
@@ -191,6 +201,7 @@ def workload5_run(Context, steps=2):
     ctx.set_scorer(*learned.load_weights())
     tile = autotune.apply(ctx, 1)
     proj = ctx.projection()[0]
+    rule = ctx.block_rule()
     thr = gen.min_count(0.005, ctx.n_txn)
     ctx.mine(thr, flags=ORDER_LEARNED, convert=False)      # warm: buffers, captured rounds
     ts, r = [], None
@@ -226,7 +237,8 @@ def workload5_run(Context, steps=2):
             "valu_frac_count_kernel": 4.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e12 / VALU_PEAK_TLANE
             if st["count_ms"] > 0 else None,
             "stage_ms_direct_launch": stages, "direct_launch_seconds": rd.stats["elapsed_s"],
-            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj},
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj,
+                     "block_rule": {"dense": rule[0], "min_children": rule[1]}},
             "coverage": {"learned": cl, "plain": cp,
                          "identical_final_sets": cl["fingerprint"] == cp["fingerprint"] == fp_full,
                          "t25_ratio": cl["t25_s"] / cp["t25_s"], "t50_ratio": cl["t50_s"] / cp["t50_s"],
@@ -235,14 +247,36 @@ def workload5_run(Context, steps=2):
                                        "search began (times of confirmation, host clock)"}}
 
 
-def workload4_run(Context, n_seeds=4, batch_nodes=4096):
+def _w4_curve(ctx, thr, flags):
+    """One anytime search to completion, round by round: per-round (elapsed s, counts, MFIs
+    so far) and the final set's fingerprint (flat arrays; no per-itemset objects)."""
+    rec, parts, n = [], [], 0
+    ctx.anytime_begin(thr, flags)
+    while True:
+        st = ctx.anytime_step(1)
+        r = ctx.anytime_take(convert=False)
+        if r.count:
+            parts.append(r.arrays)
+            n += r.count
+        rec.append((st["elapsed_s"], st["counts"], n))
+        if st["done"]:
+            break
+    it = np.concatenate([p[0] for p in parts]) if parts else np.zeros(0, np.int32)
+    sup = np.concatenate([p[2] for p in parts]) if parts else np.zeros(0, np.int64)
+    lens = np.concatenate([np.diff(p[1]) for p in parts]) if parts else np.zeros(0, np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    return rec, fingerprint(it, off, sup)
+
+
+def workload4_run(Context, n_seeds=20, batch_nodes=4096):
     """Anytime coverage on workload 4 (500,000 txns x 256 items, 2% floor): time until 25%,
     50% and 95% of the final MFI set is confirmed, learned frontier order against the plain
-    support order, on the first held-out seeds of the proposal network's evaluation (the
+    support order, on the 20 held-out seeds of the proposal network's evaluation (the
     committed weights). Each stream gets an untimed run first, and the timed runs alternate
     which order goes first."""
     from sgfim import autotune, gen, learned
-    from sgfim.train_proposal import compare_orders, streams, t_at
+    from sgfim._lib import ORDER_LEARNED
+    from sgfim.train_proposal import streams, t_at
     ctx = Context(device=0, batch_nodes=batch_nodes)
     ctx.set_scorer(*learned.load_weights())
     tile = None
@@ -252,21 +286,28 @@ def workload4_run(Context, n_seeds=4, batch_nodes=4096):
         ctx.load_csr(off, items)
         tile = tile or autotune.apply(ctx, 1)
         thr = gen.min_count(0.02, ctx.n_txn)
-        rp, fp, rl, fl = compare_orders(ctx, thr, plain_first=k % 2 == 0)
-        tot = len(fp)
-        rows.append({"seed": s, "mfis": tot, "identical": fp == fl, "plain_first": k % 2 == 0,
+        _w4_curve(ctx, thr, 0)   # untimed: buffers, captured rounds
+        order = (0, ORDER_LEARNED) if k % 2 == 0 else (ORDER_LEARNED, 0)
+        got = {fl: _w4_curve(ctx, thr, fl) for fl in order}
+        (rp, fp), (rl, fl_) = got[0], got[ORDER_LEARNED]
+        tot = rp[-1][2]
+        rows.append({"seed": s, "mfis": tot, "identical": fp == fl_ and rl[-1][2] == tot, "plain_first": k % 2 == 0,
                      "t95_plain_s": t_at(rp, 0.95, tot, 0), "t95_learned_s": t_at(rl, 0.95, tot, 0),
                      "counts95_plain": t_at(rp, 0.95, tot, 1), "counts95_learned": t_at(rl, 0.95, tot, 1),
                      "t25_plain_s": t_at(rp, 0.25, tot, 0), "t25_learned_s": t_at(rl, 0.25, tot, 0),
                      "t50_plain_s": t_at(rp, 0.5, tot, 0), "t50_learned_s": t_at(rl, 0.5, tot, 0),
-                     "counts50_plain": t_at(rp, 0.5, tot, 1), "counts50_learned": t_at(rl, 0.5, tot, 1)})
-        log(f"workload 4 seed {s}: t95 plain {rows[-1]['t95_plain_s']:.4f} s, learned {rows[-1]['t95_learned_s']:.4f} s")
+                     "counts50_plain": t_at(rp, 0.5, tot, 1), "counts50_learned": t_at(rl, 0.5, tot, 1),
+                     "rounds_plain": len(rp), "rounds_learned": len(rl)})
+        log(f"workload 4 seed {s}: t50 plain {rows[-1]['t50_plain_s']:.4f} s, learned {rows[-1]['t50_learned_s']:.4f} s; "
+            f"t95 plain {rows[-1]['t95_plain_s']:.4f} s, learned {rows[-1]['t95_learned_s']:.4f} s")
     proj = ctx.projection()[0]
+    rule = ctx.block_rule()
     ctx.close()
     med = lambda v: float(np.median(v))
     return {"workload": f"workload 4: 500,000 txns x 256 items, 2% floor, held-out seeds {seeds[0]}..{seeds[-1]}, "
                         f"anytime rounds of up to {batch_nodes} records",
             "metric": "time to 25%, 50% and 95% MFI coverage, learned / plain",
+            "n_seeds": len(rows),
             "median_t95_ratio": med([r["t95_learned_s"] / r["t95_plain_s"] for r in rows]),
             "median_counts95_ratio": med([r["counts95_learned"] / max(r["counts95_plain"], 1) for r in rows]),
             "median_t25_ratio": med([r["t25_learned_s"] / r["t25_plain_s"] for r in rows]),
@@ -279,7 +320,9 @@ def workload4_run(Context, n_seeds=4, batch_nodes=4096):
                               "first) reaches 95% coverage only after 0.79-0.85 of the plain order's counting "
                               "work on these streams (CPU simulation with the oracle's node trace), so no "
                               "reordering reaches 0.5x at 95% on this shape",
-            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj}, "seeds": rows}
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj,
+                     "block_rule": {"dense": rule[0], "min_children": rule[1]}},
+            "seeds": rows}
 
 
 def cpu_extra(budget_s=15.0):
@@ -324,6 +367,11 @@ def main():
                     help="mine the three supports as three searches instead of one batched search")
     ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5"), default=None,
                     help="run one extra workload alone and print its JSON")
+    ap.add_argument("--projection", type=int, default=None,
+                    help="projection factor of the headline search (default: the tuned one)")
+    ap.add_argument("--full-rows-steps", type=int, default=5,
+                    help="steps of the same search with projection off, for the counting kernel's "
+                         "roofline on full-length rows (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -350,9 +398,14 @@ def main():
     from sgfim import autotune
     tile_key = autotune.context_key(ctx, 1 if args.sequential else len(thrs))
     tile = autotune.apply(ctx, 1 if args.sequential else len(thrs))
+    if args.projection is not None:
+        ctx.set_projection(args.projection)
     proj = ctx.projection()[0]
+    rule = ctx.block_rule()
+    setting = setting_name(ctx)
     tile_src = "tile_cache.json" if autotune.lookup(tile_key) else "default"
-    log(f"counting tile {tile[0]} words x {tile[1]} chunks, projection factor {proj} ({tile_src}, {tile_key})")
+    log(f"counting tile {tile[0]} words x {tile[1]} chunks, projection factor {proj}, block rule {rule} "
+        f"({tile_src}, {tile_key})")
 
     pair_ms = []
     pair_counts = ctx.n_items * (ctx.n_items + 1) // 2   # distinct pair supports (the diagonal included)
@@ -388,21 +441,49 @@ def main():
         pair_ms.append(ctx.pair_time())
         return counts, words, cms, launches, mfis
 
-    for i in range(args.warmup):
-        step()
-        log(f"warmup step {i + 1}/{args.warmup} done")
-    pair_ms.clear()
-    tot_counts = tot_words = tot_launch = 0
-    tot_cms = 0.0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        c, w, cms, nl, mfis = step()
-        tot_counts += c
-        tot_words += w
-        tot_cms += cms
-        tot_launch += nl
-        log(f"step {i + 1}/{args.steps}: {c} counts, MFIs per support {mfis}")
-    dt = time.perf_counter() - t0
+    def timed(steps, warmup, what):
+        for i in range(warmup):
+            step()
+            log(f"{what}: warmup step {i + 1}/{warmup} done")
+        pair_ms.clear()
+        tot = {"counts": 0, "words": 0, "cms": 0.0, "launches": 0, "mfis": None}
+        t0 = time.perf_counter()
+        for i in range(steps):
+            c, w, cms, nl, mfis = step()
+            tot["counts"] += c
+            tot["words"] += w
+            tot["cms"] += cms
+            tot["launches"] += nl
+            tot["mfis"] = mfis
+            log(f"{what}: step {i + 1}/{steps}: {c} counts, MFIs per support {mfis}")
+        tot["dt"] = time.perf_counter() - t0
+        return tot
+
+    def count_roofline(tot, setting_key):
+        """k_count_group against the VALU issue peak, with its HBM traffic from the counter
+        profile of the same search setting."""
+        avg = tot["cms"] / max(tot["launches"], 1)
+        # the AND + popcount a count needs: per 64-bit word two 32-bit ANDs and two popcounts,
+        # over the words of every needed count and of every child prefix, per second of counting
+        achieved = 4.0 * tot["words"] / (tot["cms"] * 1e-3) / 1e12 if tot["cms"] > 0 else 0.0
+        traffic = pmc_traffic(setting_key)
+        hbm = None
+        if traffic and avg > 0:
+            gbs = traffic["bytes"] / (avg * 1e-3) / 1e9
+            hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                   "traffic_source": traffic["source"], "measured_at_commit": traffic["commit"]}
+        return {"bound": "valu", "kernel": "k_count_group", "achieved": achieved, "peak": VALU_PEAK_TLANE,
+                "unit": "T lane-op/s", "frac": achieved / VALU_PEAK_TLANE,
+                "counted": "4 lane-ops (2 AND, 2 popcount) per 64-bit word of every needed count and "
+                           "every child prefix, over the rows each record counts on",
+                "traffic": traffic["bytes"] if traffic else None, "setting": setting_key,
+                "avg_launch_ms": avg, "launches": tot["launches"], "hbm": hbm,
+                "alg_bytes_per_launch": 8.0 * tot["words"] / max(tot["launches"], 1)}
+
+    head = timed(args.steps, args.warmup, "headline")
+    pair_avg = sum(pair_ms) / len(pair_ms) if pair_ms else 0.0
+    dt = head["dt"]
+    tot_counts, mfis = head["counts"], head["mfis"]
     ms_per_step = 1000.0 * dt / args.steps
     # cross-check of the counting kernel's duration with HIP events on its stream
     # (one direct-launch run at the lowest support, outside the timed steps)
@@ -412,20 +493,20 @@ def main():
                    "launches": rx["count_launches"]}
     copy_gbs = ctx.copy_probe(1 << 31, 5)   # on-box HBM copy rate, 2 GiB buffers (8x the MALL)
     value = tot_counts / dt
-    pair_avg = sum(pair_ms) / len(pair_ms) if pair_ms else 0.0
     pair_ops = 2.0 * pair_tiles(((ctx.n_items + 127) // 128) * 128) * 256 * 256 * T.n_txn
     pair_tops = pair_ops / (pair_avg * 1e-3) / 1e12 if pair_avg > 0 else None
-    avg_ms = tot_cms / max(tot_launch, 1)
-    # the AND + popcount a count needs: per 64-bit word two 32-bit ANDs and two popcounts,
-    # over the words of every needed count and of every child prefix, per second of counting
-    valu_achieved = 4.0 * tot_words / (tot_cms * 1e-3) / 1e12 if tot_cms > 0 else 0.0
-    traffic = pmc_traffic()
-    hbm = None
-    if traffic and avg_ms > 0:
-        gbs = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
-        hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-               "traffic_source": traffic["source"], "measured_at_commit": traffic["commit"]}
-    alg_bytes = 8.0 * tot_words / max(tot_launch, 1)
+    roof = count_roofline(head, setting)
+    roof["timing_check"] = event_check
+    roof["copy_probe_gbs"] = copy_gbs
+    full = None
+    if proj and args.full_rows_steps > 0:
+        # the same search with projection off: every count over the full 15,625-word rows
+        ctx.set_projection(0)
+        fr = timed(args.full_rows_steps, 1, "full rows")
+        full = count_roofline(fr, setting_name(ctx))
+        full["ms_per_step"] = 1000.0 * fr["dt"] / args.full_rows_steps
+        full["counts_per_s"] = fr["counts"] / fr["dt"]
+        ctx.set_projection(proj)
     out = {
         "metric": "support counts/s (exact MFI mining, workload 2 at 10%/5%/2%)",
         "value": value,
@@ -445,18 +526,13 @@ def main():
                              else "one batched search over the three supports",
                    "counts": "needed AND+popcount supports of the search plus d(d+1)/2 pair supports",
                    "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj,
-                            "source": tile_src}},
+                            "block_rule": {"dense": rule[0], "min_children": rule[1]}, "source": tile_src}},
         # the bound that holds for the counting kernel is VALU issue: tiles reuse every loaded
         # chunk across 16 children and workload 2's 64 MB of bitmaps sit in the MALL, so HBM
-        # is far from saturated (hbm below, from the counter profile)
-        "roofline": {"bound": "valu", "kernel": "k_count_group", "achieved": valu_achieved,
-                     "peak": VALU_PEAK_TLANE, "unit": "T lane-op/s", "frac": valu_achieved / VALU_PEAK_TLANE,
-                     "counted": "4 lane-ops (2 AND, 2 popcount) per 64-bit word of every needed count and "
-                                "every child prefix",
-                     "traffic": traffic["bytes"] if traffic else None,
-                     "avg_launch_ms": avg_ms, "launches": tot_launch, "hbm": hbm,
-                     "alg_bytes_per_launch": alg_bytes,
-                     "timing_check": event_check, "copy_probe_gbs": copy_gbs},
+        # is far from saturated (hbm below, from the counter profile of the same setting)
+        "roofline": roof,
+        # the same kernel on the same search with projected blocks off (full-length rows)
+        "roofline_full_rows": full,
         # the pair matrix (part of every step): 2 x 256^2 x N int8 ops per computed tile
         "pair_roofline": {"bound": "mfma", "kernel": "k_pair_tile", "avg_launch_ms": pair_avg,
                           "achieved": pair_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
