@@ -137,6 +137,8 @@ struct sgfim_ctx {
     int32_t* d_sup = nullptr;
     int32_t* d_pair = nullptr;
     bool pair_valid = false;
+    int32_t pair_kind = SG_PAIR_FP4;    // pair-matrix kernel (sgfim_set_pair_kernel)
+    int32_t pair_split_cap = 0;         // its transaction splits at most (0: automatic)
     hipEvent_t pair_ev[2] = {nullptr, nullptr};   // around the latest pair-matrix computation
     bool pair_timed = false;
     int32_t* h_pair_pin = nullptr;      // pinned staging buffer for pair-matrix read-back (dpad x dpad)
@@ -410,12 +412,14 @@ int ensure_pair_u(sgfim_ctx* c, const Universe& U)
     if (*U.pair_valid) return SGFIM_OK;
     const bool timed = c->pair_ev[0] && hipEventRecord(c->pair_ev[0], c->stream) == hipSuccess;
     if (c->nseg == 1) {
-        HIPCHK(c, sg_launch_pair_mfma(U.rows, U.dpad, U.Wp, U.W, U.pair, c->stream));
+        HIPCHK(c, sg_launch_pair_mfma(U.rows, U.dpad, U.Wp, U.W, U.pair, c->pair_kind, c->pair_split_cap,
+                                      c->stream));
     } else {
         int64_t maxw = 0;
         for (int32_t w = 0; w < c->nseg; w++) maxw = std::max(maxw, U.segs[w].words);
         HIPCHK(c, sg_launch_zero32(U.pair, (int64_t)U.dpad * U.dpad * c->nseg, c->stream));
-        HIPCHK(c, sg_launch_pair_mfma_segs(U.rows, U.d_seg, c->nseg, U.dpad, maxw, U.pair, c->stream));
+        HIPCHK(c, sg_launch_pair_mfma_segs(U.rows, U.d_seg, c->nseg, U.dpad, maxw, U.pair, c->pair_kind,
+                                           c->pair_split_cap, c->stream));
     }
     c->pair_timed = timed && hipEventRecord(c->pair_ev[1], c->stream) == hipSuccess;
     *U.pair_valid = true;
@@ -1527,6 +1531,23 @@ int sgfim_pair_time(sgfim_ctx* c, double* ms)
     return SGFIM_OK;
 }
 int sgfim_pair_matrix_valu(sgfim_ctx* c, int32_t* out) { return pair_out(c, out, true); }
+
+static_assert(SG_PAIR_FP4 == SGFIM_PAIR_FP4 && SG_PAIR_INT8 == SGFIM_PAIR_INT8, "pair kernel ids");
+
+int sgfim_set_pair_kernel(sgfim_ctx* c, int32_t kind, int32_t max_splits)
+{
+    if (!c) return SGFIM_E_INVAL;
+    if (kind != SGFIM_PAIR_FP4 && kind != SGFIM_PAIR_INT8)
+        return fail(c, SGFIM_E_INVAL, "pair kernel %d: expected SGFIM_PAIR_FP4 or SGFIM_PAIR_INT8", kind);
+    if (max_splits < 0) return fail(c, SGFIM_E_INVAL, "pair kernel: max_splits %d < 0", max_splits);
+    if (kind != c->pair_kind || max_splits != c->pair_split_cap) {
+        c->pair_kind = kind;
+        c->pair_split_cap = max_splits;
+        c->pair_valid = false;
+        c->cu.pair_valid = false;
+    }
+    return SGFIM_OK;
+}
 
 // The counting behind sgfim_count_batch and sgfim_count_batch_noalloc: the device scratch
 // (node descriptors, compacted candidates, counts, work counter) is allocated per call, or

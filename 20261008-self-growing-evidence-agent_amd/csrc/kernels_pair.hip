@@ -1,4 +1,5 @@
-// Pair co-occurrence matrix C[i][j] = |T_i & T_j| with int8 MFMA on 0/1 tiles.
+// Pair co-occurrence matrix C[i][j] = |T_i & T_j| with MFMA on 0/1 tiles: int8 below,
+// fp4 (the default) further down.
 //
 // The reference counts the pairs as the first wave of its 1-bit BMMA engine: A =
 // the single-item root masks, B = all survivor bitmaps, C = popc(A_i AND B_j)
@@ -145,6 +146,202 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
             }
 }
 
+// ------------------------------------------------------------------ fp4 MFMA tile
+//
+// The same matrix on v_mfma_scale_f32_32x32x64_f8f6f4 with e2m1 (fp4) operands, which
+// the matrix cores run at twice the int8 rate (64 transactions per K step in the 32
+// cycles the int8 instruction needs for 32). Every product is 0 or 1 and the fp32
+// accumulators hold integers. A split spans at most PF_MAXW words = 2^20 transactions,
+// so every partial sum is an integer of at most 21 bits, exact in fp32 (24-bit
+// significand) with room to spare; it is converted to int32 before the atomics add the
+// splits.
+//
+// Widening by bit phase: K step p (0..3) of a group of four bitmap words takes, for each
+// word, the transactions at bit p of every nibble. Masking the word with 0x1111...,
+// 0x2222... or 0x4444... leaves them as the e2m1 values 0.5, 1.0 or 2.0 (bit 3 alone is
+// -0.0, so phase 3 shifts bit 3 to bit 2 first). The E8M0 scale operands of the
+// instruction (2^1, 2^0 or 2^-1 for both A and B) make every matching pair multiply to
+// exactly 1.0. So one VALU instruction (two for phase 3) widens 8 transactions, and
+// lane (r, h) of K step p holds, in VGPR j, half h of word j's phase-p nibbles of row r.
+// A and B use the same (lane, VGPR, nibble) -> transaction map, so the K order inside
+// an instruction does not change the sum.
+//
+// Tiling: a workgroup of 8 waves (two per SIMD, so one wave's widening and LDS reads
+// overlap the other's matrix instructions) owns a 256 x 256 output tile, each wave 64 x
+// 128 as 2 x 4 MFMA tiles (128 fp32 accumulators per lane). Bit tiles of PK words are
+// staged in LDS with rows laid out [half][word], so a lane reads four K steps of its
+// half with one 16-byte read; they are double-buffered (one barrier per stage: the next
+// stage's tile is stored into the other buffer while this one is multiplied), and each
+// four-word group's reads are issued before the previous group's matrix instructions.
+// Upper triangle only, split-N partial tiles added with integer atomics.
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+#define PF_K 8                // words per stage (two groups of four words)
+#define PF_MAXW (1ll << 14)   // words per split (2^20 transactions; a multiple of PF_K)
+
+template <int P>
+__device__ __forceinline__ int sg_f4_phase(uint32_t x)
+{
+    if (P == 0) return (int)(x & 0x11111111u);
+    if (P == 1) return (int)(x & 0x22222222u);
+    if (P == 2) return (int)(x & 0x44444444u);
+    return (int)((x >> 1) & 0x44444444u);
+}
+
+// E8M0 scale of phase P for each operand, so that a matching pair multiplies to 1.0:
+// 0.5 * 0.5 * 2 * 2, 1 * 1, 2 * 2 * 2^-1 * 2^-1.
+template <int P>
+__device__ __forceinline__ constexpr int sg_f4_scale()
+{
+    return P == 0 ? 128 : P == 1 ? 127 : 126;
+}
+
+__device__ __forceinline__ v8i sg_f4_mask_frag(const uint4& v, int p)
+{
+    if (p == 0) return v8i{sg_f4_phase<0>(v.x), sg_f4_phase<0>(v.y), sg_f4_phase<0>(v.z), sg_f4_phase<0>(v.w), 0, 0, 0, 0};
+    if (p == 1) return v8i{sg_f4_phase<1>(v.x), sg_f4_phase<1>(v.y), sg_f4_phase<1>(v.z), sg_f4_phase<1>(v.w), 0, 0, 0, 0};
+    if (p == 2) return v8i{sg_f4_phase<2>(v.x), sg_f4_phase<2>(v.y), sg_f4_phase<2>(v.z), sg_f4_phase<2>(v.w), 0, 0, 0, 0};
+    return v8i{sg_f4_phase<3>(v.x), sg_f4_phase<3>(v.y), sg_f4_phase<3>(v.z), sg_f4_phase<3>(v.w), 0, 0, 0, 0};
+}
+
+// Workgroup q of a one-segment grid of n: consecutive workgroup ids go to the eight
+// XCDs in turn, so q is renumbered XCD-major; the tiles of one transaction split then
+// share an XCD and its L2 (they read the same rows).
+__device__ __forceinline__ int sg_xcd_major(int q, int n)
+{
+    const int x = q & 7, k = q >> 3;
+    return x * (n >> 3) + min(x, n & 7) + k;
+}
+
+template <bool SEGS, int PK>
+__global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ rows_all,
+                                                  const SgSegment* __restrict__ segs, SgSegment one,
+                                                  int32_t* __restrict__ out_all, int32_t splits)
+{
+    constexpr int NT = 512;
+    constexpr int MA = 2;                 // A fragments (32 rows each) per wave
+    constexpr int LDW = 2 * PK + 4;       // LDS row stride in dwords: [half][word] + pad
+    constexpr int LQ = PK;                // loader threads per row
+    constexpr int LR = NT / LQ;           // rows per loader pass
+    constexpr int LP = PT_TILE / LR;      // loader passes per matrix
+    constexpr int G = PK / 4;             // four-word groups per stage
+    __shared__ __attribute__((aligned(16))) uint32_t As[2][PT_TILE * LDW];
+    __shared__ __attribute__((aligned(16))) uint32_t Bs[2][PT_TILE * LDW];
+    const SgSegment sg = SEGS ? segs[blockIdx.z] : one;
+    const int dpad = sg.dpad;
+    const int T = (dpad + PT_TILE - 1) / PT_TILE;
+    int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+    if (!SEGS) {
+        const int q = sg_xcd_major(bx + (int)gridDim.x * by, (int)(gridDim.x * gridDim.y));
+        bx = q % (int)gridDim.x;
+        by = q / (int)gridDim.x;
+    }
+    if (bx >= T * (T + 1) / 2) return;
+    int ti, tj;
+    sg_tri_pair(bx, T, ti, tj);
+    const int I0 = ti * PT_TILE, J0 = tj * PT_TILE;
+    const int64_t per = ((sg.words + splits - 1) / splits + PK - 1) / PK * PK;
+    const int64_t wb = (int64_t)by * per;
+    const int64_t we = min(sg.words, wb + per);
+    if (wb >= we) return;
+    const uint64_t* rows = rows_all + sg.row_base;
+    int32_t* out = out_all + sg.pair_base;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r = lane & 31, h = lane >> 5;
+    const int wr = wave >> 1, wc = wave & 1;
+    // loader: thread t moves word (t % PK) of rows t / PK + LR i, i < LP, of A and B. Rows
+    // past dpad (a partial last tile) are clamped to row dpad - 1: they only feed output
+    // rows and columns that are never stored. A row's words are zero from the segment's
+    // last word up to its stride (a multiple of 64), and a split's end is a multiple of PK
+    // otherwise, so the loads need no guard.
+    const int lq = tid % LQ, lr = tid / LQ;
+    const uint64_t* pa[LP];
+    const uint64_t* pb[LP];
+#pragma unroll
+    for (int i = 0; i < LP; i++) {
+        pa[i] = rows + (int64_t)min(I0 + lr + LR * i, dpad - 1) * sg.stride + lq;
+        pb[i] = rows + (int64_t)min(J0 + lr + LR * i, dpad - 1) * sg.stride + lq;
+    }
+    uint64_t ga[LP], gb[LP];
+    auto gload = [&](int64_t w0) {
+#pragma unroll
+        for (int i = 0; i < LP; i++) {
+            ga[i] = pa[i][w0];
+            gb[i] = pb[i][w0];
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < LP; i++) {
+            const int row = lr + LR * i;
+            As[buf][row * LDW + lq] = (uint32_t)ga[i];
+            As[buf][row * LDW + PK + lq] = (uint32_t)(ga[i] >> 32);
+            Bs[buf][row * LDW + lq] = (uint32_t)gb[i];
+            Bs[buf][row * LDW + PK + lq] = (uint32_t)(gb[i] >> 32);
+        }
+    };
+    uint4 av[2][MA], bv[2][4];
+    auto lread = [&](int buf, int g, int slot) {
+#pragma unroll
+        for (int a = 0; a < MA; a++)
+            av[slot][a] = *(const uint4*)&As[buf][(32 * MA * wr + 32 * a + r) * LDW + PK * h + 4 * g];
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            bv[slot][b] = *(const uint4*)&Bs[buf][(128 * wc + 32 * b + r) * LDW + PK * h + 4 * g];
+    };
+    v16f acc[MA][4];
+#pragma unroll
+    for (int a = 0; a < MA; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = v16f{};
+    auto mma = [&](int slot) {
+#pragma unroll
+        for (int dd = 0; dd < 4; dd++) {
+            v8i A[MA], B[4];
+#pragma unroll
+            for (int a = 0; a < MA; a++) A[a] = sg_f4_mask_frag(av[slot][a], dd);
+#pragma unroll
+            for (int b = 0; b < 4; b++) B[b] = sg_f4_mask_frag(bv[slot][b], dd);
+            const int sc = dd == 0 ? sg_f4_scale<0>() : dd == 1 ? sg_f4_scale<1>() : sg_f4_scale<2>();
+#pragma unroll
+            for (int a = 0; a < MA; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[a], B[b], acc[a][b], 4, 4, 0, sc,
+                                                                                 0, sc);
+        }
+    };
+    gload(wb);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t w0 = wb; w0 < we; w0 += PK) {
+        const bool more = w0 + PK < we;
+        if (more) gload(w0 + PK);
+        lread(buf, 0, 0);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            if (g + 1 < G) lread(buf, g + 1, (g + 1) & 1);
+            mma(g & 1);
+        }
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int a = 0; a < MA; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int row = I0 + 32 * MA * wr + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
+                const int col = J0 + 128 * wc + 32 * b + r;
+                const int32_t x = (int32_t)acc[a][b][v];
+                if (x && row < dpad && col < dpad) atomicAdd(&out[(int64_t)row * dpad + col], x);
+            }
+}
+
 // Lower triangle from the upper one: C[i][j] = C[j][i] for i > j.
 template <bool SEGS>
 __global__ void k_pair_mirror(const SgSegment* __restrict__ segs, SgSegment one, int32_t* __restrict__ out_all)
@@ -190,13 +387,24 @@ __global__ __launch_bounds__(256) void k_pair_ok(const int32_t* __restrict__ pai
     if (lane == 0) ok[sg.ok_base + (int64_t)i * sg.dw + w] = v;
 }
 
-// Transaction splits so that about two workgroups per CU (512) are in flight.
-static int32_t pair_splits(int32_t tiles, int64_t words)
+// Transaction splits of `units` tile grids (tiles x segments). One workgroup per CU fits
+// (256 accumulators per lane leave one wave per SIMD), so the grid is sized in whole
+// dispatch rounds of 256 workgroups: a count just above a multiple of 256 would leave a
+// last round of a few workgroups with the chip waiting on them. The fp4 kernel's
+// splits also stay within PF_MAXW words, where its fp32 sums are exact. `cap` > 0 limits
+// the splits (never below what exactness needs).
+static int32_t pair_splits(int64_t units, int64_t words, bool f4, int32_t cap)
 {
-    int64_t splits = (512 + tiles - 1) / tiles;
-    const int64_t steps = (words + PT_K - 1) / PT_K;   // at least one K step per split
-    if (splits > steps) splits = steps > 0 ? steps : 1;
-    return (int32_t)(splits < 1 ? 1 : splits);
+    const int64_t steps = std::max<int64_t>(1, (words + PT_K - 1) / PT_K);   // one K step per split at least
+    const int64_t need = f4 ? std::max<int64_t>(1, (words + PF_MAXW - 1) / PF_MAXW) : 1;
+    int64_t splits = std::max<int64_t>(need, 256 / std::max<int64_t>(units, 1));
+    if (units * splits > 256) {   // more than one round: fill the last one
+        const int64_t rounds = (units * splits + 255) / 256;
+        splits = std::max(splits, rounds * 256 / units);
+    }
+    if (cap > 0) splits = std::min<int64_t>(splits, std::max<int64_t>(cap, need));
+    splits = std::min(splits, steps);
+    return (int32_t)std::max<int64_t>(splits, 1);
 }
 
 // Zero n int32s (16-byte stores, grid-stride): the split-N tiles add into the matrix. A
@@ -219,7 +427,7 @@ hipError_t sg_launch_zero32(int32_t* p, int64_t n, hipStream_t s)
 }
 
 hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
-                               int32_t* pair, hipStream_t s)
+                               int32_t* pair, int32_t kind, int32_t split_cap, hipStream_t s)
 {
     hipError_t e = sg_launch_zero32(pair, (int64_t)dpad * dpad, s);
     if (e != hipSuccess || W <= 0) return e;
@@ -231,27 +439,36 @@ hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, i
     one.pair_base = 0;
     const int T = (dpad + PT_TILE - 1) / PT_TILE;
     const int tiles = T * (T + 1) / 2;
-    const int32_t splits = pair_splits(tiles, W);
-    hipLaunchKernelGGL(k_pair_tile<false>, dim3((unsigned)tiles, (unsigned)splits, 1), dim3(256), 0, s, rows,
-                       (const SgSegment*)nullptr, one, pair, splits);
+    const bool f4 = kind == SG_PAIR_FP4;
+    const int32_t splits = pair_splits(tiles, W, f4, split_cap);
+    const dim3 grid((unsigned)tiles, (unsigned)splits, 1);
+    if (f4)
+        hipLaunchKernelGGL((k_pair_f4<false, PF_K>), grid, dim3(512), 0, s, rows, (const SgSegment*)nullptr, one,
+                           pair, splits);
+    else
+        hipLaunchKernelGGL(k_pair_tile<false>, grid, dim3(256), 0, s, rows, (const SgSegment*)nullptr, one, pair,
+                           splits);
     hipLaunchKernelGGL(k_pair_mirror<false>, dim3(256, 1), dim3(256), 0, s, (const SgSegment*)nullptr, one, pair);
     return hipGetLastError();
 }
 
 // Windows: every segment's matrix in one launch; `pair` must be zeroed.
 hipError_t sg_launch_pair_mfma_segs(const uint64_t* rows, const SgSegment* segs, int32_t nseg,
-                                    int32_t max_dpad, int64_t max_words, int32_t* pair, hipStream_t s)
+                                    int32_t max_dpad, int64_t max_words, int32_t* pair, int32_t kind,
+                                    int32_t split_cap, hipStream_t s)
 {
     if (nseg <= 0) return hipSuccess;
     const int T = (max_dpad + PT_TILE - 1) / PT_TILE;
     const int tiles = T * (T + 1) / 2;
-    int64_t splits = (512 + (int64_t)tiles * nseg - 1) / ((int64_t)tiles * nseg);
-    const int64_t steps = (max_words + PT_K - 1) / PT_K;
-    if (splits > steps) splits = steps;
-    if (splits < 1) splits = 1;
+    const bool f4 = kind == SG_PAIR_FP4;
+    const int32_t splits = pair_splits((int64_t)tiles * nseg, max_words, f4, split_cap);
     SgSegment none{};
-    hipLaunchKernelGGL(k_pair_tile<true>, dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg), dim3(256), 0, s,
-                       rows, segs, none, pair, (int32_t)splits);
+    if (f4)
+        hipLaunchKernelGGL((k_pair_f4<true, PF_K>), dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg),
+                           dim3(512), 0, s, rows, segs, none, pair, (int32_t)splits);
+    else
+        hipLaunchKernelGGL(k_pair_tile<true>, dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg), dim3(256), 0,
+                           s, rows, segs, none, pair, (int32_t)splits);
     hipLaunchKernelGGL(k_pair_mirror<true>, dim3(64, (unsigned)nseg), dim3(256), 0, s, segs, none, pair);
     return hipGetLastError();
 }

@@ -95,15 +95,19 @@ hipError_t sg_launch_gen_zipf(uint64_t* rows, int64_t Wp, int64_t n_txn, int32_t
                               uint64_t seed, int32_t* sup, hipStream_t s);
 hipError_t sg_launch_copy_probe(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
+// Pair-matrix kernels: fp4 (e2m1) MFMA with fp32 sums, or int8 MFMA with int32 sums.
+#define SG_PAIR_FP4 0
+#define SG_PAIR_INT8 1
 hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
-                               int32_t* pair, hipStream_t s);
+                               int32_t* pair, int32_t kind, int32_t split_cap, hipStream_t s);
 // p[0 .. n) = 0 with a kernel (16-byte stores).
 hipError_t sg_launch_zero32(int32_t* p, int64_t n, hipStream_t s);
 hipError_t sg_launch_pair_valu(const uint64_t* rows, int32_t dpad, int64_t Wp, int64_t W,
                                int32_t* pair, hipStream_t s);
 // Batched pair matrices over segments (one launch for all windows).
 hipError_t sg_launch_pair_mfma_segs(const uint64_t* rows, const SgSegment* segs, int32_t nseg,
-                                    int32_t max_dpad, int64_t max_words, int32_t* pair, hipStream_t s);
+                                    int32_t max_dpad, int64_t max_words, int32_t* pair, int32_t kind,
+                                    int32_t split_cap, hipStream_t s);
 hipError_t sg_launch_pair_ok(const int32_t* pair, const SgSegment* segs, int32_t nseg,
                              int32_t max_d, uint64_t* ok, hipStream_t s);
 

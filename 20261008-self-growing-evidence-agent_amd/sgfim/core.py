@@ -305,6 +305,15 @@ class Context:
         self._check(self._L.sgfim_pair_time(self._h, C.byref(ms)), "pair_time")
         return ms.value
 
+    PAIR_FP4 = 0
+    PAIR_INT8 = 1
+
+    def set_pair_kernel(self, kind: int = 0, max_splits: int = 0):
+        """Pair-matrix kernel: PAIR_FP4 (default, e2m1 MFMA with exact fp32 sums) or PAIR_INT8
+        (int8 MFMA), and a cap on its transaction splits (0 = automatic); the matrix does
+        not depend on either."""
+        self._check(self._L.sgfim_set_pair_kernel(self._h, int(kind), int(max_splits)), "set_pair_kernel")
+
     def set_tile(self, chunk_words: int = 0, chunk_range: int = 0):
         """Counting-tile shape of later searches (0 = default); results do not depend on it."""
         self._check(self._L.sgfim_set_tile(self._h, int(chunk_words), int(chunk_range)), "set_tile")

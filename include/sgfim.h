@@ -253,14 +253,26 @@ int sgfim_item_ids(const sgfim_ctx* ctx, int32_t* ids /* n_items */);
 int sgfim_item_supports(sgfim_ctx* ctx, int64_t* out);
 
 /* Pair co-occurrence matrix over all loaded items, out[i*n + j] = |T_i & T_j|
- * (n = n_items, dense order), computed with int8 MFMA on 0/1 tiles. Replaces the
- * first BMMA wave over the single-item roots (kernel_bmma.cu:772-872,1224-1236). */
+ * (n = n_items, dense order), computed with MFMA on 0/1 tiles (fp4 by default, see
+ * sgfim_set_pair_kernel). Replaces the first BMMA wave over the single-item roots
+ * (kernel_bmma.cu:772-872,1224-1236). */
 int sgfim_pair_matrix(sgfim_ctx* ctx, int32_t* out);
 
-/* Device duration in ms of the most recent int8-MFMA pair-matrix computation of the
+/* Device duration in ms of the most recent MFMA pair-matrix computation of the
  * context (HIP events around its kernels; 0 if none yet). No reference counterpart:
  * the measurement of the pair path. */
 int sgfim_pair_time(sgfim_ctx* ctx, double* ms);
+
+/* The pair-matrix kernel of the context: SGFIM_PAIR_FP4 (default; e2m1 MFMA
+ * v_mfma_scale_f32_32x32x64_f8f6f4 on 0/1 nibbles, fp32 partial sums of at most 2^20
+ * transactions per split, so exact) or SGFIM_PAIR_INT8 (v_mfma_i32_32x32x32_i8 on 0/1
+ * bytes, int32 sums). max_splits > 0 caps the transaction splits of the grid (never
+ * below what the fp4 bound needs; 0 = sized to whole dispatch rounds). The matrix does
+ * not depend on either; a change recomputes it on next use. No reference counterpart:
+ * the reference has one BMMA kernel (kernel_bmma.cu:636-768). */
+#define SGFIM_PAIR_FP4 0
+#define SGFIM_PAIR_INT8 1
+int sgfim_set_pair_kernel(sgfim_ctx* ctx, int32_t kind, int32_t max_splits);
 
 /* Same matrix by the VALU AND+popcount kernel (cross-check path). */
 int sgfim_pair_matrix_valu(sgfim_ctx* ctx, int32_t* out);
