@@ -96,6 +96,9 @@ struct SearchState {
     double event_ms = 0.0;           // HIP-event duration (direct launches only)
     double stage_ms[SG_STAGES] = {};   // HIP-event duration of each round stage (direct launches only)
     int64_t count_launches = 0;
+    // rounds queued on the stream and not yet processed (their snapshot slots are
+    // [pipe_head, pipe_head + pipe_n) mod 2); they carry over from one step call to the next
+    int pipe_n = 0, pipe_head = 0;
     SearchBufs* B = nullptr;         // the context's buffers used by this run
     Universe U;                      // the items it searches
     // host sources of the run's asynchronous uploads, alive until the next run
@@ -103,6 +106,8 @@ struct SearchState {
     std::vector<int64_t> vibase;
     std::vector<int32_t> roots;
 };
+
+static void pipe_settle(sgfim_ctx* c);
 
 struct sgfim_ctx {
     int device = 0;
@@ -864,6 +869,7 @@ void sgfim_destroy(sgfim_ctx* c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    pipe_settle(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     data_free(c);
     bufs_free(c);
@@ -958,6 +964,7 @@ int sgfim_load_windows(sgfim_ctx* c, int32_t n_windows, const int64_t* win_off, 
 {
     (void)hipGetLastError();
     if (!c) return SGFIM_E_INVAL;
+    pipe_settle(c);
     if (n_windows < 1 || !win_off || n_txn < 0 || nnz < 0 || (!off && n_txn > 0) || (!items && nnz > 0))
         return fail(c, SGFIM_E_INVAL, "bad CSR or window arguments");
     if (win_off[0] != 0 || win_off[n_windows] != n_txn) return fail(c, SGFIM_E_INVAL, "win_off must span 0..n_txn");
@@ -1046,6 +1053,7 @@ int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_
 {
     (void)hipGetLastError();
     if (!c) return SGFIM_E_INVAL;
+    pipe_settle(c);
     const int32_t S = n_sensors, B = n_bins;
     if (!x || n_rows < 1 || S < 1 || B < 1 || window < 1 || window > 64 || stride < 1 || n_rows < window)
         return fail(c, SGFIM_E_INVAL, "bad sensor matrix arguments (1 <= window <= 64, window <= rows)");
@@ -1172,6 +1180,7 @@ int sgfim_load_sensors_online(sgfim_ctx* c, const double* x, int64_t n_rows, int
 {
     (void)hipGetLastError();
     if (!c) return SGFIM_E_INVAL;
+    pipe_settle(c);
     const int32_t S = n_sensors, B = n_bins;
     if (!x || n_rows < 1 || S < 1 || B < 2 || window < 1 || window > 64 || stride < 1 || n_rows < window)
         return fail(c, SGFIM_E_INVAL, "bad sensor matrix arguments (n_bins >= 2, 1 <= window <= 64, window <= rows)");
@@ -1349,6 +1358,7 @@ int sgfim_load_zipf(sgfim_ctx* c, int64_t n_txn, int32_t n_items, double zipf_s,
 {
     (void)hipGetLastError();
     if (!c) return SGFIM_E_INVAL;
+    pipe_settle(c);
     if (n_txn < 1 || n_txn >= ((int64_t)1 << 31) || n_items < 1 || n_items > 65535 || n_planted > 4096)
         return fail(c, SGFIM_E_INVAL, "bad synthetic set size");
     std::vector<double> q((size_t)n_items), rate((size_t)std::max(n_planted, 1));
@@ -1479,6 +1489,7 @@ static int pair_out(sgfim_ctx* c, int32_t* out, bool valu)
 {
     (void)hipGetLastError();
     if (!c || !out) return SGFIM_E_INVAL;
+    pipe_settle(c);
     if (!c->loaded) return fail(c, SGFIM_E_STATE, "no transactions loaded");
     if (c->nseg != 1) return fail(c, SGFIM_E_STATE, "pair_matrix needs a single loaded transaction set");
     (void)hipSetDevice(c->device);
@@ -1879,6 +1890,71 @@ int sgfim_anytime_begin(sgfim_ctx* c, const sgfim_mine_args* a)
     return rc;
 }
 
+// Queue one round and the copy of its control snapshot behind it.
+static int pipe_push(sgfim_ctx* c, SearchState* S)
+{
+    SearchBufs* B = S->B;
+    const int slot = (S->pipe_head + S->pipe_n) & 1;
+    int64_t* snap = B->h_ctl + (1 + slot) * CTL_SLOTS;
+    int rc;
+    if ((rc = launch_round(c, S))) return rc;
+    hipError_t e = hipMemcpyAsync(snap, B->D.ctl, CTL_SLOTS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(B->snap_ev[slot], c->stream);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round snapshot: %s", hipGetErrorString(e));
+    S->pipe_n++;
+    return SGFIM_OK;
+}
+
+// Wait for the oldest queued round and take in its snapshot; with `drain`, check its
+// errors and move its MFIs to the pending result. Sets S->done when the frontier is empty.
+static int pipe_pop(sgfim_ctx* c, SearchState* S, bool drain)
+{
+    SearchBufs* B = S->B;
+    const int slot = S->pipe_head;
+    S->pipe_head ^= 1;
+    S->pipe_n--;
+    hipError_t e = hipEventSynchronize(B->snap_ev[slot]);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round: %s", hipGetErrorString(e));
+    memcpy(B->h_ctl, B->h_ctl + (1 + slot) * CTL_SLOTS, CTL_SLOTS * sizeof(int64_t));
+    if (B->h_ctl[CTL_BATCH_N] > 0) S->rounds++;   // a round queued after the frontier emptied is not counted
+    S->count_launches++;   // every queued round launches the counting kernel once (an empty one returns at once)
+    if (B->h_ctl[CTL_UNITS] > 0 && B->h_ctl[CTL_BATCH_N] > 0) {
+        // device-side duration of the counting kernel: first workgroup start to last
+        // workgroup end, from the constant-rate wall clock
+        const int64_t t0 = B->h_ctl[CTL_T0], t1 = B->h_ctl[CTL_T1];
+        if (t1 > t0 && c->wall_khz > 0) S->count_ms += (double)(t1 - t0) / (double)c->wall_khz;
+        if (!S->use_graph) {   // one round in flight: its stage events are this round's
+            for (int i = 0; i < SG_STAGES; i++) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, B->ev[i], B->ev[i + 1]) == hipSuccess) S->stage_ms[i] += ms;
+                else (void)hipGetLastError();
+                if (i == 5) S->event_ms += ms;
+            }
+        }
+    }
+    if (!drain) return SGFIM_OK;
+    int rc;
+    if ((rc = check_err(c, S)) || (rc = drain_ring(c, S, B->h_ctl[CTL_RING_POS], now_s(S)))) return rc;
+    if (frontier_size(B->h_ctl) == 0) S->done = true;
+    return SGFIM_OK;
+}
+
+// Process every round still queued (a call that changes what the rounds read, or needs
+// the stream, first lets them finish); their MFIs go to the pending result.
+static void pipe_settle(sgfim_ctx* c)
+{
+    SearchState* S = c ? c->S : nullptr;
+    if (!S) return;
+    while (S->pipe_n > 0) {
+        const int rc = pipe_pop(c, S, S->status == SGFIM_OK);
+        if (rc != SGFIM_OK && S->status == SGFIM_OK) {
+            S->status = rc;
+            S->done = true;
+        }
+    }
+    if (S->active) fill_stats(c, S);
+}
+
 int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
 {
     if (!c) return SGFIM_E_INVAL;
@@ -1886,80 +1962,40 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
     if (!S || !S->active) return fail(c, SGFIM_E_STATE, "no anytime search in progress");
     (void)hipSetDevice(c->device);
     int rc = S->status;
-    if (rc == SGFIM_OK && !S->done && S->cancel) {
-        rc = S->status = fail(c, SGFIM_E_CANCELLED, "search cancelled");
-        S->done = true;
-    }
-    if (rc != SGFIM_OK || S->done) {
+    if (rc != SGFIM_OK || (S->done && S->pipe_n == 0)) {
         fill_stats(c, S);
         if (st) *st = S->stats;
         return rc;
     }
-    SearchBufs* B = S->B;
     // Rounds are pipelined two deep with captured graphs: round r+1 is queued before the
     // host reads round r's control snapshot and drains its MFIs, so the host work and the
-    // launch latency hide behind the device's work. A round queued after the frontier
-    // emptied does nothing. Direct launches (event timing) run one round at a time.
+    // launch latency hide behind the device's work. The queue stays full when the call
+    // returns after max_rounds rounds, so the device keeps working while the caller
+    // handles the MFIs. A round queued after the frontier emptied does nothing. Direct
+    // launches (event timing) keep one round queued.
     const int depth = S->use_graph ? 2 : 1;
-    int64_t launched = 0;
-    int inflight = 0, head = 0;   // snapshots [head, head + inflight) mod 2 are pending
-    bool stop = false;
+    int64_t processed = 0;
+    bool stop = S->done;
     while (true) {
-        while (!stop && inflight < depth && (max_rounds <= 0 || launched < max_rounds)) {
+        while (!stop && S->pipe_n < depth) {
             if (S->cancel) { rc = fail(c, SGFIM_E_CANCELLED, "search cancelled"); stop = true; break; }
             if (S->args.deadline_s > 0 && now_s(S) > S->args.deadline_s) {
                 rc = fail(c, SGFIM_E_TIMEOUT, "deadline of %.3f s passed", S->args.deadline_s);
                 stop = true;
                 break;
             }
-            const int slot = (head + inflight) & 1;
-            int64_t* snap = B->h_ctl + (1 + slot) * CTL_SLOTS;
-            if ((rc = launch_round(c, S))) { stop = true; break; }
-            hipError_t e = hipMemcpyAsync(snap, B->D.ctl, CTL_SLOTS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipEventRecord(B->snap_ev[slot], c->stream);
-            if (e != hipSuccess) { rc = fail(c, SGFIM_E_DEVICE, "round snapshot: %s", hipGetErrorString(e)); stop = true; break; }
-            launched++;
-            inflight++;
+            if ((rc = pipe_push(c, S))) { stop = true; break; }
         }
-        if (inflight == 0) break;
-        // process the oldest round in flight
-        const int slot = head;
-        head ^= 1;
-        inflight--;
-        hipError_t e = hipEventSynchronize(B->snap_ev[slot]);
-        if (e != hipSuccess) {
-            if (rc == SGFIM_OK) rc = fail(c, SGFIM_E_DEVICE, "round: %s", hipGetErrorString(e));
-            stop = true;
-            continue;
-        }
-        memcpy(B->h_ctl, B->h_ctl + (1 + slot) * CTL_SLOTS, CTL_SLOTS * sizeof(int64_t));
-        if (B->h_ctl[CTL_BATCH_N] > 0) S->rounds++;   // a round queued after the frontier emptied is not counted
-        S->count_launches++;   // every queued round launches the counting kernel once (an empty one returns at once)
-        if (B->h_ctl[CTL_UNITS] > 0 && B->h_ctl[CTL_BATCH_N] > 0) {
-            // device-side duration of the counting kernel: first workgroup start to last
-            // workgroup end, from the constant-rate wall clock
-            const int64_t t0 = B->h_ctl[CTL_T0], t1 = B->h_ctl[CTL_T1];
-            if (t1 > t0 && c->wall_khz > 0) S->count_ms += (double)(t1 - t0) / (double)c->wall_khz;
-            if (!S->use_graph) {   // one round in flight: its stage events are this round's
-                for (int i = 0; i < SG_STAGES; i++) {
-                    float ms = 0.0f;
-                    if (hipEventElapsedTime(&ms, B->ev[i], B->ev[i + 1]) == hipSuccess) S->stage_ms[i] += ms;
-                    else (void)hipGetLastError();
-                    if (i == 5) S->event_ms += ms;
-                }
-            }
-        }
-        if (rc != SGFIM_OK) continue;   // an earlier failure: only wait for the rounds in flight
-        int r2;
-        if ((r2 = check_err(c, S)) || (r2 = drain_ring(c, S, B->h_ctl[CTL_RING_POS], now_s(S)))) {
-            rc = r2;
-            stop = true;
-            continue;
-        }
-        if (frontier_size(B->h_ctl) == 0) {
-            S->done = true;
+        if (S->pipe_n == 0) break;
+        if (!stop && max_rounds > 0 && processed >= max_rounds) break;   // queued rounds carry over
+        // process the oldest round queued (after an earlier failure, only wait for it)
+        const int r2 = pipe_pop(c, S, rc == SGFIM_OK);
+        processed++;
+        if (r2 != SGFIM_OK) {
+            if (rc == SGFIM_OK) rc = r2;
             stop = true;
         }
+        if (S->done) stop = true;
     }
     if (rc != SGFIM_OK) {
         S->status = rc;
@@ -2093,6 +2129,7 @@ int sgfim_last_stats(const sgfim_ctx* c, sgfim_round_stats* st)
 int sgfim_set_scorer(sgfim_ctx* c, const float* w1, const float* b1, const float* w2, float b2)
 {
     if (!c || !w1 || !b1 || !w2) return SGFIM_E_INVAL;
+    pipe_settle(c);
     (void)hipSetDevice(c->device);
     c->scorer_set = false;
     auto bf16 = [](float f) -> uint16_t {
@@ -2161,6 +2198,7 @@ int sgfim_copy_probe(sgfim_ctx* c, int64_t bytes, int32_t iters, double* gbps)
 int sgfim_set_trace(sgfim_ctx* c, int32_t enable, int64_t cap)
 {
     if (!c || cap < 0) return SGFIM_E_INVAL;
+    pipe_settle(c);
     c->trace = enable != 0;
     c->trace_cap = enable ? (cap > 0 ? cap : (int64_t)1 << 22) : 0;
     return SGFIM_OK;
@@ -2171,6 +2209,7 @@ int sgfim_trace_get(sgfim_ctx* c, int64_t* n_birth, int64_t* n_mfi, int64_t* key
                     int64_t cap)
 {
     if (!c || !n_birth || !n_mfi) return SGFIM_E_INVAL;
+    pipe_settle(c);
     SearchState* S = c->S;
     if (!S || !S->B || !S->B->D.trace_cap) return fail(c, SGFIM_E_STATE, "no traced run");
     (void)hipSetDevice(c->device);

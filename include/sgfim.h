@@ -312,7 +312,12 @@ int sgfim_mine(sgfim_ctx* ctx, const sgfim_mine_args* args, sgfim_result** out);
 /* Anytime search: begin, advance by rounds, poll, cancel. MFIs are confirmed
  * final when they are reported. Replaces the AnyFIM stage loop
  * (AnyTime kernel.cu:844-915) with a threshold-free, priority-ordered frontier
- * that runs down to args->min_count (the support floor). */
+ * that runs down to args->min_count (the support floor). sgfim_anytime_step
+ * processes up to max_rounds rounds (0: to the end); with captured graphs it returns
+ * with the next rounds already queued, so the device keeps searching while the caller
+ * takes and handles the MFIs. Calls that reload data, change the scorer or the trace,
+ * or read the pair matrix first process the queued rounds (their MFIs become pending);
+ * after a cancel the next step waits for them without reporting their MFIs. */
 int sgfim_anytime_begin(sgfim_ctx* ctx, const sgfim_mine_args* args);
 int sgfim_anytime_step(sgfim_ctx* ctx, int32_t max_rounds, sgfim_round_stats* stats);
 int sgfim_anytime_poll(sgfim_ctx* ctx, sgfim_round_stats* stats);

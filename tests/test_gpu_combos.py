@@ -135,3 +135,38 @@ def test_anytime_rounds_on_a_compact_universe_in_learned_order():
                 break
     assert rounds > 1
     assert got == ref
+
+
+def test_anytime_queue_carries_over_steps_and_settles_for_other_calls():
+    """Captured rounds of 64 records taken one or three at a time: each step returns with
+    the next rounds still queued. Every few steps a pair-matrix read first processes the
+    queued rounds (their MFIs become pending) before it runs; the search then goes on.
+    Every MFI is reported exactly once and the union is the oracle's family."""
+    print("case: workload 4 seed 1000010 at 60,000 txns, 2%, graph anytime, rounds of 64, settle by pair reads",
+          flush=True)
+    T = gen.workload4(1_000_010, 60_000)
+    thr = gen.min_count(0.02, T.n_txn)
+    ref = _oracle(T, thr)
+    rows = cref.bitmap(T.offsets, T.items, T.n_items)
+    pair_ref = cref.pair_matrix(rows)
+    with Context(device=0, batch_nodes=64) as ctx:
+        ctx.load_csr(T.offsets, T.items)
+        ctx.anytime_begin(thr, 0)
+        got = {}
+        steps = 0
+        while True:
+            st = ctx.anytime_step(1 if steps % 2 else 3)
+            r = ctx.anytime_take()
+            for x, s in zip(r.itemsets, r.supports):
+                k = frozenset(x)
+                assert k not in got
+                got[k] = int(s)
+            steps += 1
+            if st["done"]:
+                break
+            if steps % 5 == 0:
+                assert (ctx.pair_matrix() == pair_ref).all()
+        assert steps > 10
+        r = ctx.anytime_take()
+        assert r.count == 0
+    assert got == ref
