@@ -2195,6 +2195,37 @@ int sgfim_copy_probe(sgfim_ctx* c, int64_t bytes, int32_t iters, double* gbps)
     return SGFIM_OK;
 }
 
+int sgfim_mfma_probe(sgfim_ctx* c, int32_t kind, int32_t iters, double* tops)
+{
+    (void)hipGetLastError();
+    if (!c || iters < 1 || !tops) return SGFIM_E_INVAL;
+    if (kind != SGFIM_PAIR_FP4 && kind != SGFIM_PAIR_INT8)
+        return fail(c, SGFIM_E_INVAL, "mfma probe %d: expected SGFIM_PAIR_FP4 or SGFIM_PAIR_INT8", kind);
+    (void)hipSetDevice(c->device);
+    const int32_t blocks = 4 * c->n_cu;   // four workgroups of four waves per CU: four waves per SIMD
+    std::vector<DevBuf> tmp;
+    float* d = nullptr;
+    int rc;
+    if ((rc = dev_alloc_t(c, tmp, &d, (size_t)blocks))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = sg_launch_mfma_probe(d, blocks, iters, kind, c->stream);   // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    if (e == hipSuccess) e = sg_launch_mfma_probe(d, blocks, iters, kind, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free_all(c, tmp);
+    if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "mfma probe: %s", hipGetErrorString(e));
+    const double k = kind == SGFIM_PAIR_FP4 ? 64.0 : 32.0;   // K of one instruction
+    *tops = 2.0 * 32.0 * 32.0 * k * 8.0 * iters * 4.0 * blocks / (ms * 1e-3) / 1e12;
+    return SGFIM_OK;
+}
+
 int sgfim_set_trace(sgfim_ctx* c, int32_t enable, int64_t cap)
 {
     if (!c || cap < 0) return SGFIM_E_INVAL;

@@ -342,6 +342,52 @@ __global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ ro
             }
 }
 
+// ------------------------------------------------------------------ MFMA rate probe
+// Matrix-core throughput with no memory traffic: every wave issues `iters` x 8
+// independent 32 x 32 MFMAs (fp4 v_mfma_scale_f32_32x32x64_f8f6f4 or int8
+// v_mfma_i32_32x32x32_i8) on register operands; one lane per workgroup stores a sum so
+// the work is kept. What the pair kernels are measured against beside the published peak.
+template <bool F4>
+__global__ __launch_bounds__(256) void k_mfma_probe(float* __restrict__ out, int32_t iters, int32_t seed)
+{
+    v8i a, b;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        a[i] = (int)(((uint32_t)seed * (threadIdx.x + i)) & 0x22222222u);
+        b[i] = (int)(((uint32_t)seed ^ (threadIdx.x * 7u + i)) & 0x22222222u);
+    }
+    v16f acc[8];
+    v16i iacc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        acc[j] = v16f{};
+        iacc[j] = v16i{};
+    }
+    const v4i a4{a[0], a[1], a[2], a[3]}, b4{b[0], b[1], b[2], b[3]};
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (F4) acc[j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, acc[j], 4, 4, 0, 127, 0, 127);
+            else iacc[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a4, b4, iacc[j], 0, 0, 0);
+        }
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) sum += F4 ? acc[j][v] : (float)iacc[j][v];
+    if (threadIdx.x == 0) out[blockIdx.x] = sum;
+}
+
+hipError_t sg_launch_mfma_probe(float* out, int32_t blocks, int32_t iters, int32_t kind, hipStream_t s)
+{
+    if (kind == SG_PAIR_FP4)
+        hipLaunchKernelGGL(k_mfma_probe<true>, dim3((unsigned)blocks), dim3(256), 0, s, out, iters, 3);
+    else
+        hipLaunchKernelGGL(k_mfma_probe<false>, dim3((unsigned)blocks), dim3(256), 0, s, out, iters, 3);
+    return hipGetLastError();
+}
+
 // Lower triangle from the upper one: C[i][j] = C[j][i] for i > j.
 template <bool SEGS>
 __global__ void k_pair_mirror(const SgSegment* __restrict__ segs, SgSegment one, int32_t* __restrict__ out_all)

@@ -94,6 +94,8 @@ hipError_t sg_launch_gen_zipf(uint64_t* rows, int64_t Wp, int64_t n_txn, int32_t
                               const uint64_t* r53, const int32_t* item_sets_off, const int32_t* item_sets,
                               uint64_t seed, int32_t* sup, hipStream_t s);
 hipError_t sg_launch_copy_probe(const void* src, void* dst, int64_t bytes, hipStream_t s);
+// MFMA throughput probe: `blocks` workgroups of 4 waves, iters x 8 MFMAs per wave; out [blocks].
+hipError_t sg_launch_mfma_probe(float* out, int32_t blocks, int32_t iters, int32_t kind, hipStream_t s);
 
 // Pair-matrix kernels: fp4 (e2m1) MFMA with fp32 sums, or int8 MFMA with int32 sums.
 #define SG_PAIR_FP4 0

@@ -360,6 +360,12 @@ class Context:
         self._check(self._L.sgfim_device_info(self._h, buf, 64, C.byref(n)), "device_info")
         return buf.value.decode(), n.value
 
+    def mfma_probe(self, kind: int = 0, iters: int = 4000) -> float:
+        """Sustained MFMA rate in TOP/s with no memory traffic (kind PAIR_FP4 or PAIR_INT8)."""
+        g = C.c_double()
+        self._check(self._L.sgfim_mfma_probe(self._h, int(kind), int(iters), C.byref(g)), "mfma_probe")
+        return g.value
+
     def copy_probe(self, nbytes: int = 1 << 31, iters: int = 5) -> float:
         """On-device copy bandwidth in GB/s (bytes read + written)."""
         g = C.c_double()

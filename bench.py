@@ -36,21 +36,22 @@ FP4_PEAK_TFLOPS = 10100.0   # MI355X dense fp4 (and fp6) MFMA
 VALU_PEAK_TLANE = 256 * 4 * 16 * 2.4e9 / 1e12
 SUPPORTS = (0.10, 0.05, 0.02)
 PMC_PROFILE = os.path.join(ROOT, "profiles", "k_count_group_workload2_pmc.md")
+PMC_PROFILE_W5 = os.path.join(ROOT, "profiles", "k_count_group_workload5_pmc.md")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(setting: str, kernel_prefix: str = "k_count_group"):
+def pmc_traffic(setting: str, kernel_prefix: str = "k_count_group", profile: str = PMC_PROFILE):
     """HBM bytes per dispatch of the counting kernel from the committed counter profile: the
     line `pmc: kernel=... setting=... fetch_kib=F write_kib=W fetch_scale=a write_scale=b`
     written from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, mean
     over the run) of the bench at that search setting, scaled by what the same counters read
     for the copy probe of known size. None when no line matches the setting."""
-    if not os.path.exists(PMC_PROFILE):
+    if not os.path.exists(profile):
         return None
-    for ln in open(PMC_PROFILE):
+    for ln in open(profile):
         m = re.match(r"\s*pmc:\s*(.*)$", ln)
         if not m:
             continue
@@ -60,7 +61,7 @@ def pmc_traffic(setting: str, kernel_prefix: str = "k_count_group"):
         b = 1024.0 * (float(kv["fetch_scale"]) * float(kv["fetch_kib"]) +
                       float(kv["write_scale"]) * float(kv["write_kib"]))
         return {"bytes": b, "kernel": kv["kernel"], "setting": setting, "commit": kv.get("commit"),
-                "source": os.path.relpath(PMC_PROFILE, ROOT)}
+                "source": os.path.relpath(profile, ROOT)}
     return None
 
 
@@ -188,6 +189,27 @@ def _t_at(t, frac):
     return float(t[max(0, int(np.ceil(frac * len(t))) - 1)])
 
 
+def workload5_search(Context):
+    """One whole learned-order search of workload 5 to its 0.5% floor (no warm-up run):
+    the run the counter profiles of workload 5 are taken on."""
+    from sgfim import autotune, gen, learned
+    from sgfim._lib import ORDER_LEARNED
+    ctx = Context(device=0, frontier_cap=1 << 23)
+    ctx.load_zipf(seed=0, **gen.WORKLOAD5)
+    ctx.set_scorer(*learned.load_weights())
+    tile = autotune.apply(ctx, 1)
+    thr = gen.min_count(0.005, ctx.n_txn)
+    t0 = time.perf_counter()
+    r = ctx.mine(thr, flags=ORDER_LEARNED, convert=False)
+    dt = time.perf_counter() - t0
+    st = r.stats
+    ctx.close()
+    return {"workload": "workload 5: one learned-order search to the 0.5% floor", "seconds": dt, "mfis": len(r),
+            "counts": st["counts"] + st["pair_counts"], "rounds": st["round"], "count_kernel_ms": st["count_ms"],
+            "count_launches": st["count_launches"], "words": st["words"],
+            "tile": {"chunk_words": tile[0], "chunk_range": tile[1]}}
+
+
 def workload5_run(Context, steps=2):
     """10M txns x 2,048 Zipf items (2.56 GB of bitmaps, 10x the MALL) mined to the 0.5% floor,
     to completion. The learned frontier order (the workload's stated order) is timed as whole
@@ -215,6 +237,17 @@ def workload5_run(Context, steps=2):
     fp_full = fingerprint(*r.arrays)
     counts = st["counts"] + st["pair_counts"]
     sec = float(np.median(ts))
+    # the counting kernel's HBM traffic per launch from the workload's counter profile (same
+    # setting), over this run's own average launch time
+    avg_ms = st["count_ms"] / max(st["count_launches"], 1)
+    traffic = pmc_traffic(setting_name(ctx), profile=PMC_PROFILE_W5)
+    hbm = None
+    if traffic and avg_ms > 0:
+        gbs = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
+        hbm = {"kernel": "k_count_group", "bytes_per_launch": traffic["bytes"], "avg_launch_ms": avg_ms,
+               "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+               "setting": traffic["setting"], "traffic_source": traffic["source"],
+               "measured_at_commit": traffic["commit"]}
     rd = ctx.mine(thr, flags=ORDER_LEARNED | NO_GRAPH, convert=False)
     stages = ctx.stage_times()
     log(f"workload 5 stage times (direct launches): {stages}")
@@ -237,6 +270,7 @@ def workload5_run(Context, steps=2):
             "words_per_count_avg": st["words"] / max(st["counts"], 1),
             "valu_frac_count_kernel": 4.0 * st["words"] / (st["count_ms"] * 1e-3) / 1e12 / VALU_PEAK_TLANE
             if st["count_ms"] > 0 else None,
+            "count_kernel_hbm": hbm,
             "stage_ms_direct_launch": stages, "direct_launch_seconds": rd.stats["elapsed_s"],
             "tile": {"chunk_words": tile[0], "chunk_range": tile[1], "proj_factor": proj,
                      "block_rule": {"dense": rule[0], "min_children": rule[1]}},
@@ -366,7 +400,7 @@ def main():
                     help="skip the workload 3, 4 and 5 measurements")
     ap.add_argument("--sequential", action="store_true",
                     help="mine the three supports as three searches instead of one batched search")
-    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5"), default=None,
+    ap.add_argument("--extra-only", choices=("workload3", "workload4", "workload5", "workload5-search"), default=None,
                     help="run one extra workload alone and print its JSON")
     ap.add_argument("--projection", type=int, default=None,
                     help="projection factor of the headline search (default: the tuned one)")
@@ -384,7 +418,8 @@ def main():
 
     if args.extra_only:
         res = {"workload3": lambda: workload3_run(Context), "workload4": lambda: workload4_run(Context),
-               "workload5": lambda: workload5_run(Context)}[args.extra_only]()
+               "workload5": lambda: workload5_run(Context),
+               "workload5-search": lambda: workload5_search(Context)}[args.extra_only]()
         print(json.dumps(res))
         return
 
@@ -513,6 +548,9 @@ def main():
             if i > 0:
                 ab[name].append(ctx.pair_time())
     pair_ab = {k: sum(v) / len(v) for k, v in ab.items()}
+    # the rate the matrix cores sustain on this device with no memory traffic
+    mfma_fp4 = ctx.mfma_probe(ctx.PAIR_FP4)
+    mfma_int8 = ctx.mfma_probe(ctx.PAIR_INT8)
     full = None
     if proj and args.full_rows_steps > 0:
         # the same search with projection off: every count over the full 15,625-word rows
@@ -558,11 +596,14 @@ def main():
                           "timed": "HIP events around the zero fill, the tile kernel and the mirror pass",
                           "full_square_equivalent_tflops": 2.0 * ctx.n_items ** 2 * T.n_txn / (pair_avg * 1e-3) / 1e12
                           if pair_avg > 0 else None,
+                          "sustained_tflops": mfma_fp4,
+                          "frac_of_sustained": pair_tops / mfma_fp4 if pair_tops and mfma_fp4 else None,
                           "kernels_ab_ms": pair_ab,
                           "int8_kernel": {"kernel": "k_pair_tile", "avg_launch_ms": pair_ab["int8"],
                                           "achieved": pair_ops / (pair_ab["int8"] * 1e-3) / 1e12,
                                           "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                                          "frac": pair_ops / (pair_ab["int8"] * 1e-3) / 1e12 / INT8_PEAK_TOPS}},
+                                          "frac": pair_ops / (pair_ab["int8"] * 1e-3) / 1e12 / INT8_PEAK_TOPS,
+                                          "sustained_tops": mfma_int8}},
     }
     ctx.close()
     if not args.no_extra:

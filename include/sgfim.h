@@ -343,6 +343,13 @@ void sgfim_result_free(sgfim_result* r);
  * calibrating hardware byte counters. No reference counterpart. */
 int sgfim_copy_probe(sgfim_ctx* ctx, int64_t bytes, int32_t iters, double* gbps);
 
+/* Matrix-core throughput of this device in TOP/s (fp4 flops or int8 ops, 2 per
+ * multiply-add): iters x 8 independent 32 x 32 MFMAs per wave on register operands, four
+ * waves per SIMD, no memory traffic, timed with HIP events. kind: SGFIM_PAIR_FP4 or
+ * SGFIM_PAIR_INT8. No reference counterpart: the sustained rate the pair kernels are
+ * compared with beside the published peak. */
+int sgfim_mfma_probe(sgfim_ctx* ctx, int32_t kind, int32_t iters, double* tops);
+
 /* Cumulative search statistics of the last sgfim_mine / anytime run. */
 int sgfim_last_stats(const sgfim_ctx* ctx, sgfim_round_stats* stats);
 

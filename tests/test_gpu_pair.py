@@ -119,3 +119,14 @@ def test_pair_kernel_rejects_unknown_kind(ctx):
         ctx.set_pair_kernel(7, 0)
     with pytest.raises(Exception):
         ctx.set_pair_kernel(Context.PAIR_FP4, -1)
+
+
+def test_mfma_probe_reports_a_plausible_rate(ctx):
+    say("case: MFMA probe fp4 and int8")
+    f4 = ctx.mfma_probe(Context.PAIR_FP4, 200)
+    i8 = ctx.mfma_probe(Context.PAIR_INT8, 200)
+    # dense peaks 10.1 PFLOP/s (fp4) and 5.0 POP/s (int8); the sustained rate is below them
+    assert 500.0 < f4 < 10100.0 * 1.05
+    assert 250.0 < i8 < 5000.0 * 1.05
+    with pytest.raises(Exception):
+        ctx.mfma_probe(7, 10)
