@@ -2043,9 +2043,26 @@ int sgfim_mine(sgfim_ctx* c, const sgfim_mine_args* a, sgfim_result** out)
             sgfim_round_stats st;
             rc = sgfim_anytime_step(c, 0, &st);
         }
-        // with the default frontier capacity, an overflow restarts the search with 4x the room
+        // with the default frontier capacity, an overflow restarts the search with 4x the room,
+        // while the larger frontier fits the device memory (and the context's cap) left
         if (rc == SGFIM_E_CAPACITY && c->opts.frontier_cap == 0 && c->SB && c->SB->D.cap < ((int64_t)1 << 26)) {
-            c->cap_hint = 4 * c->SB->D.cap;
+            const SgSearchDev& D = c->SB->D;
+            // bytes per frontier record: itemset, meta, extension mask, keys, free list and
+            // its slot in each of the 64 priority stacks
+            const int64_t per = (int64_t)(SG_MAX_DEPTH * sizeof(*D.items) + sizeof(*D.meta) + D.dw_max * sizeof(*D.cand) +
+                                          sizeof(*D.key) + sizeof(*D.pkey) + sizeof(*D.free_idx) +
+                                          SG_NBUCKETS * sizeof(*D.bucket));
+            const int64_t cap2 = 4 * D.cap;
+            const int64_t need = (cap2 - D.cap) * per;   // the current frontier's buffers are reused
+            size_t fr = 0, tt = 0;
+            int64_t avail = hipMemGetInfo(&fr, &tt) == hipSuccess ? (int64_t)fr : 0;
+            if (c->mem_cap > 0) avail = std::min(avail, c->mem_cap - c->mem_used);
+            if (need > avail)
+                return fail(c, SGFIM_E_CAPACITY,
+                            "frontier of %lld records is full; %lld records would need %.2f GB more device "
+                            "memory, %.2f GB available (set opts.frontier_cap, or a higher threshold)",
+                            (long long)D.cap, (long long)cap2, need / 1e9, avail / 1e9);
+            c->cap_hint = cap2;
             continue;
         }
         if (rc) return rc;

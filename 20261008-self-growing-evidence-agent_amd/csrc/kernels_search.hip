@@ -221,18 +221,24 @@ __device__ void sg_make_record(const SgSearchDev& S, const SgSegment& sg, const 
     }
 }
 
-// Write MFI X at ring position pos (pinned host memory mapped into the device).
+// Write MFI X at ring position pos (pinned host memory mapped into the device). Called by
+// the whole wave: the items go out as one coalesced store (lane i writes item i) and the
+// four meta fields as one 16-byte store, so an MFI costs two writes across the host link
+// instead of one per item and field.
 __device__ __forceinline__ void sg_emit_at(const SgSearchDev& S, int64_t pos, const uint16_t* px, int k, int32_t sup,
                                            int seg, int64_t parent_key)
 {
+    const int lane = threadIdx.x & 63;
     const int64_t r = pos % S.ring_cap;
     uint16_t* ri = S.ring_items + r * SG_MAX_DEPTH;
-    for (int i = 0; i < k; i++) ri[i] = px[i];
-    int32_t* rm = S.ring_meta + r * 4;
-    rm[0] = k;
-    rm[1] = sup;
-    rm[2] = seg;
-    rm[3] = (int32_t)S.ctl[CTL_ROUND];
+    if (lane < k) ri[lane] = px[lane];
+    if (lane != 0) return;
+    int4 mv;
+    mv.x = k;
+    mv.y = sup;
+    mv.z = seg;
+    mv.w = (int32_t)S.ctl[CTL_ROUND];
+    *(int4*)(S.ring_meta + r * 4) = mv;
     if (S.trace_cap) {
         const int64_t t = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_TRACE_MFI], 1ull);
         if (t < S.trace_cap) S.trace_mfi[t] = parent_key;
@@ -378,7 +384,7 @@ __global__ __launch_bounds__(SG_SEED_WAVES * 64) void k_seed(SgSearchDev S, cons
     const SgSegment sg = segs[seg];
     const int32_t* isup = sup_all + seg_item_base[seg];
     if (act == SG_ACT_EMIT) {
-        if (lane == 0) sg_emit_at(S, s_ring + rank, sX[wv], k, isup[r], seg, -1);
+        sg_emit_at(S, s_ring + rank, sX[wv], k, isup[r], seg, -1);
     } else {
         const int64_t nb = s_nb + rank;
         int32_t slot = -1;
@@ -1663,7 +1669,7 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
             const SgNodeMeta m = S.meta[slot];
             const SgSegment sg = segs[m.seg];
             if (act == SG_ACT_EMIT) {
-                if (lane == 0) sg_emit_at(S, s_ring + rank, sX[wv], k, sup_e, m.seg, S.key[slot]);
+                sg_emit_at(S, s_ring + rank, sX[wv], k, sup_e, m.seg, S.key[slot]);
             } else {
                 const int64_t nbi = s_nb + rank;
                 int32_t nslot = -1;
