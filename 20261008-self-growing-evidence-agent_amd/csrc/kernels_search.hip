@@ -1569,107 +1569,138 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
 
 // ------------------------------------------------------------------ expansion
 #define SG_EXP_WAVES 16
+#define SG_EXP_PER 4   // most children per wave per workgroup step
 
-// Workgroups of 16 waves stride over the batch's children (static assignment: the
-// per-child work is nearly uniform). Child j of record g is Y = X u {e}; its row of the
-// count matrix gives F(Y) and EQ(Y) over U \ {e}; then the node step. MFI ring
-// positions and record slots are reserved once per workgroup step (one atomic per
-// counter for 16 children), and the cumulative counters once per workgroup.
+// Workgroups of 16 waves stride over the batch's children, 16 x per per step, per =
+// ceil(children / (16 x workgroups)) capped at 4: wave w takes children w x per ..
+// w x per + per - 1 of the step one after another (small rounds keep one child per wave
+// and all workgroups busy). Child j of record g is
+// Y = X u {e}; its row of the count matrix gives F(Y) and EQ(Y) over U \ {e}; then the
+// node step. MFI ring positions and record slots are reserved once per workgroup step
+// (one atomic per counter for the step's children), and the cumulative counters once per
+// workgroup. A step waits at its barriers for its slowest wave; four children per wave
+// average that wait out on large rounds (one child per wave left the waves idle about
+// half of the time on 500,000 x 256-item streams, whose children differ widely in
+// candidates).
 __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchDev S, const SgSegment* __restrict__ segs,
                                                                        const int32_t* __restrict__ pair,
                                                                        const uint64_t* __restrict__ ok,
                                                                        const int32_t* __restrict__ sup_all,
                                                                        const int64_t* __restrict__ seg_item_base)
 {
-    __shared__ uint64_t sF[SG_EXP_WAVES][64];
-    __shared__ uint64_t sE[SG_EXP_WAVES][64];
+    constexpr int NSM = SG_EXP_WAVES * SG_EXP_PER;
+    __shared__ uint64_t sF[NSM][64];
+    __shared__ uint64_t sE[NSM][64];
     __shared__ uint64_t sN[SG_EXP_WAVES][64];   // the child's need row (the counting tile's rule)
-    __shared__ uint16_t sX[SG_EXP_WAVES][SG_MAX_DEPTH];
-    __shared__ int32_t s_act[SG_EXP_WAVES];
+    __shared__ uint16_t sX[NSM][SG_MAX_DEPTH];
+    __shared__ int32_t s_act[NSM], s_k[NSM], s_g[NSM], s_e[NSM], s_sup[NSM];
     __shared__ int64_t s_ring, s_nb, s_top, s_key;
     __shared__ unsigned long long s_counts, s_words;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t nchild = S.ctl[CTL_CHILDREN];
+    const int64_t lanes = (int64_t)gridDim.x * SG_EXP_WAVES;
+    const int per = (int)min((int64_t)SG_EXP_PER, max((int64_t)1, (nchild + lanes - 1) / lanes));
+    const int NS = SG_EXP_WAVES * per;   // children per step
     if (threadIdx.x == 0) { s_counts = 0; s_words = 0; }
-    for (int64_t q0 = (int64_t)blockIdx.x * SG_EXP_WAVES; q0 < nchild; q0 += (int64_t)gridDim.x * SG_EXP_WAVES) {
-        const int64_t q = q0 + wv;
-        int act = SG_ACT_NONE, k = 0, g = 0;
-        int32_t e = 0, sup_e = 0;
-        sF[wv][lane] = 0ull;
-        sE[wv][lane] = 0ull;
-        __syncthreads();
-        if (q < nchild) {
-            g = S.chrec[q];
-            const int j = (int)(q - S.choff[g]);
-            const int32_t slot = S.batch[g];
-            const SgNodeMeta m = S.meta[slot];
-            const SgSegment sg = segs[m.seg];
-            const int32_t u0 = S.coff[g];
-            const int nu = m.ncand;
-            e = S.clist[u0 + j];
-            sup_e = S.ccount[S.choff[g] + j];
-            const int32_t* crow = S.cells + S.celloff[g] + (int64_t)j * nu;
-            sN[wv][lane] = lane < sg.dw ? S.need[q * S.dw_max + lane] : 0ull;   // the counting tile's rule
-            if (lane < m.k) sX[wv][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
-            if (lane == 0) sX[wv][m.k] = (uint16_t)e;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            int needed = 0;
-            for (int i = lane; i < nu; i += 64) {
-                if (i == j) continue;
-                const int32_t f = S.clist[u0 + i];
-                // a later child's support with e sits in that child's row (counted once)
-                const bool mirror = i < m.ntail && i > j;
-                const int32_t cnt = mirror ? crow[(int64_t)(i - j) * nu + j] : crow[i];
-                if (!mirror && ((sN[wv][f >> 6] >> (f & 63)) & 1ull)) needed++;
-                if (cnt >= sg.thr) {
-                    atomicOr((unsigned long long*)&sF[wv][f >> 6], 1ull << (f & 63));
-                    if (cnt == sup_e) atomicOr((unsigned long long*)&sE[wv][f >> 6], 1ull << (f & 63));
+    for (int64_t q0 = (int64_t)blockIdx.x * NS; q0 < nchild; q0 += (int64_t)gridDim.x * NS) {
+#pragma unroll 1
+        for (int c = 0; c < per; c++) {
+            const int sl = wv * per + c;
+            const int64_t q = q0 + sl;
+            int act = SG_ACT_NONE, k = 0, g = 0;
+            int32_t e = 0, sup_e = 0;
+            sF[sl][lane] = 0ull;
+            sE[sl][lane] = 0ull;
+            if (q < nchild) {   // wave-uniform
+                g = S.chrec[q];
+                const int j = (int)(q - S.choff[g]);
+                const int32_t slot = S.batch[g];
+                const SgNodeMeta m = S.meta[slot];
+                const SgSegment sg = segs[m.seg];
+                const int32_t u0 = S.coff[g];
+                const int nu = m.ncand;
+                e = S.clist[u0 + j];
+                sup_e = S.ccount[S.choff[g] + j];
+                const int32_t* crow = S.cells + S.celloff[g] + (int64_t)j * nu;
+                sN[wv][lane] = lane < sg.dw ? S.need[q * S.dw_max + lane] : 0ull;   // the counting tile's rule
+                if (lane < m.k) sX[sl][lane] = S.items[(int64_t)slot * SG_MAX_DEPTH + lane];
+                if (lane == 0) sX[sl][m.k] = (uint16_t)e;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                int needed = 0;
+                for (int i = lane; i < nu; i += 64) {
+                    if (i == j) continue;
+                    const int32_t f = S.clist[u0 + i];
+                    // a later child's support with e sits in that child's row (counted once)
+                    const bool mirror = i < m.ntail && i > j;
+                    const int32_t cnt = mirror ? crow[(int64_t)(i - j) * nu + j] : crow[i];
+                    if (!mirror && ((sN[wv][f >> 6] >> (f & 63)) & 1ull)) needed++;
+                    if (cnt >= sg.thr) {
+                        atomicOr((unsigned long long*)&sF[sl][f >> 6], 1ull << (f & 63));
+                        if (cnt == sup_e) atomicOr((unsigned long long*)&sE[sl][f >> 6], 1ull << (f & 63));
+                    }
                 }
+                for (int o = 32; o > 0; o >>= 1) needed += __shfl_xor(needed, o, 64);
+                if (lane == 0) {
+                    // words per count: the row length of the view the record counted on
+                    const int64_t view = S.bview[g];
+                    const int64_t vw = view >= 0 ? S.pdesc[view % S.pdesc_cap].words : sg.words;
+                    atomicAdd(&s_counts, (unsigned long long)needed);
+                    atomicAdd(&s_words, (unsigned long long)(needed + 1) * (unsigned long long)vw);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // F(X u {e}) is child e's row of X's sibling block (before the node step takes
+                // the equal-support items out of it)
+                const int64_t sb = S.sib_off[g];
+                if (sb >= 0 && lane < S.dw_max) S.sib[(sb + (int64_t)(1 + j) * S.dw_max + lane) % S.sib_cap] = sF[sl][lane];
+                k = m.k + 1;
+                act = sg_node_decide(S, sg, sF[sl], sE[sl], sX[sl], k, e);
+                // the next child reuses sN[wv]
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            for (int o = 32; o > 0; o >>= 1) needed += __shfl_xor(needed, o, 64);
             if (lane == 0) {
-                // words per count: the row length of the view the record counted on
-                const int64_t view = S.bview[g];
-                const int64_t vw = view >= 0 ? S.pdesc[view % S.pdesc_cap].words : sg.words;
-                atomicAdd(&s_counts, (unsigned long long)needed);
-                atomicAdd(&s_words, (unsigned long long)(needed + 1) * (unsigned long long)vw);
+                s_act[sl] = act;
+                s_k[sl] = k;
+                s_g[sl] = g;
+                s_e[sl] = e;
+                s_sup[sl] = sup_e;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // F(X u {e}) is child e's row of X's sibling block (before the node step takes
-            // the equal-support items out of it)
-            const int64_t sb = S.sib_off[g];
-            if (sb >= 0 && lane < S.dw_max) S.sib[(sb + (int64_t)(1 + j) * S.dw_max + lane) % S.sib_cap] = sF[wv][lane];
-            k = m.k + 1;
-            act = sg_node_decide(S, sg, sF[wv], sE[wv], sX[wv], k, e);
         }
-        if (lane == 0) s_act[wv] = act;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 4) {   // the four reservations in flight together
             int ne = 0, nr = 0;
-            for (int w = 0; w < SG_EXP_WAVES; w++) {
+            for (int w = 0; w < NS; w++) {
                 ne += s_act[w] == SG_ACT_EMIT;
                 nr += s_act[w] == SG_ACT_RECORD;
             }
-            if (ne) s_ring = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], (unsigned long long)ne);
-            if (nr) {
+            if (threadIdx.x == 0 && ne)
+                s_ring = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_RING_POS], (unsigned long long)ne);
+            if (threadIdx.x == 1 && nr)
                 s_nb = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_NEWBORN], (unsigned long long)nr);
+            if (threadIdx.x == 2 && nr)
                 s_top = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_FREE_TOP], (unsigned long long)(-(int64_t)nr));
+            if (threadIdx.x == 3 && nr)
                 s_key = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_KEY], (unsigned long long)nr);
-            }
         }
         __syncthreads();
-        if (act != SG_ACT_NONE) {   // wave-uniform
+#pragma unroll 1
+        for (int c = 0; c < per; c++) {
+            const int sl = wv * per + c;
+            const int act = s_act[sl];
+            if (act == SG_ACT_NONE) continue;   // wave-uniform
+            const int g = s_g[sl], k = s_k[sl];
             int rank = 0;
-            for (int w = 0; w < wv; w++) rank += s_act[w] == act;
+            for (int w = 0; w < sl; w++) rank += s_act[w] == act;
             const int32_t slot = S.batch[g];
             const SgNodeMeta m = S.meta[slot];
             const SgSegment sg = segs[m.seg];
             if (act == SG_ACT_EMIT) {
-                sg_emit_at(S, s_ring + rank, sX[wv], k, sup_e, m.seg, S.key[slot]);
+                sg_emit_at(S, s_ring + rank, sX[sl], k, s_sup[sl], m.seg, S.key[slot]);
             } else {
                 const int64_t nbi = s_nb + rank;
                 int32_t nslot = -1;
@@ -1682,8 +1713,8 @@ __global__ __launch_bounds__(SG_EXP_WAVES * 64) void k_expand_children(SgSearchD
                     // projected at the parent this round)
                     const int64_t view = S.bview[g];
                     const int32_t pk = view >= 0 ? (S.bpnew[g] ? m.k : m.pk) : 0;
-                    sg_make_record(S, sg, sup_all + seg_item_base[m.seg], pair, nslot, sX[wv], k, sup_e, e, m.seg,
-                                   sF[wv], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot],
+                    sg_make_record(S, sg, sup_all + seg_item_base[m.seg], pair, nslot, sX[sl], k, s_sup[sl], s_e[sl],
+                                   m.seg, sF[sl], m.sup, m.ncand, m.ntail, m.k == 1, fo, s_key + rank, S.key[slot],
                                    S.sib_off[g], view, pk);
                     if (lane == 0) S.newborn[nbi] = nslot;
                 }
