@@ -1058,6 +1058,7 @@ int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_
     if (!x || n_rows < 1 || S < 1 || B < 1 || window < 1 || window > 64 || stride < 1 || n_rows < window)
         return fail(c, SGFIM_E_INVAL, "bad sensor matrix arguments (1 <= window <= 64, window <= rows)");
     if ((int64_t)S * B > 65535) return fail(c, SGFIM_E_INVAL, "sensors x bins must be <= 65535");
+    if (n_rows > (int64_t)UINT32_MAX) return fail(c, SGFIM_E_INVAL, "at most 2^32 - 1 sensor rows");
     const int64_t n_win = (n_rows - window) / stride + 1;
     if (n_win >= ((int64_t)1 << 31)) return fail(c, SGFIM_E_INVAL, "too many windows");
     for (int64_t i = 0; i < n_rows * S; i++)
@@ -1066,18 +1067,49 @@ int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_
     HIPCHK(c, hipStreamSynchronize(c->stream));
     data_free(c);
     const int64_t nv = n_rows * S;
+    // order statistics for the quantiles: positions floor(v) and floor(v) + 1 of
+    // v = (n - 1) q for q = i / B (the last one 1), the same in every column; the select
+    // takes them once each, in ascending order
+    const int32_t J = 2 * (B + 1);
+    std::vector<int64_t> pos((size_t)J);
+    {
+        const double step = 1.0 / (double)B;
+        for (int32_t i = 0; i <= B; i++) {
+            const double q = i == B ? 1.0 : (double)i * step;
+            const double v = (double)(n_rows - 1) * q;
+            int64_t lo, hi;
+            if (v >= (double)(n_rows - 1)) lo = hi = n_rows - 1;
+            else { lo = (int64_t)std::floor(v); hi = lo + 1; }
+            pos[(size_t)i * 2] = lo;
+            pos[(size_t)i * 2 + 1] = hi;
+        }
+    }
+    std::vector<int64_t> uranks(pos);
+    std::sort(uranks.begin(), uranks.end());
+    uranks.erase(std::unique(uranks.begin(), uranks.end()), uranks.end());
+    const int32_t R = (int32_t)uranks.size();
+    std::vector<int32_t> umap((size_t)J);
+    for (int32_t j = 0; j < J; j++)
+        umap[(size_t)j] = (int32_t)(std::lower_bound(uranks.begin(), uranks.end(), pos[(size_t)j]) - uranks.begin());
+    std::vector<int64_t> rank0((size_t)S * R);
+    for (int32_t s = 0; s < S; s++) std::copy(uranks.begin(), uranks.end(), rank0.begin() + (size_t)s * R);
+    const std::vector<int32_t> ngrp0((size_t)S, 1);
     std::vector<DevBuf> tmp;
-    double *d_x = nullptr, *d_xc = nullptr, *d_xs = nullptr, *d_val = nullptr, *d_edges = nullptr;
-    int64_t *d_beg = nullptr, *d_end = nullptr, *d_pos = nullptr, *d_off = nullptr;
-    int32_t *d_ne = nullptr, *d_items = nullptr;
-    uint32_t* d_present = nullptr;
-    const int64_t m = (int64_t)S * (B + 1) * 2;
+    double *d_x = nullptr, *d_xc = nullptr, *d_val = nullptr, *d_edges = nullptr;
+    int64_t *d_rank = nullptr, *d_off = nullptr;
+    uint64_t *d_pref = nullptr, *d_gpref = nullptr;
+    int32_t *d_grp = nullptr, *d_ngrp = nullptr, *d_umap = nullptr, *d_ne = nullptr, *d_items = nullptr;
+    uint32_t *d_hist = nullptr, *d_present = nullptr;
+    const int64_t m = (int64_t)S * J;
+    const size_t sr = (size_t)S * R;
     int rc;
     if ((rc = dev_alloc_t(c, tmp, &d_x, (size_t)nv)) || (rc = dev_alloc_t(c, tmp, &d_xc, (size_t)nv)) ||
-        (rc = dev_alloc_t(c, tmp, &d_xs, (size_t)nv)) || (rc = dev_alloc_t(c, tmp, &d_beg, (size_t)S)) ||
-        (rc = dev_alloc_t(c, tmp, &d_end, (size_t)S)) || (rc = dev_alloc_t(c, tmp, &d_pos, (size_t)m)) ||
-        (rc = dev_alloc_t(c, tmp, &d_val, (size_t)m)) || (rc = dev_alloc_t(c, tmp, &d_edges, (size_t)S * (B + 1))) ||
-        (rc = dev_alloc_t(c, tmp, &d_ne, (size_t)S)) || (rc = dev_alloc_t(c, tmp, &d_items, (size_t)(n_win * S))) ||
+        (rc = dev_alloc_t(c, tmp, &d_rank, sr)) || (rc = dev_alloc_t(c, tmp, &d_pref, sr)) ||
+        (rc = dev_alloc_t(c, tmp, &d_grp, sr)) || (rc = dev_alloc_t(c, tmp, &d_gpref, sr)) ||
+        (rc = dev_alloc_t(c, tmp, &d_ngrp, (size_t)S)) || (rc = dev_alloc_t(c, tmp, &d_hist, sr * 256)) ||
+        (rc = dev_alloc_t(c, tmp, &d_umap, (size_t)J)) || (rc = dev_alloc_t(c, tmp, &d_val, (size_t)m)) ||
+        (rc = dev_alloc_t(c, tmp, &d_edges, (size_t)S * (B + 1))) || (rc = dev_alloc_t(c, tmp, &d_ne, (size_t)S)) ||
+        (rc = dev_alloc_t(c, tmp, &d_items, (size_t)(n_win * S))) ||
         (rc = dev_alloc_t(c, tmp, &d_present, (size_t)S * B)) || (rc = dev_alloc_t(c, tmp, &d_off, (size_t)n_win + 1))) {
         dev_free_all(c, tmp);
         return rc;
@@ -1086,39 +1118,19 @@ int sgfim_load_sensors(sgfim_ctx* c, const double* x, int64_t n_rows, int32_t n_
         dev_free_all(c, tmp);
         return fail(c, SGFIM_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
     };
-    // columns, sorted per sensor
-    std::vector<int64_t> beg(S), end(S);
-    for (int32_t s = 0; s < S; s++) { beg[s] = (int64_t)s * n_rows; end[s] = beg[s] + n_rows; }
+    // columns, then their order statistics
     hipError_t e = hipMemcpyAsync(d_x, x, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_beg, beg.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_end, end.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_rank, rank0.data(), sr * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ngrp, ngrp0.data(), (size_t)S * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_umap, umap.data(), (size_t)J * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_pref, 0, sr * 8, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_grp, 0, sr * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_gpref, 0, sr * 8, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_hist, 0, sr * 256 * 4, c->stream);
     if (e == hipSuccess) e = sg_sensor_columns(d_x, n_rows, S, d_xc, c->stream);
-    size_t temp_bytes = 0;
-    if (e == hipSuccess) e = sg_sensor_sort(nullptr, &temp_bytes, d_xc, d_xs, n_rows, S, d_beg, d_end, c->stream);
-    if (e != hipSuccess) return bail("sensor columns", e);
-    void* d_temp = nullptr;
-    if ((rc = dev_alloc(c, tmp, &d_temp, std::max<size_t>(temp_bytes, 16)))) {
-        dev_free_all(c, tmp);
-        return rc;
-    }
-    e = sg_sensor_sort(d_temp, &temp_bytes, d_xc, d_xs, n_rows, S, d_beg, d_end, c->stream);
-    // order statistics for the quantiles: floor(v) and floor(v) + 1 of v = (n - 1) q
-    std::vector<int64_t> pos((size_t)m);
-    {
-        const double step = 1.0 / (double)B;
-        for (int32_t s = 0; s < S; s++)
-            for (int32_t i = 0; i <= B; i++) {
-                const double q = i == B ? 1.0 : (double)i * step;
-                const double v = (double)(n_rows - 1) * q;
-                int64_t lo, hi;
-                if (v >= (double)(n_rows - 1)) lo = hi = n_rows - 1;
-                else { lo = (int64_t)std::floor(v); hi = lo + 1; }
-                pos[(size_t)(s * (B + 1) + i) * 2] = (int64_t)s * n_rows + lo;
-                pos[(size_t)(s * (B + 1) + i) * 2 + 1] = (int64_t)s * n_rows + hi;
-            }
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(d_pos, pos.data(), (size_t)m * 8, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = sg_sensor_gather(d_xs, d_pos, m, d_val, c->stream);
+    if (e == hipSuccess)
+        e = sg_sensor_select(d_xc, n_rows, S, R, d_rank, d_pref, d_grp, d_ngrp, d_gpref, d_hist, d_umap, J, d_val,
+                             c->stream);
     std::vector<double> val((size_t)m);
     if (e == hipSuccess) e = hipMemcpyAsync(val.data(), d_val, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

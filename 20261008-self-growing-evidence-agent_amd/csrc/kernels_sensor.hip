@@ -13,12 +13,13 @@
 //                              even window).
 // The online variant (k_online_p2, k_online_bins) bins each window median against
 // streaming P^2 estimates instead (code/pipeline/online_discretize.py:22-131).
-// The item of sensor s in bin b is s * B + b + 1. The column sort runs on the device
-// (hipCUB segmented radix sort); the host turns the gathered order statistics into the
-// edges with NumPy's interpolation arithmetic, so the bins are the reference's bit for bit.
+// The item of sensor s in bin b is s * B + b + 1. The quantiles need two order statistics
+// each (positions floor((n - 1) q) and the one after); a radix select on the device finds
+// them without sorting the columns, and the host turns them into the edges with NumPy's
+// interpolation arithmetic, so the bins are the reference's bit for bit.
 #include "sgfim_impl.h"
 
-#include <hipcub/hipcub.hpp>
+#include <algorithm>
 
 #define SG_MAX_WINDOW 64
 
@@ -31,13 +32,6 @@ __global__ void k_sensor_columns(const double* __restrict__ x, int64_t n, int32_
         const int32_t s = (int32_t)(i - t * S);
         xc[(int64_t)s * n + t] = x[i];
     }
-}
-
-__global__ void k_sensor_gather(const double* __restrict__ xs, const int64_t* __restrict__ pos, int64_t m,
-                                double* __restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) out[i] = xs[pos[i]];
 }
 
 // Median of the window col[0], col[S], ..., col[(window - 1) S] (insertion sort; the mean
@@ -257,18 +251,153 @@ hipError_t sg_sensor_columns(const double* x, int64_t n, int32_t S, double* xc, 
     return hipGetLastError();
 }
 
-// Sort each column (S segments of n doubles); temp/temp_bytes as hipCUB's two-phase call.
-hipError_t sg_sensor_sort(void* temp, size_t* temp_bytes, const double* xc, double* xs, int64_t n, int32_t S,
-                          const int64_t* seg_begin, const int64_t* seg_end, hipStream_t s)
+// ------------------------------------------------------------------ order statistics
+// Radix select of R ranks (ascending, the same for every column) in each of S columns of n
+// doubles, 8 key bits per pass. Keys order the doubles as unsigned integers (sign bit set
+// for non-negative values, all bits inverted for negative ones), the order a radix sort of
+// the column gives. Before pass p every target holds the key's top 8p bits it has decided
+// (its prefix) and its rank among the column's values with that prefix. Targets sorted by
+// rank have ascending prefixes, so equal prefixes form runs, the column's groups; groups
+// are disjoint key ranges, so a value falls in at most one of them. Pass p counts the next
+// 8 bits of the values in each group; each target then takes the digit its rank falls in.
+// After 8 passes the prefix is the whole key of the target's order statistic.
+#define SG_SEL_LDS_GROUPS 64
+
+__device__ __forceinline__ uint64_t sg_dkey(double v)
 {
-    return hipcub::DeviceSegmentedRadixSort::SortKeys(temp, *temp_bytes, xc, xs, n * S, S, seg_begin, seg_end, 0,
-                                                      (int)(sizeof(double) * 8), s);
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-hipError_t sg_sensor_gather(const double* xs, const int64_t* pos, int64_t m, double* out, hipStream_t s)
+__device__ __forceinline__ double sg_dval(uint64_t k)
 {
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sensor_gather, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, xs, pos, m, out);
+    const uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+// The group of prefix hp among the column's ng ascending group prefixes P, or -1.
+__device__ __forceinline__ int sg_sel_group(const uint64_t* P, int ng, uint64_t hp)
+{
+    int lo = 0, hi = ng;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (P[mid] < hp) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < ng && P[lo] == hp) ? lo : -1;
+}
+
+// Histograms of pass `pass`: hist[s][g][256] += values of column s in group g by digit.
+// Grid (blocks per column, S); 256 threads.
+__global__ __launch_bounds__(256) void k_sel_hist(const double* __restrict__ xc, int64_t n, int32_t pass, int32_t R,
+                                                  const int32_t* __restrict__ ngrp, const uint64_t* __restrict__ gpref,
+                                                  uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t lh[SG_SEL_LDS_GROUPS * 256];
+    __shared__ uint64_t lp[SG_SEL_LDS_GROUPS];
+    const int32_t s = blockIdx.y;
+    const int ng = ngrp[s];
+    const uint64_t* gp = gpref + (int64_t)s * R;
+    uint32_t* gh = hist + (int64_t)s * R * 256;
+    const bool local = ng <= SG_SEL_LDS_GROUPS;   // block-uniform
+    if (local) {
+        for (int i = threadIdx.x; i < ng * 256; i += 256) lh[i] = 0u;
+        for (int i = threadIdx.x; i < ng; i += 256) lp[i] = gp[i];
+    }
+    __syncthreads();
+    const int ds = 56 - 8 * pass;          // shift of this pass's digit
+    const double* col = xc + (int64_t)s * n;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const uint64_t key = sg_dkey(col[t]);
+        int g = 0;
+        if (pass > 0) {
+            const uint64_t hp = key >> (ds + 8);
+            g = local ? sg_sel_group(lp, ng, hp) : sg_sel_group(gp, ng, hp);
+            if (g < 0) continue;
+        }
+        const uint32_t d = (uint32_t)(key >> ds) & 255u;
+        if (local) atomicAdd(&lh[g * 256 + d], 1u);
+        else atomicAdd(&gh[(int64_t)g * 256 + d], 1u);
+    }
+    if (local) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < ng * 256; i += 256)
+            if (lh[i]) atomicAdd(&gh[i], lh[i]);
+    }
+}
+
+// After pass `pass`: each target of column s (one workgroup per column) takes the digit its
+// rank falls in, the histograms read are cleared, and the groups are rebuilt as runs of
+// equal prefixes.
+__global__ __launch_bounds__(256) void k_sel_refine(int32_t R, int64_t* __restrict__ rank, uint64_t* __restrict__ pref,
+                                                    int32_t* __restrict__ grp, int32_t* __restrict__ ngrp,
+                                                    uint64_t* __restrict__ gpref, uint32_t* __restrict__ hist)
+{
+    __shared__ int32_t sc[256];
+    __shared__ int32_t s_base;
+    const int32_t s = blockIdx.x;
+    int64_t* rk = rank + (int64_t)s * R;
+    uint64_t* pf = pref + (int64_t)s * R;
+    int32_t* gr = grp + (int64_t)s * R;
+    uint64_t* gp = gpref + (int64_t)s * R;
+    uint32_t* gh = hist + (int64_t)s * R * 256;
+    const int ng = ngrp[s];
+    for (int r = threadIdx.x; r < R; r += 256) {
+        const uint32_t* h = gh + (int64_t)gr[r] * 256;
+        int64_t k = rk[r];
+        uint32_t d = 0;
+        while (d < 255u && k >= (int64_t)h[d]) k -= h[d++];
+        rk[r] = k;
+        pf[r] = (pf[r] << 8) | d;
+    }
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < (int64_t)ng * 256; i += 256) gh[i] = 0u;
+    for (int r0 = 0; r0 < R; r0 += 256) {
+        const int r = r0 + threadIdx.x;
+        const int f = (r < R && (r == 0 || pf[r] != pf[r - 1])) ? 1 : 0;
+        sc[threadIdx.x] = f;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {   // inclusive scan of the run starts
+            const int v = threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (r < R) {
+            const int g = s_base + sc[threadIdx.x] - 1;
+            gr[r] = g;
+            if (f) gp[g] = pf[r];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_base += sc[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ngrp[s] = s_base;
+}
+
+// out[s * J + j] = the value of target umap[j] of column s.
+__global__ void k_sel_out(const uint64_t* __restrict__ pref, int32_t R, const int32_t* __restrict__ umap, int32_t J,
+                          int32_t S, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)J * S) return;
+    const int64_t s = i / J;
+    out[i] = sg_dval(pref[s * R + umap[i - s * J]]);
+}
+
+hipError_t sg_sensor_select(const double* xc, int64_t n, int32_t S, int32_t R, int64_t* rank, uint64_t* pref,
+                            int32_t* grp, int32_t* ngrp, uint64_t* gpref, uint32_t* hist, const int32_t* umap,
+                            int32_t J, double* out, hipStream_t s)
+{
+    const int64_t per_col = std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, std::max<int64_t>(1, 4096 / S)));
+    for (int32_t pass = 0; pass < 8; pass++) {
+        hipLaunchKernelGGL(k_sel_hist, dim3((unsigned)per_col, (unsigned)S), dim3(256), 0, s, xc, n, pass, R, ngrp, gpref,
+                           hist);
+        hipLaunchKernelGGL(k_sel_refine, dim3((unsigned)S), dim3(256), 0, s, R, rank, pref, grp, ngrp, gpref, hist);
+    }
+    const int64_t m = (int64_t)J * S;
+    hipLaunchKernelGGL(k_sel_out, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, pref, R, umap, J, S, out);
     return hipGetLastError();
 }
 

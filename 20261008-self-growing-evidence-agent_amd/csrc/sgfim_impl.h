@@ -272,9 +272,12 @@ enum SgCtl {
 struct SgScorer;  // proposal network weights on the device (kernels_mlp.hip)
 
 hipError_t sg_sensor_columns(const double* x, int64_t n, int32_t S, double* xc, hipStream_t s);
-hipError_t sg_sensor_sort(void* temp, size_t* temp_bytes, const double* xc, double* xs, int64_t n, int32_t S,
-                          const int64_t* seg_begin, const int64_t* seg_end, hipStream_t s);
-hipError_t sg_sensor_gather(const double* xs, const int64_t* pos, int64_t m, double* out, hipStream_t s);
+// Order statistics of each column of xc[S][n]: R ascending ranks shared by every column
+// (rank[S][R] initialised to them, pref/grp/gpref zero, ngrp 1, hist zero); out[s][j] is the
+// value at rank umap[j] of column s.
+hipError_t sg_sensor_select(const double* xc, int64_t n, int32_t S, int32_t R, int64_t* rank, uint64_t* pref,
+                            int32_t* grp, int32_t* ngrp, uint64_t* gpref, uint32_t* hist, const int32_t* umap,
+                            int32_t J, double* out, hipStream_t s);
 hipError_t sg_sensor_bins(const double* x, int32_t S, int32_t B, int32_t window, int32_t stride, int64_t n_win,
                           const double* edges, const int32_t* ne, int32_t* items, uint32_t* present, int64_t* off,
                           hipStream_t s);

@@ -24,7 +24,13 @@ def _matrix(seed, n, S):
 
 
 @pytest.mark.parametrize("seed,n,S,B,window,stride", [(0, 5000, 6, 5, 30, 15), (1, 4099, 5, 3, 1, 1),
-                                                      (2, 3000, 8, 8, 7, 3), (3, 640, 4, 6, 64, 64)])
+                                                      (2, 3000, 8, 8, 7, 3), (3, 640, 4, 6, 64, 64),
+                                                      # one window per row bins every value, so
+                                                      # each order statistic behind an edge shows:
+                                                      # 302 ranks (more key groups than LDS holds),
+                                                      # many workgroups per column, a single row
+                                                      (4, 20000, 4, 150, 1, 1), (5, 300000, 4, 8, 1, 1),
+                                                      (6, 1, 4, 4, 1, 1)])
 def test_sensor_front_end_matches_numpy(seed, n, S, B, window, stride):
     print(f"case: sensors seed={seed} n={n} S={S} B={B} window={window} stride={stride}", flush=True)
     x = _matrix(seed, n, S)
