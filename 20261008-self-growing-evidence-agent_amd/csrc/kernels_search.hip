@@ -115,10 +115,12 @@ __device__ __forceinline__ uint64_t sg_low_mask(int w, int last)
 #define SG_RC 4       // candidates held in registers per wave step
 
 // Row-chunk ranges (units of `range` chunks of `chunk` words) of a window of `words` words.
+// chunk is 128, 256 or 512 and rows hold fewer than 2^25 words, so the chunk count fits a
+// 32-bit division.
 __device__ __forceinline__ int32_t sg_ranges(int64_t words, int32_t chunk, int32_t range)
 {
-    const int64_t nchunk = (words + chunk - 1) / chunk;
-    return (int32_t)((nchunk + range - 1) / range);
+    const uint32_t nchunk = (uint32_t)((words + chunk - 1) >> __builtin_ctz((uint32_t)chunk));
+    return (int32_t)((nchunk + (uint32_t)range - 1u) / (uint32_t)range);
 }
 
 __device__ __forceinline__ int32_t sg_wave_imin(int32_t v)
@@ -405,45 +407,56 @@ __global__ void k_zero_newborn(SgSearchDev S)
 }
 
 // ------------------------------------------------------------------ select
-// Block-wide exclusive scans of N int64s per thread at once (SEL_THREADS threads): one
-// pass of wave scans, wave totals scanned by wave 0, two barriers for all N.
+// Block-wide exclusive scans of N int64s per thread at once (SEL_THREADS threads, N no
+// more than the block's waves). The values go through LDS transposed: wave n scans
+// sequence n, each lane 16 consecutive entries (a serial prefix in registers, then one
+// wave scan of the lane totals), so the block pays one 6-step wave scan per sequence
+// instead of N per wave. sc holds N x (SEL_THREADS + SEL_THREADS / 16) entries (one pad
+// per 16 spreads a lane's run over the banks); total[n] receives sequence n's sum.
+#define SEL_SCAN_STRIDE (SEL_THREADS + SEL_THREADS / 16)
 template <int N>
-__device__ __forceinline__ void sg_block_excl_scan(int64_t (&v)[N], int64_t (*wsum)[SEL_THREADS / 64], int64_t* total)
+__device__ __forceinline__ void sg_block_excl_scan(int64_t (&v)[N], int64_t* sc, int64_t* total)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int64_t x[N];
+    static_assert(N <= SEL_THREADS / 64 && SEL_THREADS == 1024, "one wave per sequence, 16 entries per lane");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int at = tid + (tid >> 4);
 #pragma unroll
-    for (int n = 0; n < N; n++) x[n] = v[n];
-    for (int o = 1; o < 64; o <<= 1) {
+    for (int n = 0; n < N; n++) sc[n * SEL_SCAN_STRIDE + at] = v[n];
+    __syncthreads();
+    if (wv < N) {
+        int64_t* q = sc + wv * SEL_SCAN_STRIDE + 17 * lane;   // entries 16 lane .. 16 lane + 15
+        int64_t x[16];
+        int64_t run = 0;
 #pragma unroll
-        for (int n = 0; n < N; n++) {
-            const int64_t y = __shfl_up(x[n], o, 64);
-            if (lane >= o) x[n] += y;
+        for (int j = 0; j < 16; j++) {
+            x[j] = run;
+            run += q[j];
         }
-    }
-    if (lane == 63)
+        int64_t incl = run;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const int64_t base = incl - run;
 #pragma unroll
-        for (int n = 0; n < N; n++) wsum[n][wv] = x[n];
-    __syncthreads();
-    if (threadIdx.x < N) {   // one thread per scan runs over the SEL_THREADS / 64 wave totals
-        int64_t a = 0;
-        for (int w = 0; w < SEL_THREADS / 64; w++) { const int64_t t = wsum[threadIdx.x][w]; wsum[threadIdx.x][w] = a; a += t; }
-        total[threadIdx.x] = a;
+        for (int j = 0; j < 16; j++) q[j] = base + x[j];
+        if (lane == 63) total[wv] = incl;
     }
     __syncthreads();
 #pragma unroll
-    for (int n = 0; n < N; n++) v[n] = wsum[n][wv] + x[n] - v[n];
+    for (int n = 0; n < N; n++) v[n] = sc[n * SEL_SCAN_STRIDE + at];
 }
 
 // ------------------------------------------------------------------ projected blocks
 // A block p is usable this round when neither ring can wrap onto it before the round ends:
 // the round allocates at most SG_BMAX descriptors and proj_round_words arena words, and
 // (pn0, an0) are the ring positions before it. The descriptor of a usable block is p's.
-__device__ __forceinline__ bool sg_block_usable(const SgSearchDev& S, int64_t p, int64_t pn0, int64_t an0, SgProj& D)
+// d_off is the arena position of block p's descriptor slot, read ahead.
+__device__ __forceinline__ bool sg_block_usable(const SgSearchDev& S, int64_t p, int64_t pn0, int64_t an0,
+                                                int64_t d_off)
 {
     if (p < 0 || p >= pn0 || pn0 + SG_BMAX - p > S.pdesc_cap) return false;
-    D = S.pdesc[p % S.pdesc_cap];
-    return an0 + S.proj_round_words - D.off <= S.parena_cap;
+    return an0 + S.proj_round_words - d_off <= S.parena_cap;
 }
 
 // Row of item f in block p (its rank in U of the block's itemset). Every item a record
@@ -482,26 +495,34 @@ struct SgPlan {
     int32_t nr;        // row-chunk ranges of the view
 };
 
-__device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgSegment& sg, const SgNodeMeta& m,
-                                                 int64_t pn0, int64_t an0)
+// What the plan reads of a record, its window and its block's descriptor slot.
+struct SgSelIn {
+    int64_t proj;       // the record's block (-1: none)
+    int64_t sg_words;   // words of the window's rows
+    int64_t d_off;      // descriptor slot of the block: arena position, words, transactions
+    int64_t d_words;
+    int32_t sup, ncand, ntail, nblk;
+    int32_t sg_n, d_n;
+};
+
+__device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgSelIn& m, int64_t pn0, int64_t an0)
 {
     SgPlan P;
     P.view = -1;
     P.src = -1;
-    P.words = sg.words;
-    P.src_words = sg.words;
+    P.words = m.sg_words;
+    P.src_words = m.sg_words;
     P.blk = 0;
     P.list = 0;
     P.ents = 0;
     P.units = 0;
     P.pnew = 0;
-    int64_t n_view = sg.n;
+    int64_t n_view = m.sg_n;
     if (S.proj_factor > 0) {
-        SgProj D;
-        if (sg_block_usable(S, m.proj, pn0, an0, D)) {
+        if (sg_block_usable(S, m.proj, pn0, an0, m.d_off)) {
             P.view = m.proj;
-            P.words = D.words;
-            n_view = D.n;
+            P.words = m.d_words;
+            n_view = m.d_n;
         }
         // A record with one child gets no block of its own: the child can project itself
         // at no more cost when it is counted, and needs none if it is an MFI. A dense block
@@ -534,6 +555,47 @@ __device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgS
     return P;
 }
 
+// Records per load group of the selection passes, and the group's loads: the fields the
+// plan reads of the records nid[0 .. SEL_GROUP) (slot 0 for absent ones, nid < 0), then of
+// their windows and block descriptor slots (window 0 and slot 0 for absent ones; no
+// descriptors with projection off). No load waits on a branch of its own record, so the
+// group's loads are in flight together. pn0m = pn0 % pdesc_cap: a usable block p lies
+// less than pdesc_cap before pn0, so its slot is pn0m - (pn0 - p), wrapped once.
+#define SEL_GROUP 2
+__device__ __forceinline__ void sel_load_group(const SgSearchDev& S, const SgSegment* __restrict__ segs,
+                                               const int32_t* nid, int64_t pn0, int64_t pn0m,
+                                               SgSelIn (&m)[SEL_GROUP])
+{
+    int32_t seg[SEL_GROUP];
+#pragma unroll
+    for (int c = 0; c < SEL_GROUP; c++) {
+        const SgNodeMeta* mp = S.meta + max(nid[c], 0);
+        seg[c] = nid[c] >= 0 ? mp->seg : 0;
+        m[c].proj = nid[c] >= 0 ? mp->proj : -1;
+        m[c].sup = mp->sup;
+        m[c].ncand = mp->ncand;
+        m[c].ntail = mp->ntail;
+        m[c].nblk = mp->nblk;
+    }
+#pragma unroll
+    for (int c = 0; c < SEL_GROUP; c++) {
+        m[c].sg_words = segs[seg[c]].words;
+        m[c].sg_n = segs[seg[c]].n;
+        m[c].d_off = 0;
+        m[c].d_words = 0;
+        m[c].d_n = 0;
+        if (S.proj_factor > 0) {   // uniform
+            const int64_t back = pn0 - m[c].proj;   // blocks since p (p >= 0 and this < pdesc_cap if usable)
+            int64_t slot = 0;
+            if (m[c].proj >= 0 && back > 0 && back < S.pdesc_cap) slot = pn0m - back + (pn0m < back ? S.pdesc_cap : 0);
+            const SgProj* dp = S.pdesc + slot;
+            m[c].d_off = dp->off;
+            m[c].d_words = dp->words;
+            m[c].d_n = dp->n;
+        }
+    }
+}
+
 // Take up to bmax records from the highest buckets, as many as fit the budgets: new
 // records (at most one per child) within the free slots and newborn table, count
 // cells within the cell table, new blocks within the round's transaction-list and arena
@@ -546,7 +608,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
 {
     constexpr int NQ = 13;
     __shared__ int32_t tops[SG_NBUCKETS], ends[SG_NBUCKETS];
-    __shared__ int64_t wsum[NQ][SEL_THREADS / 64];
+    __shared__ int64_t sc[NQ * SEL_SCAN_STRIDE];
     __shared__ int64_t tot[NQ];
     __shared__ int64_t s_budget;
     __shared__ int32_t s_total, s_cut, s_maxr;
@@ -581,13 +643,16 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
         S.ctl[CTL_T1] = 0;
     }
     const int64_t pn0 = S.ctl[CTL_PROJ_NEXT], an0 = S.ctl[CTL_ARENA_NEXT];
+    const int64_t pn0m = S.proj_factor > 0 ? pn0 % S.pdesc_cap : 0;
     __syncthreads();
     const int total = s_total;
     constexpr int PER = SG_BMAX / SEL_THREADS;
     // thread tid takes batch positions tid * pe .. tid * pe + pe - 1: contiguous, so the
     // scans below stay in batch order, and spread over every thread of the block
     const int pe = (total + SEL_THREADS - 1) / SEL_THREADS;
-    // pass 1: record slots, then sizes (independent loads within each phase)
+    // pass 1: record slots, then sizes. A thread's records go in groups of four whose loads
+    // (metas, then their segments and block descriptors; absent records read slot 0) are
+    // all issued before any is used, so a group waits on memory twice, not twice per record.
     int32_t nid[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -612,19 +677,22 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
     for (int n = 0; n < NQ; n++) sm[n] = 0;
     int32_t mr = 0;   // longest window (row-chunk ranges) among the taken records on window rows
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int i = tid * pe + q;
-        if (nid[q] >= 0) {
-            const SgNodeMeta m = S.meta[nid[q]];
-            const SgPlan P = sg_plan_record(S, segs[m.seg], m, pn0, an0);
+    for (int g = 0; g < PER; g += SEL_GROUP) {
+        if (g >= pe) break;   // block-uniform
+        SgSelIn m[SEL_GROUP];
+        sel_load_group(S, segs, nid + g, pn0, pn0m, m);
+#pragma unroll
+        for (int c = 0; c < SEL_GROUP; c++) {
+            if (nid[g + c] < 0) continue;
+            const SgPlan P = sg_plan_record(S, m[c], pn0, an0);
             const bool onb = P.view != -1;
             if (!onb) mr = max(mr, P.nr);
-            sm[0] += m.ncand;
-            sm[1] += m.ntail;
-            sm[2] += (int64_t)m.ncand * m.ntail;
-            sm[3] += onb ? 0 : m.nblk;
-            sm[4] += (int64_t)(1 + m.ntail) * S.dw_max;
-            sm[5] += onb ? (int64_t)m.nblk * P.nr : 0;
+            sm[0] += m[c].ncand;
+            sm[1] += m[c].ntail;
+            sm[2] += (int64_t)m[c].ncand * m[c].ntail;
+            sm[3] += onb ? 0 : m[c].nblk;
+            sm[4] += (int64_t)(1 + m[c].ntail) * S.dw_max;
+            sm[5] += onb ? (int64_t)m[c].nblk * P.nr : 0;
             if (P.pnew) {
                 sm[6] += P.list;
                 sm[7] += P.blk;
@@ -633,67 +701,75 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(SgSearchDev S, const SgS
                 sm[P.pnew == 2 ? 12 : 10] += P.units;
                 sm[11] += P.ents;
             }
-            S.batch[i] = nid[q];
         }
     }
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+        if (nid[q] >= 0) S.batch[tid * pe + q] = nid[q];
     int64_t ex[NQ];
 #pragma unroll
     for (int n = 0; n < NQ; n++) ex[n] = sm[n];
-    sg_block_excl_scan<NQ>(ex, wsum, tot);
+    sg_block_excl_scan<NQ>(ex, sc, tot);
     const int64_t sib_base = S.ctl[CTL_SIB_NEXT];
     // pass 2: offsets; the cut is the first record past a budget (offsets only grow)
     const int64_t budget = s_budget;
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int i = tid * pe + q;
-        if (nid[q] >= 0) {
-            const SgNodeMeta m = S.meta[nid[q]];
-            const SgSegment sg = segs[m.seg];
-            SgPlan P = sg_plan_record(S, sg, m, pn0, an0);
-            const int32_t nu = m.ncand, nt = m.ntail;
-            const int64_t cells = (int64_t)nu * nt;
-            if (ex[1] + nt > budget || ex[2] + cells > S.cell_cap ||
-                (P.pnew && (ex[6] + P.list > S.ptid_cap || ex[7] + P.blk > S.proj_round_words ||
-                            ex[11] + P.ents > S.pent_cap)))
-                atomicMin(&s_cut, i);
-            S.coff[i] = (int32_t)ex[0];
-            S.choff[i] = (int32_t)ex[1];
-            S.celloff[i] = ex[2];
-            S.unit_off[i] = ex[3];
-            // a block that would wrap onto another of this round gets none
-            const int64_t sz = (int64_t)(1 + nt) * S.dw_max;
-            S.sib_off[i] = ex[4] + sz <= S.sib_cap ? sib_base + ex[4] : -1;
-            S.bunitB[i] = ex[5];
-            S.bunitT[i] = ex[9];
-            S.bunitG[i] = ex[10];
-            S.bunitC[i] = ex[12];
-            // ring positions before this record (the allocation at the cut); a new block
-            // gets descriptor pn0 + ex[8] and arena words an0 + ex[7] onwards, and its
-            // descriptor is written by k_candlist for the records before the cut only
-            S.bpd[i] = pn0 + ex[8];
-            S.bpa[i] = an0 + ex[7];
-            if (P.pnew) {
-                P.view = pn0 + ex[8];
-                S.ptoff[i] = ex[6];
-                S.peoff[i] = ex[11];
-            }
-            S.bview[i] = P.view;
-            S.bsrc[i] = P.src;
-            S.bpnew[i] = P.pnew;
-            S.bnr[i] = P.nr;
-            ex[0] += nu;
-            ex[1] += nt;
-            ex[2] += cells;
-            ex[3] += P.view == -1 ? m.nblk : 0;
-            ex[4] += sz;
-            ex[5] += P.view == -1 ? 0 : (int64_t)m.nblk * P.nr;
-            if (P.pnew) {
-                ex[6] += P.list;
-                ex[7] += P.blk;
-                ex[8] += 1;
-                ex[9] += (P.src_words + SG_PT_WORDS - 1) / SG_PT_WORDS;
-                ex[P.pnew == 2 ? 12 : 10] += P.units;
-                ex[11] += P.ents;
+    for (int g = 0; g < PER; g += SEL_GROUP) {
+        if (g >= pe) break;   // block-uniform
+        SgSelIn mg[SEL_GROUP];
+        sel_load_group(S, segs, nid + g, pn0, pn0m, mg);
+#pragma unroll
+        for (int c = 0; c < SEL_GROUP; c++) {
+            const int q = g + c;
+            const int i = tid * pe + q;
+            if (nid[q] >= 0) {
+                const SgSelIn& m = mg[c];
+                SgPlan P = sg_plan_record(S, m, pn0, an0);
+                const int32_t nu = m.ncand, nt = m.ntail;
+                const int64_t cells = (int64_t)nu * nt;
+                if (ex[1] + nt > budget || ex[2] + cells > S.cell_cap ||
+                    (P.pnew && (ex[6] + P.list > S.ptid_cap || ex[7] + P.blk > S.proj_round_words ||
+                                ex[11] + P.ents > S.pent_cap)))
+                    atomicMin(&s_cut, i);
+                S.coff[i] = (int32_t)ex[0];
+                S.choff[i] = (int32_t)ex[1];
+                S.celloff[i] = ex[2];
+                S.unit_off[i] = ex[3];
+                // a block that would wrap onto another of this round gets none
+                const int64_t sz = (int64_t)(1 + nt) * S.dw_max;
+                S.sib_off[i] = ex[4] + sz <= S.sib_cap ? sib_base + ex[4] : -1;
+                S.bunitB[i] = ex[5];
+                S.bunitT[i] = ex[9];
+                S.bunitG[i] = ex[10];
+                S.bunitC[i] = ex[12];
+                // ring positions before this record (the allocation at the cut); a new block
+                // gets descriptor pn0 + ex[8] and arena words an0 + ex[7] onwards, and its
+                // descriptor is written by k_candlist for the records before the cut only
+                S.bpd[i] = pn0 + ex[8];
+                S.bpa[i] = an0 + ex[7];
+                if (P.pnew) {
+                    P.view = pn0 + ex[8];
+                    S.ptoff[i] = ex[6];
+                    S.peoff[i] = ex[11];
+                }
+                S.bview[i] = P.view;
+                S.bsrc[i] = P.src;
+                S.bpnew[i] = P.pnew;
+                S.bnr[i] = P.nr;
+                ex[0] += nu;
+                ex[1] += nt;
+                ex[2] += cells;
+                ex[3] += P.view == -1 ? m.nblk : 0;
+                ex[4] += sz;
+                ex[5] += P.view == -1 ? 0 : (int64_t)m.nblk * P.nr;
+                if (P.pnew) {
+                    ex[6] += P.list;
+                    ex[7] += P.blk;
+                    ex[8] += 1;
+                    ex[9] += (P.src_words + SG_PT_WORDS - 1) / SG_PT_WORDS;
+                    ex[P.pnew == 2 ? 12 : 10] += P.units;
+                    ex[11] += P.ents;
+                }
             }
         }
     }
