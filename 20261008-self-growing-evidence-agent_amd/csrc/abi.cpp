@@ -47,6 +47,7 @@ struct SearchBufs {
     hipGraph_t graph[2] = {nullptr, nullptr};       // one captured round per frontier order
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int32_t graph_feat[2] = {0, 0};                 // the D.feat_on each round was captured with
+    int32_t graph_ck[2] = {0, 0};                   // the D.count_kind each round was captured with
     uint64_t graph_gen[2] = {0, 0};                 // the search universe each round was captured on
     hipEvent_t ev[SG_STAGES + 1] = {};             // around the round's stages (direct launches)
     int64_t* h_ctl = nullptr;                       // pinned: processed copy of the control block, then
@@ -144,6 +145,7 @@ struct sgfim_ctx {
     bool pair_valid = false;
     int32_t pair_kind = SG_PAIR_FP4;    // pair-matrix kernel (sgfim_set_pair_kernel)
     int32_t pair_split_cap = 0;         // its transaction splits at most (0: automatic)
+    int32_t count_kind = SG_COUNT_VALU;   // counting kernel of the searches (sgfim_set_count_kernel)
     hipEvent_t pair_ev[2] = {nullptr, nullptr};   // around the latest pair-matrix computation
     bool pair_timed = false;
     int32_t* h_pair_pin = nullptr;      // pinned staging buffer for pair-matrix read-back (dpad x dpad)
@@ -668,9 +670,12 @@ int launch_round(sgfim_ctx* c, SearchState* S)
         return SGFIM_OK;
     }
     const int g = S->order_mode;
-    if (B->exec[g] && (B->graph_feat[g] != B->D.feat_on || B->graph_gen[g] != S->U.gen)) graph_free(B, g);
+    if (B->exec[g] && (B->graph_feat[g] != B->D.feat_on || B->graph_ck[g] != B->D.count_kind ||
+                       B->graph_gen[g] != S->U.gen))
+        graph_free(B, g);
     if (!B->exec[g]) {
         B->graph_feat[g] = B->D.feat_on;
+        B->graph_ck[g] = B->D.count_kind;
         B->graph_gen[g] = S->U.gen;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = sg_search_round(B->D, S->U.rows, c->d_vseg, S->U.pair, S->U.ok, S->U.sup,
@@ -721,7 +726,8 @@ int ensure_bufs(sgfim_ctx* c, int64_t cap, int64_t newborn_cap, int64_t ring_cap
     D.bmax = bmax;
     D.chunk_words = chunk;
     D.chunk_range = range;
-    D.count_grid = sg_count_group_blocks_per_cu(chunk) * c->n_cu;   // one persistent wave of workgroups
+    D.count_kind = c->count_kind;
+    D.count_grid = sg_count_group_blocks_per_cu(chunk, D.count_kind) * c->n_cu;   // one persistent wave of workgroups
     D.trace_cap = trace_cap;
     D.cell_cap = cell_cap;
     D.sib_cap = sib_cap;
@@ -1572,6 +1578,17 @@ int sgfim_set_pair_kernel(sgfim_ctx* c, int32_t kind, int32_t max_splits)
     return SGFIM_OK;
 }
 
+int sgfim_set_count_kernel(sgfim_ctx* c, int32_t kind)
+{
+    static_assert(SG_COUNT_VALU == SGFIM_COUNT_VALU && SG_COUNT_F4 == SGFIM_COUNT_F4, "kernel ids");
+    if (!c) return SGFIM_E_INVAL;
+    if (kind != SGFIM_COUNT_F4 && kind != SGFIM_COUNT_VALU)
+        return fail(c, SGFIM_E_INVAL, "count kernel %d: expected SGFIM_COUNT_F4 or SGFIM_COUNT_VALU", kind);
+    if (c->S && c->S->active && !c->S->done) return fail(c, SGFIM_E_STATE, "count kernel: an anytime search is in progress");
+    c->count_kind = kind;
+    return SGFIM_OK;
+}
+
 // The counting behind sgfim_count_batch and sgfim_count_batch_noalloc: the device scratch
 // (node descriptors, compacted candidates, counts, work counter) is allocated per call, or
 // taken from the device's static scratch (noalloc: no allocation at all, sizes bounded).
@@ -1863,7 +1880,7 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
             else
                 (void)hipGetLastError();
         }
-        ps.arena = a;
+        ps.arena = a & ~(int64_t)7;   // blocks start at multiples of 8 words (16-byte row loads)
         ps.pdesc = std::min<int64_t>(std::max<int64_t>(cap, (int64_t)1 << 16), (int64_t)1 << 20);
         int64_t nmax = 0;
         for (int32_t v = 0; v < nv; v++) nmax = std::max<int64_t>(nmax, vs[v].n);
@@ -1876,6 +1893,10 @@ static int search_setup(sgfim_ctx* c, const sgfim_mine_args* a)
     S->B = c->SB;
     SgSearchDev& D = S->B->D;
     D.feat_on = (S->order_mode == 1 || D.trace_cap > 0) ? 1 : 0;   // a round captured with another value is captured again
+    if (D.count_kind != c->count_kind) {   // likewise
+        D.count_kind = c->count_kind;
+        D.count_grid = sg_count_group_blocks_per_cu(D.chunk_words, D.count_kind) * c->n_cu;
+    }
     HIPCHK(c, sg_search_init(D, c->stream));
     S->n_roots = (int32_t)n_roots;
     S->roots = std::move(rp);

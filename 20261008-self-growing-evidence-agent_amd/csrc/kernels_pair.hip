@@ -22,6 +22,7 @@
 // triangle. grid.y splits the transactions (split-N) and partial tiles are added with
 // integer atomics, which keeps the result bit-exact.
 #include "sgfim_impl.h"
+#include "f4_phase.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -156,15 +157,11 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
 // significand) with room to spare; it is converted to int32 before the atomics add the
 // splits.
 //
-// Widening by bit phase: K step p (0..3) of a group of four bitmap words takes, for each
-// word, the transactions at bit p of every nibble. Masking the word with 0x1111...,
-// 0x2222... or 0x4444... leaves them as the e2m1 values 0.5, 1.0 or 2.0 (bit 3 alone is
-// -0.0, so phase 3 shifts bit 3 to bit 2 first). The E8M0 scale operands of the
-// instruction (2^1, 2^0 or 2^-1 for both A and B) make every matching pair multiply to
-// exactly 1.0. So one VALU instruction (two for phase 3) widens 8 transactions, and
-// lane (r, h) of K step p holds, in VGPR j, half h of word j's phase-p nibbles of row r.
-// A and B use the same (lane, VGPR, nibble) -> transaction map, so the K order inside
-// an instruction does not change the sum.
+// Widening by bit phase (f4_phase.h): K step p (0..3) of a group of four bitmap words
+// takes, for each word, the transactions at bit p of every nibble, as exact fp4 values
+// whose scaled products are 1. One VALU instruction (two for phase 3) widens 8
+// transactions, and lane (r, h) of K step p holds, in VGPR j, half h of word j's phase-p
+// nibbles of row r.
 //
 // Tiling: a workgroup of 8 waves (two per SIMD, so one wave's widening and LDS reads
 // overlap the other's matrix instructions) owns a 256 x 256 output tile, each wave 64 x
@@ -174,36 +171,9 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
 // stage's tile is stored into the other buffer while this one is multiplied), and each
 // four-word group's reads are issued before the previous group's matrix instructions.
 // Upper triangle only, split-N partial tiles added with integer atomics.
-typedef int v8i __attribute__((ext_vector_type(8)));
-typedef float v16f __attribute__((ext_vector_type(16)));
 
 #define PF_K 8                // words per stage (two groups of four words)
 #define PF_MAXW (1ll << 14)   // words per split (2^20 transactions; a multiple of PF_K)
-
-template <int P>
-__device__ __forceinline__ int sg_f4_phase(uint32_t x)
-{
-    if (P == 0) return (int)(x & 0x11111111u);
-    if (P == 1) return (int)(x & 0x22222222u);
-    if (P == 2) return (int)(x & 0x44444444u);
-    return (int)((x >> 1) & 0x44444444u);
-}
-
-// E8M0 scale of phase P for each operand, so that a matching pair multiplies to 1.0:
-// 0.5 * 0.5 * 2 * 2, 1 * 1, 2 * 2 * 2^-1 * 2^-1.
-template <int P>
-__device__ __forceinline__ constexpr int sg_f4_scale()
-{
-    return P == 0 ? 128 : P == 1 ? 127 : 126;
-}
-
-__device__ __forceinline__ v8i sg_f4_mask_frag(const uint4& v, int p)
-{
-    if (p == 0) return v8i{sg_f4_phase<0>(v.x), sg_f4_phase<0>(v.y), sg_f4_phase<0>(v.z), sg_f4_phase<0>(v.w), 0, 0, 0, 0};
-    if (p == 1) return v8i{sg_f4_phase<1>(v.x), sg_f4_phase<1>(v.y), sg_f4_phase<1>(v.z), sg_f4_phase<1>(v.w), 0, 0, 0, 0};
-    if (p == 2) return v8i{sg_f4_phase<2>(v.x), sg_f4_phase<2>(v.y), sg_f4_phase<2>(v.z), sg_f4_phase<2>(v.w), 0, 0, 0, 0};
-    return v8i{sg_f4_phase<3>(v.x), sg_f4_phase<3>(v.y), sg_f4_phase<3>(v.z), sg_f4_phase<3>(v.w), 0, 0, 0, 0};
-}
 
 // Workgroup q of a one-segment grid of n: consecutive workgroup ids go to the eight
 // XCDs in turn, so q is renumbered XCD-major; the tiles of one transaction split then

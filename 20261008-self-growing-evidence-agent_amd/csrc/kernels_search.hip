@@ -26,6 +26,7 @@
 // -> [score] -> k_commit; every launch has a fixed shape and reads its sizes from the
 // control block, so the round is captured as one hipGraph.
 #include "count_core.h"
+#include "f4_phase.h"
 
 #define SEL_THREADS 1024
 
@@ -1643,6 +1644,271 @@ __global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu
     }
 }
 
+// ------------------------------------------------------------------ counting on fp4 MFMA
+#define SG_F4G 2   // candidate groups of 16 per item: a child's masked operand feeds SG_F4G MFMAs
+// The units of k_count_group on the matrix cores (sgfim_set_count_kernel). Units, set-up
+// (need rows, listed candidates) and child prefixes (T(X) & T(e) of the block's 16
+// children over a chunk, in LDS) are k_count_group's; the cells of a group of 16 listed
+// candidates are then one 16 x 16 product of 0/1 matrices over the chunk's transactions,
+// C[j][c] = |prefix_j & row_c|, on v_mfma_scale_f32_16x16x128_f8f6f4 with the bit-phase
+// fp4 operands of f4_phase.h. Lane (r, q) = (lane & 15, lane >> 4) of a K step reads words
+// 8s + 2q and 8s + 2q + 1 of child r's prefix (LDS) and of candidate r's row (the view);
+// the four bit phases of those 128 bits are four K steps, 512 transactions per 8-word
+// slice s. A wave takes items (SG_F4G candidate groups, range of slices) round-robin; a
+// child's masked phase feeds the item's SG_F4G products. The MFMA computes all 16 x 16
+// cells of a group and only the needed nonzero ones are added (one integer atomic per
+// cell, chunk and slice range), so the cell matrix equals k_count_group's bit for bit; an
+// item's fp32 sums are at most 2^15 (512 words x 64), exact. Registers are capped for four
+// waves per SIMD (two workgroups per CU). It is slower than k_count_group on the benchmark
+// workloads (the masks cost 5 VALU instructions per operand dword and phase set, reused
+// only 16 or 32 times at 16 children per unit), so it is not the default.
+template <int CW>
+__global__ __launch_bounds__(SG_TILE_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_count_f4(SgSearchDev S, const uint64_t* __restrict__ rows,
+                                                                 const SgSegment* __restrict__ segs,
+                                                                 const uint64_t* __restrict__ ok)
+{
+    constexpr int PER = CW / 64;
+    constexpr int NW = SG_TILE_THREADS / 64;
+    constexpr int CWP = CW + 2;   // prefix row stride: 16 bytes of pad spread a slice's 16 rows over the banks
+    static_assert(SG_CB == 16 && NW == 8, "one 16 x 16 product per candidate group; two children per wave");
+    __shared__ uint64_t pre[CW];                                        // T(X) over the chunk
+    __shared__ __attribute__((aligned(16))) uint64_t cp[SG_CB][CWP];   // child prefixes; need rows during set-up
+    __shared__ uint32_t s_cand[SG_DMAX];     // (children mask << 16) | position in U
+    __shared__ int32_t s_g, s_cr, s_blk;
+    __shared__ int32_t s_items[SG_MAX_DEPTH];
+    __shared__ int32_t s_child[SG_CB];
+    __shared__ int32_t s_nc;
+    static_assert(SG_CB * 64 <= SG_CB * CWP, "need rows fit the prefix buffer");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint64_t* okr = &cp[0][0];
+    const int64_t n_units = S.ctl[CTL_UNITS];
+    const int64_t n_blocks = S.ctl[CTL_BLOCKS];
+    const int64_t units_a = S.ctl[CTL_UNITS_A];
+    const int nb = (int)S.ctl[CTL_BATCH_N];
+    if (n_units == 0) return;   // a round queued after the frontier emptied: nothing to count
+    if (threadIdx.x == 0)
+        atomicMin((unsigned long long*)&S.ctl[CTL_T0], (unsigned long long)wall_clock64());
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int64_t u = (int64_t)atomicAdd((unsigned long long*)&S.ctl[CTL_UNIT_NEXT], 1ull);
+            s_nc = 0;
+            if (u >= n_units) {
+                s_g = -1;
+            } else if (u < units_a) {
+                // records on window rows, range-major: all their child blocks work on
+                // row-chunk range cr together, so the rows of one range stay in L2 while
+                // every record uses them
+                const int cr = (int)(u / n_blocks);
+                const int64_t bq = u - (int64_t)cr * n_blocks;
+                const int g = S.blkrec[bq];
+                s_g = g;
+                s_cr = cr;
+                s_blk = (int)(bq - S.unit_off[g]);
+            } else {
+                // records on projected blocks, record-major: a record's blocks range by range
+                const int64_t v = u - units_a;
+                const int g = sg_find_record(S.bunitB, nb, v);
+                const int64_t local = v - S.bunitB[g];
+                const int nblk = S.meta[S.batch[g]].nblk;
+                const int cr = (int)(local / nblk);
+                s_g = g;
+                s_cr = cr;
+                s_blk = (int)(local - (int64_t)cr * nblk);
+            }
+        }
+        __syncthreads();
+        const int g = s_g;
+        if (g < 0) break;
+        const int cr = s_cr, blk = s_blk;
+        const int32_t slot = S.batch[g];
+        const SgNodeMeta m = S.meta[slot];
+        if (cr >= S.bnr[g]) {   // past this record's (shorter) window
+            __syncthreads();
+            continue;
+        }
+        const SgSegment sg = segs[m.seg];
+        // the record's view: its projected block (T(X) of the block's itemset is every
+        // transaction of it; X's later items are ANDed) or the window's rows (all of X)
+        const int64_t view = S.bview[g];
+        const uint64_t* seg_rows;
+        int64_t vstride, vwords;
+        int k0;
+        if (view >= 0) {
+            const SgProj D = S.pdesc[view % S.pdesc_cap];
+            seg_rows = S.parena + D.off % S.parena_cap;
+            vstride = D.stride;
+            vwords = D.words;
+            k0 = S.bpnew[g] ? m.k : m.pk;
+        } else {
+            seg_rows = rows + sg.row_base;
+            vstride = sg.stride;
+            vwords = sg.words;
+            k0 = 0;
+        }
+        const int kk = m.k - k0;
+        const int64_t nchunk = (vwords + CW - 1) / CW;
+        const int c0 = cr * S.chunk_range;
+        const int c1 = (int)min((int64_t)(c0 + S.chunk_range), nchunk);
+        const int j0 = blk * SG_CB;
+        const int ncb = min(SG_CB, m.ntail - j0);
+        const int32_t u0 = S.coff[g];
+        const int nu = m.ncand;
+        if (threadIdx.x < kk) {
+            const int32_t f = S.items[(int64_t)slot * SG_MAX_DEPTH + k0 + threadIdx.x];
+            s_items[threadIdx.x] = view >= 0 ? sg_block_row(S, view, f) : f;
+        }
+        if (threadIdx.x < ncb) s_child[threadIdx.x] = S.crow[u0 + j0 + threadIdx.x];
+        __syncthreads();
+        // set-up: the children's need rows (k_need_rows), then the list of needed candidates
+        {
+            const uint64_t* nr = S.need + (int64_t)(S.choff[g] + j0) * S.dw_max;
+            for (int t = threadIdx.x; t < ncb * sg.dw; t += blockDim.x) {
+                const int j = t / sg.dw, w = t - j * sg.dw;
+                okr[j * 64 + w] = nr[(int64_t)j * S.dw_max + w];
+            }
+        }
+        __syncthreads();
+        // Candidate i needs counting for child j0 + j when the pair is frequent, except that
+        // a candidate that is itself a child (i < ntail) is counted only below the
+        // diagonal: |T(X u {e_a, e_b})| is one support, counted once in cell (a, b),
+        // a > b, and read from there for cell (b, a).
+        for (int i = threadIdx.x; i < nu; i += blockDim.x) {
+            const int32_t f = S.clist[u0 + i];
+            const int jlo = i < m.ntail ? i + 1 - j0 : 0;   // children j with j0 + j > i
+            uint32_t mask = 0u;
+            for (int j = max(jlo, 0); j < ncb; j++)
+                if ((okr[j * 64 + (f >> 6)] >> (f & 63)) & 1ull) mask |= 1u << j;
+            if (mask) s_cand[atomicAdd(&s_nc, 1)] = (mask << 16) | (uint32_t)i;
+        }
+        __syncthreads();
+        const int nc = s_nc;
+        int32_t* cells = S.cells + S.celloff[g];
+        const int32_t* crow = S.crow + u0;   // rows of U's items in the view
+        // items: (SG_F4G groups of 16 listed candidates, range of 8-word slices); kp slice
+        // ranges per item so that small units still give every wave an item
+        const int nsg = (nc + 16 * SG_F4G - 1) / (16 * SG_F4G);
+        const int kp = nsg >= NW ? 1 : NW / max(nsg, 1);
+        // lane (r, q): candidate r of its item's group (its need bits and row), slice quarter q
+        const int r = lane & 15, q = lane >> 4;
+        for (int c = c0; c < c1; c++) {
+            const int64_t w0 = (int64_t)c * CW;
+            const int wn = (int)min((int64_t)CW, vwords - w0);
+            __syncthreads();   // every wave is done with the previous chunk's prefixes
+            // child prefixes T(X) & T(e) (zero past the last word and for absent children)
+            // and the children's supports, as in k_count_group
+            {
+                uint64_t rv[2][PER];
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++) {
+                    const int j = wave + NW * jj;
+                    const uint64_t* row = seg_rows + (int64_t)(j < ncb ? s_child[j] : 0) * vstride + w0;
+#pragma unroll
+                    for (int qq = 0; qq < PER; qq++) {
+                        const int w = lane + 64 * qq;
+                        rv[jj][qq] = (j < ncb && w < wn) ? row[w] : 0ull;
+                    }
+                }
+                constexpr int SW = CW / NW;   // words of a wave's slice
+                if (lane < SW) {
+                    const int w = wave * SW + lane;
+                    uint64_t v = w < wn ? ~0ull : 0ull;
+                    for (int i = 0; i < kk; i++)
+                        if (w < wn) v &= seg_rows[(int64_t)s_items[i] * vstride + w0 + w];
+                    pre[w] = v;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++) {
+                    const int j = wave + NW * jj;
+                    uint32_t acc = 0;
+#pragma unroll
+                    for (int qq = 0; qq < PER; qq++) {
+                        const uint64_t v = pre[lane + 64 * qq] & rv[jj][qq];
+                        cp[j][lane + 64 * qq] = v;   // zero for j >= ncb
+                        acc += (uint32_t)__popcll(v);
+                    }
+                    if (j < ncb) {
+                        const uint32_t sv = sg_wave_sum(acc);
+                        if (lane == 0 && sv) atomicAdd(&S.ccount[S.choff[g] + j0 + j], (int32_t)sv);
+                    }
+                }
+            }
+            __syncthreads();
+            const int nsl = (wn + 7) >> 3;   // 8-word slices holding row words
+            for (int it = wave; it < nsg * kp; it += NW) {   // wave-uniform
+                const int gs = it / kp, part = it - gs * kp;
+                const int s0 = part * nsl / kp, s1 = (part + 1) * nsl / kp;
+                // group gg of the item: listed candidates (4 gs + gg) x 16 + r
+                uint32_t ent[SG_F4G];
+                const uint64_t* brow[SG_F4G];
+                bool have[SG_F4G];
+#pragma unroll
+                for (int gg = 0; gg < SG_F4G; gg++) {
+                    const int ci = (SG_F4G * gs + gg) * 16 + r;
+                    ent[gg] = ci < nc ? s_cand[ci] : 0u;
+                    const int32_t fr = ent[gg] ? crow[(int)(ent[gg] & 0xffffu)] : -1;
+                    have[gg] = fr >= 0;
+                    brow[gg] = seg_rows + (int64_t)(fr < 0 ? 0 : fr) * vstride + w0;
+                }
+                // B: candidate r's words w, w + 1 of slice sl in each group (one 16-byte load:
+                // rows, chunks and block bases are 16-byte aligned). A pair at or past wn, or
+                // of an absent candidate, reads word 0 of the chunk and is zeroed (w < wn keeps
+                // the pair inside the row's stride, and a word past the row meets a zero
+                // prefix word).
+                auto loadb = [&](uint4 (&bq)[SG_F4G], int sl) {
+                    const int w = 8 * sl + 2 * q;
+#pragma unroll
+                    for (int gg = 0; gg < SG_F4G; gg++) {
+                        const bool in = have[gg] && sl < s1 && w < wn;
+                        const uint4 v = *(const uint4*)(brow[gg] + (in ? w : 0));
+                        bq[gg] = in ? v : uint4{0u, 0u, 0u, 0u};
+                    }
+                };
+                // per group two accumulators (phases 0, 2 and 1, 3): independent MFMA chains
+                v4f acc[SG_F4G][2];
+#pragma unroll
+                for (int gg = 0; gg < SG_F4G; gg++) acc[gg][0] = acc[gg][1] = v4f{0.f, 0.f, 0.f, 0.f};
+                // A: child r's words w, w + 1 (LDS, zero past the chunk's words), masked once per
+                // phase for the item's four groups
+                auto mult = [&](const uint4 (&bq)[SG_F4G], int sl) {
+                    const uint4 a = *(const uint4*)&cp[r][8 * sl + 2 * q];
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        const v8i ap = sg_f4_mask_frag(a, p);
+                        const int sc = p == 0 ? sg_f4_scale<0>() : p == 1 ? sg_f4_scale<1>() : sg_f4_scale<2>();
+#pragma unroll
+                        for (int gg = 0; gg < SG_F4G; gg++)
+                            acc[gg][p & 1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(ap, sg_f4_mask_frag(bq[gg], p),
+                                                                                          acc[gg][p & 1], 4, 4, 0, sc, 0, sc);
+                        __builtin_amdgcn_sched_barrier(0);   // one phase's operands live at a time
+                    }
+                };
+                for (int sl = s0; sl < s1; sl++) {   // wave-uniform
+                    uint4 bq[SG_F4G];
+                    loadb(bq, sl);
+                    mult(bq, sl);
+                }
+                // 16 x 16 output: lane (r, q) holds children 4q .. 4q + 3 against candidate r
+#pragma unroll
+                for (int gg = 0; gg < SG_F4G; gg++) {
+                    const int il = (int)(ent[gg] & 0xffffu);
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int j = 4 * q + e;
+                        const int32_t v = (int32_t)(acc[gg][0][e] + acc[gg][1][e]);   // integers below 2^24
+                        if (j < ncb && ((ent[gg] >> (16 + j)) & 1u) && v)
+                            atomicAdd(&cells[(int64_t)(j0 + j) * nu + il], v);
+                    }
+                }
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicMax((unsigned long long*)&S.ctl[CTL_T1], (unsigned long long)wall_clock64());
+    }
+}
+
 // ------------------------------------------------------------------ expansion
 #define SG_EXP_WAVES 16
 #define SG_EXP_PER 4   // most children per wave per workgroup step
@@ -1878,14 +2144,22 @@ __global__ __launch_bounds__(256) void k_commit(SgSearchDev S, const SgSegment* 
 
 // ------------------------------------------------------------------ launchers
 // Resident workgroups per CU of the tile kernel (sizes its persistent grid).
-int sg_count_group_blocks_per_cu(int32_t chunk_words)
+int sg_count_group_blocks_per_cu(int32_t chunk_words, int32_t kind)
 {
     int n = 0;
     hipError_t e;
-    switch (chunk_words) {
-    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<128>, SG_TILE_THREADS, 0); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<512>, SG_TILE_THREADS, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<256>, SG_TILE_THREADS, 0); break;
+    if (kind == SG_COUNT_F4) {
+        switch (chunk_words) {
+        case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_f4<128>, SG_TILE_THREADS, 0); break;
+        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_f4<512>, SG_TILE_THREADS, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_f4<256>, SG_TILE_THREADS, 0); break;
+        }
+    } else {
+        switch (chunk_words) {
+        case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<128>, SG_TILE_THREADS, 0); break;
+        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<512>, SG_TILE_THREADS, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_count_group<256>, SG_TILE_THREADS, 0); break;
+        }
     }
     if (e != hipSuccess || n < 1) {
         (void)hipGetLastError();
@@ -1952,10 +2226,18 @@ hipError_t sg_search_round(const SgSearchDev& S, const uint64_t* rows, const SgS
     if ((e = mark(4)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_need_rows, dim3(1024), dim3(256), 0, s, S, segs, ok);
     if ((e = mark(5)) != hipSuccess) return e;
-    switch (S.chunk_words) {
-    case 128: hipLaunchKernelGGL(k_count_group<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
-    case 512: hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
-    default: hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+    if (S.count_kind == SG_COUNT_F4) {
+        switch (S.chunk_words) {
+        case 128: hipLaunchKernelGGL(k_count_f4<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+        case 512: hipLaunchKernelGGL(k_count_f4<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+        default: hipLaunchKernelGGL(k_count_f4<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+        }
+    } else {
+        switch (S.chunk_words) {
+        case 128: hipLaunchKernelGGL(k_count_group<128>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+        case 512: hipLaunchKernelGGL(k_count_group<512>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+        default: hipLaunchKernelGGL(k_count_group<256>, dim3(S.count_grid), dim3(SG_TILE_THREADS), 0, s, S, rows, segs, ok); break;
+        }
     }
     if ((e = mark(6)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_expand_children, dim3(1024), dim3(SG_EXP_WAVES * 64), 0, s, S, segs, pair, ok, sup_all,

@@ -200,7 +200,7 @@ struct SgSearchDev {
     int32_t proj_dense;   // a new block whose support is at least 1/proj_dense of its source's is compacted
                           // word by word (k_proj_compress), a sparser one gathered bit by bit
     int32_t proj_mintail; // children a record needs for a compacted block (2 for a gathered one)
-    int32_t pad_;
+    int32_t count_kind;   // counting kernel: SG_COUNT_F4 (fp4 MFMA) or SG_COUNT_VALU (AND + popcount)
     uint64_t* parena;     // [parena_cap + proj_max_block] words
     int64_t parena_cap;
     int64_t proj_max_block;  // largest block (words)
@@ -284,7 +284,9 @@ hipError_t sg_sensor_bins(const double* x, int32_t S, int32_t B, int32_t window,
 // Online (P^2) variant: E [n_win][S][B + 1] receives the edges used for each window.
 hipError_t sg_online_bins(const double* x, int32_t S, int32_t B, int32_t window, int32_t stride, int64_t n_win,
                           double* E, int32_t* items, uint32_t* present, int64_t* off, hipStream_t s);
-int sg_count_group_blocks_per_cu(int32_t chunk_words);
+#define SG_COUNT_VALU 0
+#define SG_COUNT_F4 1
+int sg_count_group_blocks_per_cu(int32_t chunk_words, int32_t kind);
 hipError_t sg_search_init(const SgSearchDev& S, hipStream_t s);
 hipError_t sg_search_seed(const SgSearchDev& S, const SgSegment* segs, const int32_t* pair,
                           const uint64_t* ok, const int32_t* roots, int32_t n_roots,

@@ -89,6 +89,7 @@ _SIGS = {
     "sgfim_pair_matrix_valu": (C.c_int, [P, P]),
     "sgfim_pair_time": (C.c_int, [P, C.POINTER(C.c_double)]),
     "sgfim_set_pair_kernel": (C.c_int, [P, C.c_int32, C.c_int32]),
+    "sgfim_set_count_kernel": (C.c_int, [P, C.c_int32]),
     "sgfim_mfma_probe": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "sgfim_count_batch": (C.c_int, [P, P, P, C.c_int32, P, P, P]),
     "sgfim_mine": (C.c_int, [P, C.POINTER(MineArgs), PP]),

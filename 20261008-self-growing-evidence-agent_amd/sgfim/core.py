@@ -307,6 +307,14 @@ class Context:
 
     PAIR_FP4 = 0
     PAIR_INT8 = 1
+    COUNT_VALU = 0
+    COUNT_F4 = 1
+
+    def set_count_kernel(self, kind: int = 0):
+        """Counting kernel of the searches: COUNT_VALU (default, AND + popcount per needed
+        cell) or COUNT_F4 (fp4 MFMA 16 x 16 products of children and candidates, exact); the
+        results do not depend on it."""
+        self._check(self._L.sgfim_set_count_kernel(self._h, int(kind)), "set_count_kernel")
 
     def set_pair_kernel(self, kind: int = 0, max_splits: int = 0):
         """Pair-matrix kernel: PAIR_FP4 (default, e2m1 MFMA with exact fp32 sums) or PAIR_INT8

@@ -551,6 +551,16 @@ def main():
     # the rate the matrix cores sustain on this device with no memory traffic
     mfma_fp4 = ctx.mfma_probe(ctx.PAIR_FP4)
     mfma_int8 = ctx.mfma_probe(ctx.PAIR_INT8)
+    # the fp4 MFMA counting kernel on the same steps (the MFI sets must not change)
+    ctx.set_count_kernel(ctx.COUNT_F4)
+    f4 = timed(5, 1, "fp4 counting")
+    ctx.set_count_kernel(ctx.COUNT_VALU)
+    if f4["mfis"] != mfis:
+        raise RuntimeError("counting kernels disagree")
+    count_ab = {"kernels": {"valu": "k_count_group", "f4": "k_count_f4"},
+                "ms_per_step": {"valu": ms_per_step, "f4": 1000.0 * f4["dt"] / 5},
+                "count_ms_per_launch": {"valu": head["cms"] / max(head["launches"], 1),
+                                        "f4": f4["cms"] / max(f4["launches"], 1)}}
     full = None
     if proj and args.full_rows_steps > 0:
         # the same search with projection off: every count over the full 15,625-word rows
@@ -604,6 +614,9 @@ def main():
                                           "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                                           "frac": pair_ops / (pair_ab["int8"] * 1e-3) / 1e12 / INT8_PEAK_TOPS,
                                           "sustained_tops": mfma_int8}},
+        # the counting kernel's fp4 MFMA variant (16 x 16 products of children and candidates)
+        # timed on the same search beside the AND + popcount kernel the headline uses
+        "count_kernels_ab": count_ab,
     }
     ctx.close()
     if not args.no_extra:

@@ -274,6 +274,18 @@ int sgfim_pair_time(sgfim_ctx* ctx, double* ms);
 #define SGFIM_PAIR_INT8 1
 int sgfim_set_pair_kernel(sgfim_ctx* ctx, int32_t kind, int32_t max_splits);
 
+/* The counting kernel of the context's searches: SGFIM_COUNT_VALU (default; 64-bit AND +
+ * popcount per needed cell) or SGFIM_COUNT_F4 (the cells of 16 children x 32 listed
+ * candidates as two v_mfma_scale_f32_16x16x128_f8f6f4 products of bit-phase fp4 operands
+ * per 512 transactions, exact fp32 sums). Both count the same units with the same need
+ * rules and give identical cell matrices, so results never depend on it. E_STATE while an
+ * anytime search runs. Replaces the reference's per-node counter
+ * (getResFrequencyCudaKernel, kernel_bitmap.cu:412-442) and its BMMA waves
+ * (kernel_bmma.cu:636-768). */
+#define SGFIM_COUNT_VALU 0
+#define SGFIM_COUNT_F4 1
+int sgfim_set_count_kernel(sgfim_ctx* ctx, int32_t kind);
+
 /* Same matrix by the VALU AND+popcount kernel (cross-check path). */
 int sgfim_pair_matrix_valu(sgfim_ctx* ctx, int32_t* out);
 
