@@ -15,7 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define SG_MAX_DEPTH 32
+#define SG_MAX_DEPTH 64
 #define SG_NBUCKETS 64
 #define SG_WAVE 64
 #define SG_BMAX 16384   // largest batch of frontier nodes per round

@@ -43,7 +43,8 @@ enum sgfim_status {
     SGFIM_E_CAPACITY = -8    /* a device table (frontier, MFI ring) is too small  */
 };
 
-#define SGFIM_MAX_DEPTH 32   /* longest itemset; the reference stops at 31 (kernel_bmma.cu:798) */
+#define SGFIM_MAX_DEPTH 64   /* longest itemset; the reference's BMMA packer stops at 31 (kernel_bmma.cu:798),
+                                its itemsets are vectors (kernel_bitmap.cu:38-43) */
 
 /* Mining flags (sgfim_mine / sgfim_anytime_begin). */
 #define SGFIM_ROOT_STRICT  0x1u  /* TensorFIM root rule: only items with freq/n > rel_thr root a
@@ -302,7 +303,7 @@ int sgfim_count_batch(sgfim_ctx* ctx, const int32_t* prefix_ids, const int64_t* 
  * callers that cannot allocate device memory (code under stream capture, tight serving
  * loops, harnesses that forbid hipMalloc). Same arguments, results and errors as
  * sgfim_count_batch; the device scratch (node descriptors, candidate ids, counts, work
- * counter) is a static array of each device, 10.5 MB, set up when the context is created.
+ * counter) is a static array of each device, 13.1 MB, set up when the context is created.
  * At most 16384 prefixes and 2^20 candidates present in the set per call (more:
  * SGFIM_E_INVAL). Every call on one GPU shares that GPU's array: calls that overlap on one
  * GPU (from several contexts or threads) race and must be serialised by the caller. */

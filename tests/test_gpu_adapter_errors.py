@@ -75,8 +75,8 @@ def test_error_paths():
             ctx.mine(1, flags=L.ORDER_LEARNED)
         assert e.value.code == L.E_STATE                       # no scorer loaded
         with pytest.raises(L.SgfimError) as e:
-            ctx.count_batch([list(range(1, 40))], [[1]])
-        assert e.value.code == L.E_DEPTH                       # prefix longer than 32 items
+            ctx.count_batch([list(range(1, 70))], [[1]])
+        assert e.value.code == L.E_DEPTH                       # prefix longer than 64 items
     # device-memory cap
     with Context(device=0, mem_cap_bytes=1 << 16) as ctx:
         T = gen.workload1(1, 50_000)
@@ -109,9 +109,9 @@ def test_error_paths():
 
 def test_depth_overflow_and_deadline_and_cancel():
     print("case: depth, deadline, cancel", flush=True)
-    # 40 items that always occur together with independent noise items: one MFI of length 40
+    # 70 items that always occur together with independent noise items: one MFI of length 70
     rng = np.random.default_rng(0)
-    rows = [tuple(range(1, 41)) + tuple(int(x) for x in rng.choice(np.arange(41, 61), 3, replace=False))
+    rows = [tuple(range(1, 71)) + tuple(int(x) for x in rng.choice(np.arange(71, 91), 3, replace=False))
             for _ in range(200)]
     off, items = csr_from_rows(rows)
     with Context(device=0) as ctx:
@@ -187,17 +187,17 @@ def test_adapter_failures_return_not_ok(tmp_path):
     reference adapter reports an engine that left no Results file (engine_adapter.py:82-99,
     123-129); nothing is raised."""
     print("case: adapter failures -> ok=False", flush=True)
-    # 40 items that always occur together: at 150 of 200 one MFI of 40 items, past the
+    # 70 items that always occur together: at 150 of 200 one MFI of 70 items, past the
     # longest itemset the search supports (E_DEPTH)
     rng = np.random.default_rng(0)
-    rows = [tuple(range(1, 41)) + tuple(int(x) for x in rng.choice(np.arange(41, 61), 3, replace=False))
+    rows = [tuple(range(1, 71)) + tuple(int(x) for x in rng.choice(np.arange(71, 91), 3, replace=False))
             for _ in range(200)]
     deep = tmp_path / "deep.txt"
     deep.write_text("\n".join(" ".join(map(str, r)) for r in rows) + "\n")
     r = EA.run_tensorfim("bmma", str(deep), 0.75, reps=2)
     assert r["ok"] is False and r["mfis"] == {} and r["comp_times"] == [] and "longer than" in r["log"]
     assert not os.path.exists(f"{deep}-{0.75:.6f}=Results.txt")
-    # the exact family at each stage threshold: the 40-item set is an MFI there (E_DEPTH); the
+    # the exact family at each stage threshold: the 70-item set is an MFI there (E_DEPTH); the
     # engine's stage rule (roots = the first k items, extensions below them) never reaches it
     a = EA.run_anyfim(str(deep), 3, exact=True)
     assert a["ok"] is False and a["stages"] == {} and "longer than" in a["log"]
@@ -247,3 +247,34 @@ def test_item_id_zero(tmp_path):
     assert eng["ok"] and eng["mfis"] == O.tensorfim_family(without, 0.1)
     af = EA.run_anyfim(str(p), 3)
     assert af["ok"] is False and "id 0" in af["log"]
+
+
+@pytest.mark.parametrize("k", [40, 64])
+def test_long_itemsets_up_to_the_limit(k):
+    """Itemsets of up to 64 items. Closed forms (the oracle's search has no equivalence
+    pruning, so it cannot enumerate the 2^k subsets): k items in every one of 200 rows,
+    each with 3 of 20 noise items (at most 30 rows each), give the single MFI {1..k} at
+    support 200 for any threshold above 30; two overlapping patterns in 100 rows each give
+    exactly the two patterns."""
+    print(f"case: one {k}-item pattern in every row, and two overlapping patterns", flush=True)
+    rng = np.random.default_rng(k)
+    rows = [tuple(range(1, k + 1)) + tuple(int(x) for x in rng.choice(np.arange(k + 1, k + 21), 3, replace=False))
+            for _ in range(200)]
+    off, items = csr_from_rows(rows)
+    a = tuple(range(1, 51))
+    b = tuple(range(21, 61))
+    rows2 = [a + (61 + t % 7,) for t in range(100)] + [b + (68 + t % 7,) for t in range(100)]
+    off2, items2 = csr_from_rows(rows2)
+    with Context(device=0) as ctx:
+        ctx.load_csr(off, items)
+        assert ctx.mine(150).as_dict() == {frozenset(range(1, k + 1)): 200}
+        ctx.load_csr(off2, items2)
+        assert ctx.mine(60).as_dict() == {frozenset(a): 100, frozenset(b): 100}
+        # one more item in every row: 65 items, past the limit
+        if k == 64:
+            rows3 = [tuple(range(1, 66)) for _ in range(50)]
+            off3, items3 = csr_from_rows(rows3)
+            ctx.load_csr(off3, items3)
+            with pytest.raises(L.SgfimError) as e:
+                ctx.mine(40)
+            assert e.value.code == L.E_DEPTH
