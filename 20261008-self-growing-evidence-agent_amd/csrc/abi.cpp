@@ -100,6 +100,9 @@ struct SearchState {
     // rounds queued on the stream and not yet processed (their snapshot slots are
     // [pipe_head, pipe_head + pipe_n) mod 2); they carry over from one step call to the next
     int pipe_n = 0, pipe_head = 0;
+    // queued rounds already counted in count_launches (the rounds left queued when the
+    // search ended: they run after the one that emptied the frontier and take nothing)
+    int pipe_counted = 0;
     SearchBufs* B = nullptr;         // the context's buffers used by this run
     Universe U;                      // the items it searches
     // host sources of the run's asynchronous uploads, alive until the next run
@@ -1950,7 +1953,9 @@ static int pipe_pop(sgfim_ctx* c, SearchState* S, bool drain)
     if (e != hipSuccess) return fail(c, SGFIM_E_DEVICE, "round: %s", hipGetErrorString(e));
     memcpy(B->h_ctl, B->h_ctl + (1 + slot) * CTL_SLOTS, CTL_SLOTS * sizeof(int64_t));
     if (B->h_ctl[CTL_BATCH_N] > 0) S->rounds++;   // a round queued after the frontier emptied is not counted
-    S->count_launches++;   // every queued round launches the counting kernel once (an empty one returns at once)
+    // every queued round launches the counting kernel once (an empty one returns at once)
+    if (S->pipe_counted > 0) S->pipe_counted--;
+    else S->count_launches++;
     if (B->h_ctl[CTL_UNITS] > 0 && B->h_ctl[CTL_BATCH_N] > 0) {
         // device-side duration of the counting kernel: first workgroup start to last
         // workgroup end, from the constant-rate wall clock
@@ -2021,6 +2026,15 @@ int sgfim_anytime_step(sgfim_ctx* c, int32_t max_rounds, sgfim_round_stats* st)
         }
         if (S->pipe_n == 0) break;
         if (!stop && max_rounds > 0 && processed >= max_rounds) break;   // queued rounds carry over
+        // the search ended: a round still queued runs after the one whose snapshot showed an
+        // empty frontier, so it takes no record, counts nothing and finds no MFI. It is not
+        // waited for here (the next call that needs the stream processes it), and its
+        // counting-kernel launch is counted now.
+        if (S->done && rc == SGFIM_OK) {
+            S->count_launches += S->pipe_n - S->pipe_counted;
+            S->pipe_counted = S->pipe_n;
+            break;
+        }
         // process the oldest round queued (after an earlier failure, only wait for it)
         const int r2 = pipe_pop(c, S, rc == SGFIM_OK);
         processed++;

@@ -44,6 +44,14 @@ class MineResult:
         return self.count
 
 
+def _copy(ptr: C.c_void_p, ctype, n: int, dtype) -> np.ndarray:
+    """A new array holding the n values of ctype at ptr (frombuffer on a ctypes array type
+    costs about half of np.ctypeslib.as_array per call)."""
+    if n <= 0:
+        return np.zeros(0, dtype=dtype)
+    return np.frombuffer((ctype * n).from_address(ptr.value), dtype=dtype).copy()
+
+
 def _take_result(h: C.c_void_p, stats: dict | None = None, convert: bool = True) -> MineResult:
     Lb = L.lib()
     items = C.c_void_p()
@@ -60,23 +68,23 @@ def _take_result(h: C.c_void_p, stats: dict | None = None, convert: bool = True)
         nn = n.value
         res = MineResult(stats=stats or {}, count=nn)
         if nn and not convert:
-            o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
-            it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
-            s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
+            o = _copy(off, C.c_int64, nn + 1, np.int64)
+            it = _copy(items, C.c_int32, int(o[-1]), np.int32)
+            s = _copy(sup, C.c_int64, nn, np.int64)
             res.arrays = (it, o, s)
-            res.windows = np.ctypeslib.as_array(C.cast(win, C.POINTER(C.c_int32)), shape=(nn,)).copy()
-            res.times = np.ctypeslib.as_array(C.cast(tms, C.POINTER(C.c_double)), shape=(nn,)).copy()
+            res.windows = _copy(win, C.c_int32, nn, np.int32)
+            res.times = _copy(tms, C.c_double, nn, np.float64)
         elif nn:
-            o = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_int64)), shape=(nn + 1,)).copy()
-            it = np.ctypeslib.as_array(C.cast(items, C.POINTER(C.c_int32)), shape=(int(o[-1]),)).copy()
-            s = np.ctypeslib.as_array(C.cast(sup, C.POINTER(C.c_int64)), shape=(nn,)).copy()
-            r = np.ctypeslib.as_array(C.cast(rnd, C.POINTER(C.c_int32)), shape=(nn,)).copy()
-            t = np.ctypeslib.as_array(C.cast(tms, C.POINTER(C.c_double)), shape=(nn,)).copy()
+            o = _copy(off, C.c_int64, nn + 1, np.int64)
+            it = _copy(items, C.c_int32, int(o[-1]), np.int32)
+            s = _copy(sup, C.c_int64, nn, np.int64)
+            r = _copy(rnd, C.c_int32, nn, np.int32)
+            t = _copy(tms, C.c_double, nn, np.float64)
             res.itemsets = [tuple(int(x) for x in it[o[i]:o[i + 1]]) for i in range(nn)]
             res.supports = [int(x) for x in s]
             res.rounds = [int(x) for x in r]
             res.times = [float(x) for x in t]
-            wv = np.ctypeslib.as_array(C.cast(win, C.POINTER(C.c_int32)), shape=(nn,)).copy()
+            wv = _copy(win, C.c_int32, nn, np.int32)
             res.windows = [int(x) for x in wv]
         return res
     finally:

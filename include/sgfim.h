@@ -330,7 +330,9 @@ int sgfim_mine(sgfim_ctx* ctx, const sgfim_mine_args* args, sgfim_result** out);
  * with the next rounds already queued, so the device keeps searching while the caller
  * takes and handles the MFIs. Calls that reload data, change the scorer or the trace,
  * or read the pair matrix first process the queued rounds (their MFIs become pending);
- * after a cancel the next step waits for them without reporting their MFIs. */
+ * after a cancel the next step waits for them without reporting their MFIs. When a step
+ * finds the search done, a round still queued (queued before the frontier was seen empty)
+ * takes nothing and is not waited for; the next call that needs the stream processes it. */
 int sgfim_anytime_begin(sgfim_ctx* ctx, const sgfim_mine_args* args);
 int sgfim_anytime_step(sgfim_ctx* ctx, int32_t max_rounds, sgfim_round_stats* stats);
 int sgfim_anytime_poll(sgfim_ctx* ctx, sgfim_round_stats* stats);
