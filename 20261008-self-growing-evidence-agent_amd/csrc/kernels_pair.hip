@@ -163,9 +163,12 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
 // transactions, and lane (r, h) of K step p holds, in VGPR j, half h of word j's phase-p
 // nibbles of row r.
 //
-// Tiling: a workgroup of 8 waves (two per SIMD, so one wave's widening and LDS reads
-// overlap the other's matrix instructions) owns a 256 x 256 output tile, each wave 64 x
-// 128 as 2 x 4 MFMA tiles (128 fp32 accumulators per lane). Bit tiles of PK words are
+// Tiling: a workgroup of 8 waves owns a TT x TT output tile, each wave TT/4 x TT/2 as
+// (TT/128) x (TT/64) MFMA tiles. TT = 128 (1 x 2 tiles, 32 fp32 accumulators per lane,
+// 100 VGPRs and 40 KB of LDS: two workgroups per CU, four waves per SIMD) runs 91 us on
+// 1M transactions x 512 items where TT = 256 (2 x 4 tiles, 128 accumulators, one workgroup
+// per CU) ran 117 us: a quarter of the split-N atomics per output element and twice the
+// resident waves outweigh the operands' halved reuse. Bit tiles of PK words are
 // staged in LDS with rows laid out [half][word], so a lane reads four K steps of its
 // half with one 16-byte read; they are double-buffered (one barrier per stage: the next
 // stage's tile is stored into the other buffer while this one is multiplied), and each
@@ -173,6 +176,8 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
 // Upper triangle only, split-N partial tiles added with integer atomics.
 
 #define PF_K 8                // words per stage (two groups of four words)
+#define PF_TILE 128           // output rows / columns of a workgroup tile (the kernel also takes 256)
+#define PF_SLOTS 512          // workgroups resident at once, two per CU (the grid fills whole rounds)
 #define PF_MAXW (1ll << 14)   // words per split (2^20 transactions; a multiple of PF_K)
 
 // Workgroup q of a one-segment grid of n: consecutive workgroup ids go to the eight
@@ -184,23 +189,24 @@ __device__ __forceinline__ int sg_xcd_major(int q, int n)
     return x * (n >> 3) + min(x, n & 7) + k;
 }
 
-template <bool SEGS, int PK>
+template <bool SEGS, int PK, int TT>
 __global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ rows_all,
                                                   const SgSegment* __restrict__ segs, SgSegment one,
                                                   int32_t* __restrict__ out_all, int32_t splits)
 {
     constexpr int NT = 512;
-    constexpr int MA = 2;                 // A fragments (32 rows each) per wave
+    constexpr int MA = TT / 128;          // A fragments (32 rows each) per wave
+    constexpr int NB = TT / 64;           // B fragments (32 columns each) per wave
     constexpr int LDW = 2 * PK + 4;       // LDS row stride in dwords: [half][word] + pad
     constexpr int LQ = PK;                // loader threads per row
     constexpr int LR = NT / LQ;           // rows per loader pass
-    constexpr int LP = PT_TILE / LR;      // loader passes per matrix
+    constexpr int LP = TT / LR;           // loader passes per matrix
     constexpr int G = PK / 4;             // four-word groups per stage
-    __shared__ __attribute__((aligned(16))) uint32_t As[2][PT_TILE * LDW];
-    __shared__ __attribute__((aligned(16))) uint32_t Bs[2][PT_TILE * LDW];
+    __shared__ __attribute__((aligned(16))) uint32_t As[2][TT * LDW];
+    __shared__ __attribute__((aligned(16))) uint32_t Bs[2][TT * LDW];
     const SgSegment sg = SEGS ? segs[blockIdx.z] : one;
     const int dpad = sg.dpad;
-    const int T = (dpad + PT_TILE - 1) / PT_TILE;
+    const int T = (dpad + TT - 1) / TT;
     int bx = (int)blockIdx.x, by = (int)blockIdx.y;
     if (!SEGS) {
         const int q = sg_xcd_major(bx + (int)gridDim.x * by, (int)(gridDim.x * gridDim.y));
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ ro
     if (bx >= T * (T + 1) / 2) return;
     int ti, tj;
     sg_tri_pair(bx, T, ti, tj);
-    const int I0 = ti * PT_TILE, J0 = tj * PT_TILE;
+    const int I0 = ti * TT, J0 = tj * TT;
     const int64_t per = ((sg.words + splits - 1) / splits + PK - 1) / PK * PK;
     const int64_t wb = (int64_t)by * per;
     const int64_t we = min(sg.words, wb + per);
@@ -251,33 +257,33 @@ __global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ ro
             Bs[buf][row * LDW + PK + lq] = (uint32_t)(gb[i] >> 32);
         }
     };
-    uint4 av[2][MA], bv[2][4];
+    uint4 av[2][MA], bv[2][NB];
     auto lread = [&](int buf, int g, int slot) {
 #pragma unroll
         for (int a = 0; a < MA; a++)
             av[slot][a] = *(const uint4*)&As[buf][(32 * MA * wr + 32 * a + r) * LDW + PK * h + 4 * g];
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            bv[slot][b] = *(const uint4*)&Bs[buf][(128 * wc + 32 * b + r) * LDW + PK * h + 4 * g];
+        for (int b = 0; b < NB; b++)
+            bv[slot][b] = *(const uint4*)&Bs[buf][((TT / 2) * wc + 32 * b + r) * LDW + PK * h + 4 * g];
     };
-    v16f acc[MA][4];
+    v16f acc[MA][NB];
 #pragma unroll
     for (int a = 0; a < MA; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = v16f{};
+        for (int b = 0; b < NB; b++) acc[a][b] = v16f{};
     auto mma = [&](int slot) {
 #pragma unroll
         for (int dd = 0; dd < 4; dd++) {
-            v8i A[MA], B[4];
+            v8i A[MA], B[NB];
 #pragma unroll
             for (int a = 0; a < MA; a++) A[a] = sg_f4_mask_frag(av[slot][a], dd);
 #pragma unroll
-            for (int b = 0; b < 4; b++) B[b] = sg_f4_mask_frag(bv[slot][b], dd);
+            for (int b = 0; b < NB; b++) B[b] = sg_f4_mask_frag(bv[slot][b], dd);
             const int sc = dd == 0 ? sg_f4_scale<0>() : dd == 1 ? sg_f4_scale<1>() : sg_f4_scale<2>();
 #pragma unroll
             for (int a = 0; a < MA; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++)
+                for (int b = 0; b < NB; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[a], B[b], acc[a][b], 4, 4, 0, sc,
                                                                                  0, sc);
         }
@@ -302,11 +308,11 @@ __global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ ro
 #pragma unroll
     for (int a = 0; a < MA; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+        for (int b = 0; b < NB; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
                 const int row = I0 + 32 * MA * wr + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
-                const int col = J0 + 128 * wc + 32 * b + r;
+                const int col = J0 + (TT / 2) * wc + 32 * b + r;
                 const int32_t x = (int32_t)acc[a][b][v];
                 if (x && row < dpad && col < dpad) atomicAdd(&out[(int64_t)row * dpad + col], x);
             }
@@ -411,12 +417,13 @@ __global__ __launch_bounds__(256) void k_pair_ok(const int32_t* __restrict__ pai
 // the splits (never below what exactness needs).
 static int32_t pair_splits(int64_t units, int64_t words, bool f4, int32_t cap)
 {
+    const int64_t slots = f4 ? PF_SLOTS : 256;   // workgroups resident at once
     const int64_t steps = std::max<int64_t>(1, (words + PT_K - 1) / PT_K);   // one K step per split at least
     const int64_t need = f4 ? std::max<int64_t>(1, (words + PF_MAXW - 1) / PF_MAXW) : 1;
-    int64_t splits = std::max<int64_t>(need, 256 / std::max<int64_t>(units, 1));
-    if (units * splits > 256) {   // more than one round: fill the last one
-        const int64_t rounds = (units * splits + 255) / 256;
-        splits = std::max(splits, rounds * 256 / units);
+    int64_t splits = std::max<int64_t>(need, slots / std::max<int64_t>(units, 1));
+    if (units * splits > slots) {   // more than one round: fill the last one
+        const int64_t rounds = (units * splits + slots - 1) / slots;
+        splits = std::max(splits, rounds * slots / units);
     }
     if (cap > 0) splits = std::min<int64_t>(splits, std::max<int64_t>(cap, need));
     splits = std::min(splits, steps);
@@ -453,13 +460,14 @@ hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, i
     one.words = W;
     one.dpad = dpad;
     one.pair_base = 0;
-    const int T = (dpad + PT_TILE - 1) / PT_TILE;
-    const int tiles = T * (T + 1) / 2;
     const bool f4 = kind == SG_PAIR_FP4;
+    const int tt = f4 ? PF_TILE : PT_TILE;
+    const int T = (dpad + tt - 1) / tt;
+    const int tiles = T * (T + 1) / 2;
     const int32_t splits = pair_splits(tiles, W, f4, split_cap);
     const dim3 grid((unsigned)tiles, (unsigned)splits, 1);
     if (f4)
-        hipLaunchKernelGGL((k_pair_f4<false, PF_K>), grid, dim3(512), 0, s, rows, (const SgSegment*)nullptr, one,
+        hipLaunchKernelGGL((k_pair_f4<false, PF_K, PF_TILE>), grid, dim3(512), 0, s, rows, (const SgSegment*)nullptr, one,
                            pair, splits);
     else
         hipLaunchKernelGGL(k_pair_tile<false>, grid, dim3(256), 0, s, rows, (const SgSegment*)nullptr, one, pair,
@@ -474,13 +482,14 @@ hipError_t sg_launch_pair_mfma_segs(const uint64_t* rows, const SgSegment* segs,
                                     int32_t split_cap, hipStream_t s)
 {
     if (nseg <= 0) return hipSuccess;
-    const int T = (max_dpad + PT_TILE - 1) / PT_TILE;
-    const int tiles = T * (T + 1) / 2;
     const bool f4 = kind == SG_PAIR_FP4;
+    const int tt = f4 ? PF_TILE : PT_TILE;
+    const int T = (max_dpad + tt - 1) / tt;
+    const int tiles = T * (T + 1) / 2;
     const int32_t splits = pair_splits((int64_t)tiles * nseg, max_words, f4, split_cap);
     SgSegment none{};
     if (f4)
-        hipLaunchKernelGGL((k_pair_f4<true, PF_K>), dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg),
+        hipLaunchKernelGGL((k_pair_f4<true, PF_K, PF_TILE>), dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg),
                            dim3(512), 0, s, rows, segs, none, pair, (int32_t)splits);
     else
         hipLaunchKernelGGL(k_pair_tile<true>, dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg), dim3(256), 0,

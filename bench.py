@@ -74,10 +74,14 @@ def setting_name(ctx) -> str:
     return f"tile{cw}x{cr}-f{f}" + (f"-d{d}m{m}" if f else "")
 
 
-def pair_tiles(dpad: int) -> int:
-    """256 x 256 output tiles the pair kernel computes (the upper triangle; a mirror pass
+PAIR_TILE_F4 = 128     # output tile of the fp4 pair kernel (k_pair_f4)
+PAIR_TILE_INT8 = 256   # output tile of the int8 pair kernel (k_pair_tile)
+
+
+def pair_tiles(dpad: int, tile: int) -> int:
+    """tile x tile output tiles a pair kernel computes (the upper triangle; a mirror pass
     copies the rest)."""
-    t = (dpad + 255) // 256
+    t = (dpad + tile - 1) // tile
     return t * (t + 1) // 2
 
 
@@ -529,7 +533,9 @@ def main():
                    "launches": rx["count_launches"]}
     copy_gbs = ctx.copy_probe(1 << 31, 5)   # on-box HBM copy rate, 2 GiB buffers (8x the MALL)
     value = tot_counts / dt
-    pair_ops = 2.0 * pair_tiles(((ctx.n_items + 127) // 128) * 128) * 256 * 256 * T.n_txn
+    dpad = ((ctx.n_items + 127) // 128) * 128
+    pair_ops = 2.0 * pair_tiles(dpad, PAIR_TILE_F4) * PAIR_TILE_F4 ** 2 * T.n_txn
+    pair_ops_int8 = 2.0 * pair_tiles(dpad, PAIR_TILE_INT8) * PAIR_TILE_INT8 ** 2 * T.n_txn
     pair_tops = pair_ops / (pair_avg * 1e-3) / 1e12 if pair_avg > 0 else None
     roof = count_roofline(head, setting)
     roof["timing_check"] = event_check
@@ -596,13 +602,13 @@ def main():
         "roofline": roof,
         # the same kernel on the same search with projected blocks off (full-length rows)
         "roofline_full_rows": full,
-        # the pair matrix (part of every step): 2 x 256^2 x N fp4 flops per computed tile
+        # the pair matrix (part of every step): 2 x 128^2 x N fp4 flops per computed tile
         # (0/1 operands, exact integer sums), against the dense fp4 MFMA peak; the int8 kernel
-        # on the same input is timed beside it
+        # (256 x 256 tiles) on the same input is timed beside it
         "pair_roofline": {"bound": "mfma", "kernel": "k_pair_f4", "avg_launch_ms": pair_avg,
                           "achieved": pair_tops, "peak": FP4_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": pair_tops / FP4_PEAK_TFLOPS if pair_tops else None,
-                          "tiles": pair_tiles(((ctx.n_items + 127) // 128) * 128),
+                          "tiles": pair_tiles(dpad, PAIR_TILE_F4), "tile": PAIR_TILE_F4,
                           "timed": "HIP events around the zero fill, the tile kernel and the mirror pass",
                           "full_square_equivalent_tflops": 2.0 * ctx.n_items ** 2 * T.n_txn / (pair_avg * 1e-3) / 1e12
                           if pair_avg > 0 else None,
@@ -610,9 +616,10 @@ def main():
                           "frac_of_sustained": pair_tops / mfma_fp4 if pair_tops and mfma_fp4 else None,
                           "kernels_ab_ms": pair_ab,
                           "int8_kernel": {"kernel": "k_pair_tile", "avg_launch_ms": pair_ab["int8"],
-                                          "achieved": pair_ops / (pair_ab["int8"] * 1e-3) / 1e12,
+                                          "achieved": pair_ops_int8 / (pair_ab["int8"] * 1e-3) / 1e12,
                                           "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                                          "frac": pair_ops / (pair_ab["int8"] * 1e-3) / 1e12 / INT8_PEAK_TOPS,
+                                          "frac": pair_ops_int8 / (pair_ab["int8"] * 1e-3) / 1e12 / INT8_PEAK_TOPS,
+                                          "tiles": pair_tiles(dpad, PAIR_TILE_INT8), "tile": PAIR_TILE_INT8,
                                           "sustained_tops": mfma_int8}},
         # the counting kernel's fp4 MFMA variant (16 x 16 products of children and candidates)
         # timed on the same search beside the AND + popcount kernel the headline uses
