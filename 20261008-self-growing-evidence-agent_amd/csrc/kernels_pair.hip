@@ -12,12 +12,15 @@
 // touch memory. A and B fragments use the same (lane, byte) -> transaction map,
 // so the K order inside an instruction is irrelevant to the sum.
 //
-// Tiling: a workgroup of 4 waves owns a 256 x 256 output tile and each wave a 128 x
-// 128 quarter: 4 x 4 MFMA tiles whose 16 accumulators (256 int32 per lane) stay in the
-// accumulation registers, so every widened 16-bit fragment feeds four instructions.
-// The bit tiles of a K step (8 words = 512 transactions of 256 rows of A and of B) are
-// loaded with coalesced 64-byte row segments into LDS, double-buffered through
-// registers (the next step's loads are in flight while the current one multiplies).
+// Tiling: a workgroup of 4 waves owns a TT x TT output tile and each wave a TT/2 x TT/2
+// quarter of (TT/64) x (TT/64) MFMA tiles whose accumulators stay in the accumulation
+// registers, so every widened 16-bit fragment feeds TT/64 instructions. TT = 128 (2 x 2
+// tiles, 64 int32 accumulators per lane, 144 VGPRs, 20 KB of LDS: three workgroups per
+// CU) runs 184 us on 1M transactions x 512 items where TT = 256 (4 x 4 tiles, one
+// workgroup per CU) ran 204 us. The bit tiles of a K step (8 words = 512 transactions of
+// TT rows of A and of B) are loaded with coalesced 64-byte row segments into LDS,
+// double-buffered through registers (the next step's loads are in flight while the
+// current one multiplies).
 // Only tiles on or above the diagonal are computed; a mirror pass fills the lower
 // triangle. grid.y splits the transactions (split-N) and partial tiles are added with
 // integer atomics, which keeps the result bit-exact.
@@ -37,7 +40,8 @@ __device__ __forceinline__ v4i sg_widen16(uint32_t b)
     return r;
 }
 
-#define PT_TILE 256   // output rows / columns of a workgroup tile
+#define PT_TILE 128   // output rows / columns of a workgroup tile (the kernel also takes 256)
+#define PT_SLOTS 768  // workgroups resident at once, three per CU (the grid fills whole rounds)
 #define PT_K 8        // words per K step
 #define PT_LDS 20     // LDS row stride in dwords (16 data + 4 pad; 16-byte aligned rows)
 
@@ -52,20 +56,22 @@ __device__ __forceinline__ void sg_tri_pair(int q, int T, int& ti, int& tj)
 
 // C[I0.., J0..] += A^T B over words [wb, we) of one segment (bit rows, 0/1 widened in
 // registers). SEGS: the segment is segs[blockIdx.z], else `one`.
-template <bool SEGS>
+template <bool SEGS, int TT>
 __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ rows_all,
                                                    const SgSegment* __restrict__ segs, SgSegment one,
                                                    int32_t* __restrict__ out_all, int32_t splits)
 {
-    __shared__ __attribute__((aligned(16))) uint32_t As[PT_TILE * PT_LDS];
-    __shared__ __attribute__((aligned(16))) uint32_t Bs[PT_TILE * PT_LDS];
+    constexpr int NA = TT / 64;   // MFMA tiles per wave along each side (a wave owns TT/2 x TT/2)
+    constexpr int LP = TT / 32;   // loader passes (32 rows each)
+    __shared__ __attribute__((aligned(16))) uint32_t As[TT * PT_LDS];
+    __shared__ __attribute__((aligned(16))) uint32_t Bs[TT * PT_LDS];
     const SgSegment sg = SEGS ? segs[blockIdx.z] : one;
     const int dpad = sg.dpad;
-    const int T = (dpad + PT_TILE - 1) / PT_TILE;
+    const int T = (dpad + TT - 1) / TT;
     if ((int)blockIdx.x >= T * (T + 1) / 2) return;
     int ti, tj;
     sg_tri_pair((int)blockIdx.x, T, ti, tj);
-    const int I0 = ti * PT_TILE, J0 = tj * PT_TILE;
+    const int I0 = ti * TT, J0 = tj * TT;
     const int64_t per = ((sg.words + splits - 1) / splits + PT_K - 1) / PT_K * PT_K;
     const int64_t wb = (int64_t)blockIdx.y * per;
     const int64_t we = min(sg.words, wb + per);
@@ -75,28 +81,28 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 31, h = lane >> 5;
     const int wr = wave >> 1, wc = wave & 1;
-    // loader: thread t moves word (t & 7) of rows (t >> 3) + 32 i, i < 8, of A and B
+    // loader: thread t moves word (t & 7) of rows (t >> 3) + 32 i, i < LP, of A and B
     const int lq = tid & 7, lr = tid >> 3;
-    uint64_t ga[8], gb[8];
+    uint64_t ga[LP], gb[LP];
     auto gload = [&](int64_t w0) {
         const bool wok = w0 + lq < we;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < LP; i++) {
             const int row = lr + 32 * i;
             ga[i] = (wok && I0 + row < dpad) ? rows[(int64_t)(I0 + row) * sg.stride + w0 + lq] : 0ull;
             gb[i] = (wok && J0 + row < dpad) ? rows[(int64_t)(J0 + row) * sg.stride + w0 + lq] : 0ull;
         }
     };
-    v16i acc[4][4];
+    v16i acc[NA][NA];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < NA; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = v16i{};
+        for (int b = 0; b < NA; b++) acc[a][b] = v16i{};
     gload(wb);
     for (int64_t w0 = wb; w0 < we; w0 += PT_K) {
         __syncthreads();   // the previous step has finished reading the tiles
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < LP; i++) {
             const int row = lr + 32 * i;
             As[row * PT_LDS + 2 * lq] = (uint32_t)ga[i];
             As[row * PT_LDS + 2 * lq + 1] = (uint32_t)(ga[i] >> 32);
@@ -107,41 +113,41 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
         if (w0 + PT_K < we) gload(w0 + PT_K);
 #pragma unroll 1
         for (int g4 = 0; g4 < 4; g4++) {   // four dwords = four MFMA K steps
-            uint4 av[4], bv[4];
+            uint4 av[NA], bv[NA];
 #pragma unroll
-            for (int a = 0; a < 4; a++) av[a] = *(const uint4*)&As[(128 * wr + 32 * a + r) * PT_LDS + 4 * g4];
+            for (int a = 0; a < NA; a++) av[a] = *(const uint4*)&As[((TT / 2) * wr + 32 * a + r) * PT_LDS + 4 * g4];
 #pragma unroll
-            for (int b = 0; b < 4; b++) bv[b] = *(const uint4*)&Bs[(128 * wc + 32 * b + r) * PT_LDS + 4 * g4];
+            for (int b = 0; b < NA; b++) bv[b] = *(const uint4*)&Bs[((TT / 2) * wc + 32 * b + r) * PT_LDS + 4 * g4];
 #pragma unroll
             for (int dd = 0; dd < 4; dd++) {
-                v4i A[4], B[4];
+                v4i A[NA], B[NA];
 #pragma unroll
-                for (int a = 0; a < 4; a++) {
+                for (int a = 0; a < NA; a++) {
                     const uint32_t x = dd == 0 ? av[a].x : dd == 1 ? av[a].y : dd == 2 ? av[a].z : av[a].w;
                     A[a] = sg_widen16((x >> (16 * h)) & 0xFFFFu);
                 }
 #pragma unroll
-                for (int b = 0; b < 4; b++) {
+                for (int b = 0; b < NA; b++) {
                     const uint32_t x = dd == 0 ? bv[b].x : dd == 1 ? bv[b].y : dd == 2 ? bv[b].z : bv[b].w;
                     B[b] = sg_widen16((x >> (16 * h)) & 0xFFFFu);
                 }
 #pragma unroll
-                for (int a = 0; a < 4; a++)
+                for (int a = 0; a < NA; a++)
 #pragma unroll
-                    for (int b = 0; b < 4; b++) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc[a][b], 0, 0, 0);
+                    for (int b = 0; b < NA; b++) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc[a][b], 0, 0, 0);
             }
         }
     }
     // 32 x 32 int32 accumulator layout: lane l, element v holds
     // row 8*(v/4) + 4*(l/32) + v%4, column l%32.
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < NA; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+        for (int b = 0; b < NA; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const int row = I0 + 128 * wr + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
-                const int col = J0 + 128 * wc + 32 * b + r;
+                const int row = I0 + (TT / 2) * wr + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
+                const int col = J0 + (TT / 2) * wc + 32 * b + r;
                 const int32_t x = acc[a][b][v];
                 if (x && row < dpad && col < dpad) atomicAdd(&out[(int64_t)row * dpad + col], x);
             }
@@ -409,15 +415,15 @@ __global__ __launch_bounds__(256) void k_pair_ok(const int32_t* __restrict__ pai
     if (lane == 0) ok[sg.ok_base + (int64_t)i * sg.dw + w] = v;
 }
 
-// Transaction splits of `units` tile grids (tiles x segments). One workgroup per CU fits
-// (256 accumulators per lane leave one wave per SIMD), so the grid is sized in whole
-// dispatch rounds of 256 workgroups: a count just above a multiple of 256 would leave a
-// last round of a few workgroups with the chip waiting on them. The fp4 kernel's
+// Transaction splits of `units` tile grids (tiles x segments). The grid is sized in whole
+// dispatch rounds of the workgroups that are resident at once (three per CU for the
+// int8 kernel, two for the fp4 one): a count just above a multiple of a round would
+// leave a last round of a few workgroups with the chip waiting on them. The fp4 kernel's
 // splits also stay within PF_MAXW words, where its fp32 sums are exact. `cap` > 0 limits
 // the splits (never below what exactness needs).
 static int32_t pair_splits(int64_t units, int64_t words, bool f4, int32_t cap)
 {
-    const int64_t slots = f4 ? PF_SLOTS : 256;   // workgroups resident at once
+    const int64_t slots = f4 ? PF_SLOTS : PT_SLOTS;   // workgroups resident at once
     const int64_t steps = std::max<int64_t>(1, (words + PT_K - 1) / PT_K);   // one K step per split at least
     const int64_t need = f4 ? std::max<int64_t>(1, (words + PF_MAXW - 1) / PF_MAXW) : 1;
     int64_t splits = std::max<int64_t>(need, slots / std::max<int64_t>(units, 1));
@@ -470,7 +476,7 @@ hipError_t sg_launch_pair_mfma(const uint64_t* rows, int32_t dpad, int64_t Wp, i
         hipLaunchKernelGGL((k_pair_f4<false, PF_K, PF_TILE>), grid, dim3(512), 0, s, rows, (const SgSegment*)nullptr, one,
                            pair, splits);
     else
-        hipLaunchKernelGGL(k_pair_tile<false>, grid, dim3(256), 0, s, rows, (const SgSegment*)nullptr, one, pair,
+        hipLaunchKernelGGL((k_pair_tile<false, PT_TILE>), grid, dim3(256), 0, s, rows, (const SgSegment*)nullptr, one, pair,
                            splits);
     hipLaunchKernelGGL(k_pair_mirror<false>, dim3(256, 1), dim3(256), 0, s, (const SgSegment*)nullptr, one, pair);
     return hipGetLastError();
@@ -492,7 +498,7 @@ hipError_t sg_launch_pair_mfma_segs(const uint64_t* rows, const SgSegment* segs,
         hipLaunchKernelGGL((k_pair_f4<true, PF_K, PF_TILE>), dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg),
                            dim3(512), 0, s, rows, segs, none, pair, (int32_t)splits);
     else
-        hipLaunchKernelGGL(k_pair_tile<true>, dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg), dim3(256), 0,
+        hipLaunchKernelGGL((k_pair_tile<true, PT_TILE>), dim3((unsigned)tiles, (unsigned)splits, (unsigned)nseg), dim3(256), 0,
                            s, rows, segs, none, pair, (int32_t)splits);
     hipLaunchKernelGGL(k_pair_mirror<true>, dim3(64, (unsigned)nseg), dim3(256), 0, s, segs, none, pair);
     return hipGetLastError();

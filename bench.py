@@ -75,7 +75,7 @@ def setting_name(ctx) -> str:
 
 
 PAIR_TILE_F4 = 128     # output tile of the fp4 pair kernel (k_pair_f4)
-PAIR_TILE_INT8 = 256   # output tile of the int8 pair kernel (k_pair_tile)
+PAIR_TILE_INT8 = 128   # output tile of the int8 pair kernel (k_pair_tile)
 
 
 def pair_tiles(dpad: int, tile: int) -> int:
@@ -604,7 +604,7 @@ def main():
         "roofline_full_rows": full,
         # the pair matrix (part of every step): 2 x 128^2 x N fp4 flops per computed tile
         # (0/1 operands, exact integer sums), against the dense fp4 MFMA peak; the int8 kernel
-        # (256 x 256 tiles) on the same input is timed beside it
+        # (also 128 x 128 tiles) on the same input is timed beside it
         "pair_roofline": {"bound": "mfma", "kernel": "k_pair_f4", "avg_launch_ms": pair_avg,
                           "achieved": pair_tops, "peak": FP4_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": pair_tops / FP4_PEAK_TFLOPS if pair_tops else None,

@@ -92,8 +92,8 @@ def test_pair_matrix_mfma_vs_valu_vs_oracle(ctx, seed, n):
     assert (P == R).all()
 
 
-def test_pair_matrix_three_tiles_and_timing(ctx):
-    say("case: pair matrix 700 items (3 x 3 tiles of 256, partial last tile), 20011 txns")
+def test_pair_matrix_ragged_tiles_and_timing(ctx):
+    say("case: pair matrix 700 items (6 x 6 tiles of 128, partial last tile), 20011 txns")
     rng = np.random.default_rng(11)
     n, d = 20011, 700
     dense = rng.random((n, d)) < 0.05

@@ -50,8 +50,8 @@ def test_pair_kernels_match_oracle(ctx, kind, name, seed, n):
 
 @pytest.mark.parametrize("kind,name", KINDS)
 @pytest.mark.parametrize("cap", [0, 1, 3])
-def test_pair_kernels_three_tiles_any_split(ctx, kind, name, cap):
-    say(f"case: pair matrix {name} 700 items (3 x 3 tiles), 20011 txns, max_splits={cap}")
+def test_pair_kernels_ragged_tiles_any_split(ctx, kind, name, cap):
+    say(f"case: pair matrix {name} 700 items (6 x 6 tiles, the last ragged), 20011 txns, max_splits={cap}")
     rng = np.random.default_rng(11)
     n, d = 20011, 700
     dense = rng.random((n, d)) < 0.05

@@ -1,7 +1,7 @@
 """GPU parity at the widest item sets.
 
-- 4,096 loaded items are searched in place: 64-word extension masks and a 16 x 16 grid of
-  256 x 256 pair tiles.
+- 4,096 loaded items are searched in place: 64-word extension masks and a 32 x 32 grid of
+  128 x 128 pair tiles.
 - More than 4,096 loaded items are searched on a compact copy of the items frequent at the
   search's thresholds. This covers a single set, new thresholds (a new copy), both
   frontier orders, windows and several thresholds in one search. The query functions
