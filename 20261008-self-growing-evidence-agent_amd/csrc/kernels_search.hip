@@ -1151,9 +1151,13 @@ __global__ __launch_bounds__(256) void k_proj_gather(SgSearchDev S, const uint64
 
 // Bit compress of x under mask m (the bits of x where m is set, packed to the low end), in
 // the form with the six move masks of m precomputed (Hacker's Delight, 2nd ed., sec. 7-4,
-// "compress" with the masks hoisted out of the loop): masks cost ~70 operations once per
-// mask, each compress 24 per word.
-__device__ __forceinline__ void sg_compress_masks(uint64_t m, uint64_t (&mv)[6])
+// "compress" with the masks hoisted out of the loop; the masks cost ~70 operations once per
+// mask). Stage i moves the bits of mv[i] down by 2^i and keeps the others:
+// x = (x & ~mv[i]) | ((x & mv[i]) >> 2^i), on a 32-bit pair two ANDs, a funnel shift, a
+// shift and two three-input logic instructions. Stage 0 keeps keep0 = m & ~mv[0] instead,
+// so x needs no separate AND with m (after it x holds no bits outside the moved mask), and
+// stage 5 only moves high bits into the low half: 32 instructions per word.
+__device__ __forceinline__ void sg_compress_masks(uint64_t m, uint64_t (&mv)[6], uint64_t& keep0)
 {
     uint64_t mk = ~m << 1;   // zeros of m, counted from the right
 #pragma unroll
@@ -1166,30 +1170,53 @@ __device__ __forceinline__ void sg_compress_masks(uint64_t m, uint64_t (&mv)[6])
         mp ^= mp << 32;
         const uint64_t v = mp & m;   // bits of m that move by 2^i
         mv[i] = v;
+        if (i == 0) keep0 = m & ~v;
         m = (m ^ v) | (v >> (1 << i));
         mk &= ~mp;
     }
 }
 
-__device__ __forceinline__ uint64_t sg_compress(uint64_t x, uint64_t m, const uint64_t (&mv)[6])
+// One v_bitop3_b32 each; the immediate is the truth table over the operands' patterns
+// 0xF0, 0xCC, 0xAA: (a & b) | c is 0xEA, (a & ~b) | c is 0xBA.
+__device__ __forceinline__ uint32_t sg_and_or(uint32_t a, uint32_t b, uint32_t c)
 {
-    x &= m;
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xEA);
+}
+__device__ __forceinline__ uint32_t sg_andn_or(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xBA);
+}
+
+__device__ __forceinline__ uint64_t sg_compress(uint64_t x, const uint64_t (&mv)[6], uint64_t keep0)
+{
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const uint64_t t = x & mv[i];
-        x = (x ^ t) | (t >> (1 << i));
+    for (int i = 0; i < 5; i++) {
+        const uint32_t ml = (uint32_t)mv[i], mh = (uint32_t)(mv[i] >> 32);
+        const uint64_t t = ((uint64_t)(hi & mh) << 32 | (lo & ml)) >> (1 << i);
+        if (i == 0) {
+            lo = sg_and_or(lo, (uint32_t)keep0, (uint32_t)t);
+            hi = sg_and_or(hi, (uint32_t)(keep0 >> 32), (uint32_t)(t >> 32));
+        } else {
+            lo = sg_andn_or(lo, ml, (uint32_t)t);
+            hi = sg_andn_or(hi, mh, (uint32_t)(t >> 32));
+        }
     }
-    return x;
+    // stage 5 moves bits of the high half into the low one only
+    const uint32_t mh5 = (uint32_t)(mv[5] >> 32);
+    lo |= hi & mh5;
+    hi &= ~mh5;
+    return (uint64_t)hi << 32 | lo;
 }
 
 // Rows of the round's compacted blocks. One wave per unit (record, SG_PC_WORDS block words
 // = bits [start, end), SG_PC_ROWS rows): starting at the word entry that holds bit start,
-// each lane takes one entry (a nonzero source word w with mask m and first bit p), and for
-// every row loads the source row's word w, compresses it under m and ORs the packed bits
-// into the unit's words in LDS at bit p - start (an entry that straddles the unit's start
-// or end keeps only its bits inside). Entries are in block-bit order, so the unit's entries
-// are consecutive; the wave takes 64 at a time until they pass end. The unit's words are
-// then stored whole, row by row.
+// each lane takes one entry (a nonzero source word w with mask m and first bit p), trims
+// m to its bits inside the unit (an entry that straddles the unit's start or end), and for
+// every row loads the source row's word w, compresses it under the trimmed mask and ORs
+// the packed bits into the unit's words in LDS at bit p - start. Entries are in block-bit
+// order, so the unit's entries are consecutive; the wave takes 64 at a time until they
+// pass end. The unit's words are then stored whole, row by row.
 __global__ __launch_bounds__(64) void k_proj_compress(SgSearchDev S, const uint64_t* __restrict__ rows,
                                                      const SgSegment* __restrict__ segs)
 {
@@ -1255,18 +1282,21 @@ __global__ __launch_bounds__(64) void k_proj_compress(SgSearchDev S, const uint6
             const bool act = e < nent && E.p < end && E.m != 0ull;
             if (!__ballot(act)) break;
             // inactive lanes: no bits (mask 0), word 0 of the source row, LDS word 0
-            const uint64_t mk = act ? E.m : 0ull;
+            uint64_t mk = act ? E.m : 0ull;
             const int c = __popcll(mk);
             const int64_t off = act ? (int64_t)E.p - start : 0;
             const int hs = off < 0 ? (int)(-off) : 0;          // bits before the unit's start
             const int64_t pos = off < 0 ? 0 : off;
-            const int64_t keep = act ? min((int64_t)(c - hs), end - start - pos) : 0;   // >= 1 on active lanes
-            const uint64_t km = keep >= 64 ? ~0ull : ((1ull << keep) - 1ull);
+            const int keep = act ? (int)min((int64_t)(c - hs), end - start - pos) : 0;   // >= 1 on active lanes
+            // trim the mask to the kept bits: drop its lowest hs and then its highest
+            // c - hs - keep set bits (only an entry that straddles the start or the end has any)
+            for (int i = 0; i < hs; i++) mk &= mk - 1ull;
+            for (int i = keep; i < c - hs; i++) mk ^= 1ull << (63 - __clzll(mk));
             const int q = (int)(pos >> 6), s = (int)(pos & 63);
             const bool spill = s != 0 && keep + s > 64;
             const int64_t w = act ? E.w : 0;
-            uint64_t mvv[6];
-            sg_compress_masks(mk, mvv);
+            uint64_t mvv[6], kp0;
+            sg_compress_masks(mk, mvv, kp0);
             // source word w of block row rr (rows past the last reread the last row)
             auto load = [&](int rr) -> uint64_t {
                 rr = min(rr, nr - 1);
@@ -1275,7 +1305,7 @@ __global__ __launch_bounds__(64) void k_proj_compress(SgSearchDev S, const uint6
                 return vrows[ro + w];
             };
             auto put = [&](int rr, uint64_t x) {
-                const uint64_t y = (sg_compress(x, mk, mvv) >> hs) & km;
+                const uint64_t y = sg_compress(x, mvv, kp0);
                 atomicOr(&s_out[rr][q], (unsigned long long)(y << s));
                 if (spill) atomicOr(&s_out[rr][q + 1], (unsigned long long)(y >> (64 - s)));
             };
