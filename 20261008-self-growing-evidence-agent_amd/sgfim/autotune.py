@@ -34,7 +34,7 @@ CANDIDATES = ((128, 2), (128, 4), (128, 8), (256, 2), (256, 4), (256, 8), (512, 
 DEFAULT = (256, 4)
 PROJ_CANDIDATES = (0, 4, 8, 16, 32)
 PROJ_DEFAULT = 4   # the library's default factor
-RULE_CANDIDATES = ((32, 16), (32, 4), (32, 2), (8, 16), (8, 2), (0, 2))   # (dense, min_children)
+RULE_CANDIDATES = ((32, 16), (32, 4), (32, 2), (8, 32), (8, 24), (8, 16), (8, 2), (0, 2))   # (dense, min_children)
 RULE_DEFAULT = (32, 16)   # the library's default block rule
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_cache.json")
 
