@@ -23,7 +23,8 @@
 // current one multiplies).
 // Only tiles on or above the diagonal are computed; a mirror pass fills the lower
 // triangle. grid.y splits the transactions (split-N) and partial tiles are added with
-// integer atomics, which keeps the result bit-exact.
+// integer atomics, which keeps the result bit-exact. Workgroups are renumbered XCD-major
+// (sg_xcd_major), as in the fp4 kernel below.
 #include "sgfim_impl.h"
 #include "f4_phase.h"
 
@@ -54,6 +55,15 @@ __device__ __forceinline__ void sg_tri_pair(int q, int T, int& ti, int& tj)
     tj = i + q;
 }
 
+// Workgroup q of a one-segment grid of n: consecutive workgroup ids go to the eight
+// XCDs in turn, so q is renumbered XCD-major; the tiles of one transaction split then
+// share an XCD and its L2 (they read the same rows).
+__device__ __forceinline__ int sg_xcd_major(int q, int n)
+{
+    const int x = q & 7, k = q >> 3;
+    return x * (n >> 3) + min(x, n & 7) + k;
+}
+
 // C[I0.., J0..] += A^T B over words [wb, we) of one segment (bit rows, 0/1 widened in
 // registers). SEGS: the segment is segs[blockIdx.z], else `one`.
 template <bool SEGS, int TT>
@@ -68,12 +78,18 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
     const SgSegment sg = SEGS ? segs[blockIdx.z] : one;
     const int dpad = sg.dpad;
     const int T = (dpad + TT - 1) / TT;
-    if ((int)blockIdx.x >= T * (T + 1) / 2) return;
+    int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+    if (!SEGS) {
+        const int q = sg_xcd_major(bx + (int)gridDim.x * by, (int)(gridDim.x * gridDim.y));
+        bx = q % (int)gridDim.x;
+        by = q / (int)gridDim.x;
+    }
+    if (bx >= T * (T + 1) / 2) return;
     int ti, tj;
-    sg_tri_pair((int)blockIdx.x, T, ti, tj);
+    sg_tri_pair(bx, T, ti, tj);
     const int I0 = ti * TT, J0 = tj * TT;
     const int64_t per = ((sg.words + splits - 1) / splits + PT_K - 1) / PT_K * PT_K;
-    const int64_t wb = (int64_t)blockIdx.y * per;
+    const int64_t wb = (int64_t)by * per;
     const int64_t we = min(sg.words, wb + per);
     if (wb >= we) return;
     const uint64_t* rows = rows_all + sg.row_base;
@@ -185,15 +201,6 @@ __global__ __launch_bounds__(256) void k_pair_tile(const uint64_t* __restrict__ 
 #define PF_TILE 128           // output rows / columns of a workgroup tile (the kernel also takes 256)
 #define PF_SLOTS 512          // workgroups resident at once, two per CU (the grid fills whole rounds)
 #define PF_MAXW (1ll << 14)   // words per split (2^20 transactions; a multiple of PF_K)
-
-// Workgroup q of a one-segment grid of n: consecutive workgroup ids go to the eight
-// XCDs in turn, so q is renumbered XCD-major; the tiles of one transaction split then
-// share an XCD and its L2 (they read the same rows).
-__device__ __forceinline__ int sg_xcd_major(int q, int n)
-{
-    const int x = q & 7, k = q >> 3;
-    return x * (n >> 3) + min(x, n & 7) + k;
-}
 
 template <bool SEGS, int PK, int TT>
 __global__ __launch_bounds__(512) void k_pair_f4(const uint64_t* __restrict__ rows_all,
