@@ -932,10 +932,11 @@ __device__ __forceinline__ int sg_find_record(const int64_t* __restrict__ off, i
 // view. Units are (record, SG_PT_WORDS source words); each wave takes 256 of them as 4
 // steps of 64 coalesced words and ANDs X's rows the source has not applied. A gathered
 // block's list holds the set bits' positions (ascending within the wave), appended at a
-// position reserved with one atomic. A compacted block's list holds one entry per nonzero
-// word (SgPWord: the word, its mask and the block bit of its first transaction); one
-// atomic on the packed counter (entries << 40 | transactions) reserves the wave's entries
-// and bits together, so entries are in block-bit order. The entry holding the first bit of
+// position reserved with one atomic per workgroup. A compacted block's list holds one
+// entry per nonzero word (SgPWord: the word, its mask and the block bit of its first
+// transaction); one atomic per workgroup on the packed counter (entries << 40 |
+// transactions) reserves the entries and bits together, so entries are in block-bit
+// order. The entry holding the first bit of
 // each compress unit is recorded in ptid. Every row of the block is then built in this
 // list's order, so the order across waves is free.
 __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t* __restrict__ rows,
@@ -944,6 +945,7 @@ __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t
     __shared__ int32_t s_row[SG_MAX_DEPTH];
     __shared__ int s_g;
     __shared__ int64_t s_local;
+    __shared__ unsigned long long s_req[4], s_base[4];
     const int64_t nunits = S.ctl[CTL_PROJ_TUNITS];
     const int nb = (int)S.ctl[CTL_BATCH_N];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1007,9 +1009,12 @@ __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t
             ex[i] = tot + y - cnt;
             tot += __shfl(y, 63, 64);
         }
-        if (S.bpnew[g] == 2) {
-            int32_t ne[SG_PT_WORDS / 256], el[SG_PT_WORDS / 256];
-            int32_t etot = 0;
+        // one reservation per workgroup: the four waves' requests are added in LDS and
+        // thread 0 takes them with one atomic (wave order inside the workgroup's range)
+        const bool cmp = S.bpnew[g] == 2;
+        int32_t ne[SG_PT_WORDS / 256], el[SG_PT_WORDS / 256];
+        int32_t etot = 0;
+        if (cmp) {
 #pragma unroll
             for (int i = 0; i < SG_PT_WORDS / 256; i++) {
                 const uint64_t nz = __ballot(mv[i] != 0ull);
@@ -1017,10 +1022,21 @@ __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t
                 el[i] = __popcll(nz & ((1ull << lane) - 1ull));
                 etot += ne[i];
             }
-            uint64_t base = 0ull;
-            if (lane == 0 && tot)
-                base = atomicAdd((unsigned long long*)&S.pfill[g], ((unsigned long long)etot << 40) | (unsigned long long)tot);
-            base = __shfl(base, 0, 64);
+        }
+        if (lane == 0)
+            s_req[wv] = cmp ? ((unsigned long long)etot << 40) | (unsigned long long)tot : (unsigned long long)tot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long sum = s_req[0] + s_req[1] + s_req[2] + s_req[3];
+            unsigned long long b = sum ? atomicAdd((unsigned long long*)&S.pfill[g], sum) : 0ull;
+            for (int w = 0; w < 4; w++) {
+                s_base[w] = b;
+                b += s_req[w];
+            }
+        }
+        __syncthreads();
+        if (cmp) {
+            const uint64_t base = s_base[wv];
             const int64_t e0 = (int64_t)(base >> 40), b0 = (int64_t)(base & ((1ull << 40) - 1ull));
             const int64_t ecap = min((int64_t)m.sup, vwords);
             SgPWord* ent = S.pent + S.peoff[g];
@@ -1047,9 +1063,7 @@ __global__ __launch_bounds__(256) void k_proj_tids(SgSearchDev S, const uint64_t
             }
             continue;
         }
-        int64_t base = 0;
-        if (lane == 0 && tot) base = (int64_t)atomicAdd((unsigned long long*)&S.pfill[g], (unsigned long long)tot);
-        base = __shfl(base, 0, 64);
+        const int64_t base = (int64_t)s_base[wv];
         int32_t* out = S.ptid + S.ptoff[g];
         const int64_t lim = m.sup;
 #pragma unroll
