@@ -21,6 +21,7 @@ the exact MFI family instead (the Python miner's semantics, skab_spike.py:84-135
 """
 from __future__ import annotations
 
+import itertools
 import json
 import math
 import os
@@ -50,18 +51,14 @@ def _ctx() -> Context:
 # ---------------------------------------------------------------------------- FIMI I/O
 def write_fimi(txns, fimi_path, items_map_path=None):
     """Encode 'name=state' transactions as FIMI lines of 1-based ids (first-seen order),
-    each line the sorted distinct ids; returns {item_str: id}."""
+    each line the sorted distinct ids; returns {item_str: id}. The id map is written as
+    JSON (unescaped, indent 1) when a path is given, the form the pipeline reads back."""
     mapping = {}
-    out = []
-    for txn in txns:
-        ids = set()
-        for it in txn:
-            if it not in mapping:
-                mapping[it] = len(mapping) + 1
-            ids.add(mapping[it])
-        out.append(" ".join(str(i) for i in sorted(ids)))
+    # setdefault sizes the map before inserting, so a new item gets the next id
+    rows = [sorted({mapping.setdefault(it, len(mapping) + 1) for it in txn}) for txn in txns]
+    text = "\n".join(" ".join(map(str, r)) for r in rows) + "\n"
     with open(fimi_path, "w", encoding="utf-8") as f:
-        f.write("\n".join(out) + "\n")
+        f.write(text)
     if items_map_path:
         with open(items_map_path, "w", encoding="utf-8") as f:
             json.dump(mapping, f, ensure_ascii=False, indent=1)
@@ -202,39 +199,47 @@ def write_anyfim_results(path, fimi_path, stages, thresholds, runtimes, n_txn, p
         f.write(bar + "\n")
 
 
-def parse_tensorfim_results(path):
-    """{frozenset(item ids): frequency} of a TensorFIM Results file."""
-    out = {}
+def _mfi_line(line):
+    """(frozenset(ids), frequency) of one MFI line of a Results file, or None."""
+    hit = TF_LINE.match(line)
+    return (frozenset(map(int, hit.group(1).split())), int(hit.group(2))) if hit else None
+
+
+def _lines(path):
     with open(path, encoding="utf-8", errors="replace") as f:
-        for ln in f:
-            m = TF_LINE.match(ln.strip())
-            if m:
-                out[frozenset(int(x) for x in m.group(1).split())] = int(m.group(2))
-    return out
+        for raw in f:
+            yield raw.strip()
+
+
+def parse_tensorfim_results(path):
+    """{frozenset(item ids): frequency} of a TensorFIM Results file (a repeated set keeps
+    its last frequency)."""
+    return dict(filter(None, map(_mfi_line, _lines(path))))
+
+
+# Per-stage lists in an AnyFIM Results file: the label before the first ':' and the type of
+# the comma-separated values after it.
+_AF_META = {"runtimePerStage(s)": float, "supportThrePerStage": float, "frequencyThrePerStage": float,
+            "MFIsNumPerStage": int, "itemsNumPerStage": int, "dimensionPerStage": int,
+            "preprocessing_time(s)": float}
 
 
 def parse_anyfim_results(path):
-    """(stages, meta): stages[k] = [(frozenset(ids), frequency)], meta = per-stage lists."""
+    """(stages, meta): stages[k] = [(frozenset(ids), frequency)] in file order, meta[label]
+    = the label's per-stage list. MFI lines before the first stage header are ignored."""
     stages, meta, cur = {}, {}, None
-    keys = (("runtimePerStage(s):", float), ("supportThrePerStage:", float), ("frequencyThrePerStage:", float),
-            ("MFIsNumPerStage:", int), ("itemsNumPerStage:", int), ("dimensionPerStage:", int),
-            ("preprocessing_time(s):", float))
-    with open(path, encoding="utf-8", errors="replace") as f:
-        for ln in f:
-            ln = ln.strip()
-            m = AF_STAGE.match(ln)
-            if m:
-                cur = int(m.group(1))
-                stages[cur] = []
-                continue
-            for key, cast in keys:
-                if ln.startswith(key):
-                    body = ln[len(key):].strip()
-                    meta[key.rstrip(":")] = [cast(x.rstrip(",")) for x in body.split() if x.strip(",")]
-            if cur is not None:
-                m = TF_LINE.match(ln)
-                if m:
-                    stages[cur].append((frozenset(int(x) for x in m.group(1).split()), int(m.group(2))))
+    for line in _lines(path):
+        head = AF_STAGE.match(line)
+        if head:
+            cur = stages.setdefault(int(head.group(1)), [])
+            cur.clear()
+            continue
+        label, colon, rest = line.partition(":")
+        if colon and label in _AF_META:
+            meta[label] = [_AF_META[label](v) for v in rest.replace(",", " ").split()]
+        mfi = _mfi_line(line) if cur is not None else None
+        if mfi:
+            cur.append(mfi)
     return stages, meta
 
 
@@ -399,21 +404,19 @@ def run_anyfim(fimi_path, upto_stage, timeout=3600, exact=False, batched=False):
 
 def cumulative_mfis(stages):
     """{k: set of every itemset seen in stages <= k}."""
-    cum, seen = {}, set()
-    for k in sorted(stages):
-        for itemset, _ in stages[k]:
-            seen.add(itemset)
-        cum[k] = set(seen)
-    return cum
+    order = sorted(stages)
+    per_stage = (frozenset(x for x, _ in stages[k]) for k in order)
+    return {k: set(u) for k, u in zip(order, itertools.accumulate(per_stage, frozenset.union))}
 
 
 def support_confidence_lift(itemset, txns, labels):
     """(support, P(X | label 1), lift = P(X | 1) / P(X)) over integer transactions."""
     n = len(txns)
-    s = set(itemset)
-    cnt = sum(1 for t in txns if s <= set(t))
-    pos = sum(1 for t, lab in zip(txns, labels) if lab == 1 and s <= set(t))
-    npos = sum(1 for lab in labels if lab == 1)
+    x = frozenset(itemset)
+    covers = [x.issubset(t) for t in txns]   # X within transaction t
+    cnt = sum(covers)
+    npos = sum(lab == 1 for lab in labels)
+    pos = sum(c and lab == 1 for c, lab in zip(covers, labels))
     sup = cnt / n if n else 0.0
     conf = pos / npos if npos else 0.0
     lift = (pos / npos) / (cnt / n) if cnt and npos else 0.0
