@@ -527,7 +527,7 @@ __device__ __forceinline__ SgPlan sg_plan_record(const SgSearchDev& S, const SgS
         }
         // A record with one child gets no block of its own: the child can project itself
         // at no more cost when it is counted, and needs none if it is an MFI. A dense block
-        // (built word by word: ~50 lane operations per source word and row, against 4 for
+        // (built word by word: ~40 lane operations per source word and row, against 4 for
         // one count over that word) needs proj_mintail children to repay its build.
         const int64_t blk = (int64_t)m.ncand * sg_block_stride(m.sup);
         const bool dense = S.proj_dense > 0 && (int64_t)m.sup * S.proj_dense >= n_view;

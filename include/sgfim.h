@@ -205,7 +205,7 @@ int sgfim_get_projection(const sgfim_ctx* ctx, int32_t* factor, int64_t* arena_w
 
 /* How projected blocks of later searches are built. A block whose support is at least
  * 1/dense of its source's transactions (dense = 0: none) is compacted word by word: each
- * nonzero word of T(X) in the source view is bit-compressed in every row, about 50 lane
+ * nonzero word of T(X) in the source view is bit-compressed in every row, about 40 lane
  * operations per source word and row, which a record repays only with min_children or more
  * children (2..4096, default 16); sparser blocks are gathered bit by bit and need 2
  * children. Default dense = 32. No reference counterpart (see sgfim_set_projection).
