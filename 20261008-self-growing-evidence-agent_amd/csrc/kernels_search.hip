@@ -1299,7 +1299,9 @@ __global__ __launch_bounds__(64) void k_proj_compress(SgSearchDev S, const uint6
             uint64_t mk = act ? E.m : 0ull;
             const int c = __popcll(mk);
             const int64_t off = act ? (int64_t)E.p - start : 0;
-            const int hs = off < 0 ? (int)(-off) : 0;          // bits before the unit's start
+            // bits before the unit's start (fewer than the entry's c; capped so that a
+            // corrupt list cannot stretch the trim loop below: 64 clears any mask)
+            const int hs = off < 0 ? (int)min(-off, (int64_t)64) : 0;
             const int64_t pos = off < 0 ? 0 : off;
             const int keep = act ? (int)min((int64_t)(c - hs), end - start - pos) : 0;   // >= 1 on active lanes
             // trim the mask to the kept bits: drop its lowest hs and then its highest
